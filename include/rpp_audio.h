@@ -1,0 +1,233 @@
+/*
+ * rpp_audio.h -- C-ABI of the MI355X (gfx950) batched log-mel audio library.
+ *
+ * Drop-in for the GPU half of RPP's audio call chain: the handle API of
+ * api/rpp.h and the rppt_*_gpu entry points of api/rppt_tensor_audio_augmentations.h,
+ * with the same types, struct layouts, enum values and status codes as api/rppdefs.h.
+ * Every entry point is unmangled (extern "C"); arguments are plain pointers and sizes.
+ *
+ * Memory-space rules are RPP's (api/rppt_tensor_audio_augmentations.h):
+ *   src/dst tensors         device memory
+ *   per-clip arrays         pinned host memory or device memory (kernels read them);
+ *                           rppt_spectrogram_gpu additionally needs nothing on the host
+ *   windowFunction          host memory (copied to the device, cached per content)
+ *   RpptResamplingWindow    host struct; its lookupPinned table pinned or device memory
+ * All work is enqueued on the handle's stream; no entry point blocks the host except
+ * where noted.  Handles are not thread-safe (one handle per host thread).
+ */
+#ifndef RPP_AUDIO_MI355X_H
+#define RPP_AUDIO_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <vector>
+#include <xmmintrin.h>
+extern "C" {
+#endif
+
+/* ---------------- scalar typedefs (api/rppdefs.h:92-113) ---------------- */
+typedef unsigned char Rpp8u;
+typedef int Rpp32s;
+typedef unsigned int Rpp32u;
+typedef long long Rpp64s;
+typedef float Rpp32f;
+typedef double Rpp64f;
+typedef void *RppPtr_t;
+typedef size_t RppSize_t;
+
+/* ---------------- status codes (api/rppdefs.h:117-176, :190-201) ---------------- */
+typedef enum {
+    RPP_SUCCESS = 0,
+    RPP_ERROR = -1,
+    RPP_ERROR_INVALID_ARGUMENTS = -2,
+    RPP_ERROR_LOW_OFFSET = -3,
+    RPP_ERROR_ZERO_DIVISION = -4,
+    RPP_ERROR_HIGH_SRC_DIMENSION = -5,
+    RPP_ERROR_NOT_IMPLEMENTED = -6,
+    RPP_ERROR_INVALID_SRC_CHANNELS = -7,
+    RPP_ERROR_INVALID_DST_CHANNELS = -8,
+    RPP_ERROR_INVALID_SRC_LAYOUT = -9,
+    RPP_ERROR_INVALID_DST_LAYOUT = -10,
+    RPP_ERROR_INVALID_SRC_DATATYPE = -11,
+    RPP_ERROR_INVALID_DST_DATATYPE = -12,
+    RPP_ERROR_INVALID_SRC_OR_DST_DATATYPE = -13,
+    RPP_ERROR_INSUFFICIENT_DST_BUFFER_LENGTH = -14,
+    RPP_ERROR_INVALID_PARAMETER_DATATYPE = -15,
+    RPP_ERROR_NOT_ENOUGH_MEMORY = -16,
+    RPP_ERROR_OUT_OF_BOUND_SRC_ROI = -17,
+    RPP_ERROR_LAYOUT_MISMATCH = -18,
+    RPP_ERROR_INVALID_CHANNELS = -19,
+    RPP_ERROR_INVALID_OUTPUT_TILE_LENGTH = -20,
+    RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE = -21,
+    RPP_ERROR_OUT_OF_BOUND_SCRATCH_MEMORY_SIZE = -22,
+    RPP_ERROR_INVALID_SRC_DIMS = -23,
+    RPP_ERROR_INVALID_DST_DIMS = -24,
+    RPP_ERROR_INVALID_DIM_LENGTHS = -25,
+    RPP_ERROR_INVALID_AXIS = -26
+} RppStatus;
+
+typedef enum {
+    rppStatusSuccess = 0,
+    rppStatusBadParm = -1,
+    rppStatusUnknownError = -2,
+    rppStatusNotInitialized = -3,
+    rppStatusInvalidValue = -4,
+    rppStatusAllocFailed = -5,
+    rppStatusInternalError = -6,
+    rppStatusNotImplemented = -7,
+    rppStatusUnsupportedOp = -8
+} rppStatus_t;
+
+/* ---------------- enums (api/rppdefs.h:178-186, :398-419, :464-478) ---------------- */
+typedef enum { RPP_HOST_BACKEND, RPP_HIP_BACKEND, RPP_OCL_BACKEND } RppBackend;
+typedef enum { U8, F32, F16, I8, I16 } RpptDataType;
+typedef enum { NCHW, NHWC, NCDHW, NDHWC, NHW, NFT, NTF } RpptLayout;
+typedef enum { ZERO = 0, CLAMP, REFLECT } RpptAudioBorderType;
+typedef enum { SLANEY = 0, HTK } RpptMelScaleFormula;
+
+/* ---------------- descriptors (api/rppdefs.h:570-589, :645-649) ---------------- */
+typedef struct {
+    Rpp32u nStride;
+    Rpp32u cStride;
+    Rpp32u hStride;
+    Rpp32u wStride;
+} RpptStrides;
+
+typedef struct {
+    RppSize_t numDims;
+    Rpp32u offsetInBytes;
+    RpptDataType dataType;
+    Rpp32u n, c, h, w;
+    RpptStrides strides;
+    RpptLayout layout;
+} RpptDesc, *RpptDescPtr;
+
+typedef struct {
+    Rpp32u width;
+    Rpp32u height;
+} RpptImagePatch, *RpptImagePatchPtr;
+
+/* opaque handle (api/rpp.h:64-78) */
+typedef struct rppHandle *rppHandle_t;
+typedef rppHandle_t RppHandle_t;
+
+#ifdef __cplusplus
+}  /* extern "C" */
+
+/* Resampling window (api/rppdefs.h:751-790).  Same member order and layout (96 bytes on
+ * x86-64); the GPU entry point reads scale, center, lobes, lookupSize and lookupPinned. */
+typedef struct RpptResamplingWindow {
+    Rpp32f scale = 1, center = 1;
+    Rpp32s lobes = 0, coeffs = 0;
+    Rpp32s lookupSize = 0;
+    Rpp32f *lookupPinned = nullptr;
+    std::vector<Rpp32f> lookup;
+    __m128 pCenter, pScale;
+} RpptResamplingWindow;
+
+extern "C" {
+#endif
+
+/* ======================= handle API (api/rpp.h:132-244) ======================= */
+
+/* rppCreate: api/rpp.h:132, src/modules/handle_api.cpp:29-44.  Only RPP_HIP_BACKEND is
+ * served here; stream may be NULL (null stream) or a hipStream_t.  numThreads is kept
+ * for API parity and unused on the GPU. */
+rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u numThreads, void *stream, RppBackend backend);
+/* rppDestroy: api/rpp.h:150, handle_api.cpp:46-60 */
+rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend);
+/* rppSetBatchSize / rppGetBatchSize: api/rpp.h:168/:186, handle_api.cpp:62-72 */
+rppStatus_t rppSetBatchSize(rppHandle_t handle, size_t batchSize);
+rppStatus_t rppGetBatchSize(rppHandle_t handle, size_t *batchSize);
+/* rppSetStream / rppGetStream: api/rpp.h:206/:224, handle_api.cpp:74-83 (hipStream_t) */
+rppStatus_t rppSetStream(rppHandle_t handle, void *streamID);
+rppStatus_t rppGetStream(rppHandle_t handle, void **streamID);
+
+/* ======================= audio entry points ======================= */
+
+/* api/rppt_tensor_audio_augmentations.h:156; dispatch rppt_tensor_audio_augmentations.cpp:452-484.
+ * y[0] = x[0] - c*border, y[t] = x[t] - c*x[t-1] over [0, srcLengthTensor[b]). */
+RppStatus rppt_pre_emphasis_filter_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                       RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor, Rpp32f *coeffTensor,
+                                       RpptAudioBorderType borderType, rppHandle_t rppHandle);
+
+/* api/rppt_tensor_audio_augmentations.h:232; dispatch rppt_tensor_audio_augmentations.cpp:579-629.
+ * Power (power == 2: |X|^2, else |X|) spectrogram of centered/reflect-padded Hann frames,
+ * NFT (dst[k*hStride + f]) or NTF (dst[f*hStride + k]).  FP32 arithmetic. */
+RppStatus rppt_spectrogram_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                               Rpp32s *srcLengthTensor, bool centerWindows, bool reflectPadding,
+                               Rpp32f *windowFunction, Rpp32s nfft, Rpp32s power, Rpp32s windowLength,
+                               Rpp32s windowStep, rppHandle_t rppHandle);
+
+/* api/rppt_tensor_audio_augmentations.h:274; dispatch rppt_tensor_audio_augmentations.cpp:488-535.
+ * srcDims[2b] = numBins, srcDims[2b+1] = frames; NFT in, NFT out. */
+RppStatus rppt_mel_filter_bank_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                   Rpp32s *srcDims, Rpp32f maxFreq, Rpp32f minFreq, RpptMelScaleFormula melFormula,
+                                   Rpp32s numFilter, Rpp32f sampleRate, bool normalize, rppHandle_t rppHandle);
+
+/* api/rppt_tensor_audio_augmentations.h:122; dispatch rppt_tensor_audio_augmentations.cpp:373-412.
+ * out = multiplier*log10(max(minRatio, x/ref)); ref = referenceMagnitude or, if 0, the clip max. */
+RppStatus rppt_to_decibels_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                               RpptImagePatchPtr srcDims, Rpp32f cutOffDB, Rpp32f multiplier,
+                               Rpp32f referenceMagnitude, rppHandle_t rppHandle);
+
+/* api/rppt_tensor_audio_augmentations.h:188; dispatch rppt_tensor_audio_augmentations.cpp:416-448 (HOST rules :156-186).
+ * srcDimsTensor[2b] = samples, [2b+1] = channels (interleaved input). */
+RppStatus rppt_down_mixing_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                               Rpp32s *srcDimsTensor, bool normalizeWeights, rppHandle_t rppHandle);
+
+#ifdef __cplusplus
+/* api/rppt_tensor_audio_augmentations.h:310; dispatch rppt_tensor_audio_augmentations.cpp:539-575.
+ * Windowed-sinc resampling with the caller's lookup table. */
+RppStatus rppt_resample_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                            Rpp32f *inRateTensor, Rpp32f *outRateTensor, Rpp32s *srcDimsTensor,
+                            RpptResamplingWindow &window, rppHandle_t rppHandle);
+#endif
+
+/* ======================= extensions (no counterpart in the reference) ======================= */
+
+/* Arithmetic of the fused DFT / mel contraction. */
+typedef enum {
+    RPPT_LOGMEL_FP32 = 0,      /* FP32 vector FMA throughout */
+    RPPT_LOGMEL_BF16X3 = 1,    /* MFMA bf16, split hi+lo: hi*hi + hi*lo + lo*hi, FP32 accumulate */
+    RPPT_LOGMEL_FP16X3 = 2,    /* MFMA fp16, split hi+lo with power-of-two scaling, FP32 accumulate */
+    RPPT_LOGMEL_BF16 = 3       /* MFMA bf16, single product (fast, reported error only) */
+} RpptLogMelPrecision;
+
+/* Parameters of the whole log-mel chain, the union of the four reference calls' arguments. */
+typedef struct {
+    Rpp32f preEmphasisCoeff;        /* rppt_pre_emphasis_filter coeff (same for every clip) */
+    RpptAudioBorderType borderType; /* pre-emphasis border */
+    Rpp32s nfft, windowLength, windowStep;
+    Rpp32s centerWindows, reflectPadding;
+    Rpp32s power;                   /* 2: power, else magnitude */
+    Rpp32f sampleRate, minFreq, maxFreq;
+    Rpp32s numFilter;
+    RpptMelScaleFormula melFormula;
+    Rpp32s normalize;
+    Rpp32f cutOffDB, multiplier, referenceMagnitude;  /* referenceMagnitude must be > 0 */
+} RpptLogMelParams;
+
+/* Whole chain pre-emphasis -> spectrogram -> mel -> to_decibels over a ragged batch in one
+ * persistent launch.  src [n][nStride] f32 (device), srcLengthTensor device or pinned,
+ * dst NFT [n][numFilter][hStride] f32 with hStride >= frames of every clip.  Equivalent to
+ * the four rppt_*_gpu calls with the matching arguments. */
+RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
+                                RpptLogMelPrecision precision, rppHandle_t rppHandle);
+
+/* Tile shape of the fused launch; 0 fields mean "autotuned / default". */
+typedef struct {
+    Rpp32s framesPerTile;   /* frames per work item: 32, 64 or 128 */
+    Rpp32s blocksPerCU;     /* persistent blocks per compute unit */
+} RpptLogMelTiles;
+RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);
+RppStatus rppt_logmel_get_tiles(rppHandle_t rppHandle, RpptLogMelTiles *tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RPP_AUDIO_MI355X_H */

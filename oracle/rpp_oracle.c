@@ -1,0 +1,740 @@
+/*
+ * rpp_oracle.c -- CPU oracle for the batched log-mel audio path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py load it; the product
+ * library (rpp_amd/csrc) never links, calls or falls back to it.
+ *
+ * Two families of functions:
+ *   orc_*_f32  -- a restatement of RPP's HOST (C++/AVX2/OpenMP) path in plain C,
+ *                 same formulas, same FP32 rounding points, same accumulation
+ *                 order, OpenMP over clips exactly like the reference.  The
+ *                 reference's FFT is the third-party FFTS library (prebuilt,
+ *                 no source, no pinned version); its published algorithm is a
+ *                 radix-2^k real FFT, restated here as a project-written split
+ *                 real FFT (radix-2 complex FFT of N/2 points + post-twiddle).
+ *   orc_*_f64  -- an FP64 oracle of the same mathematics (double everywhere,
+ *                 double mel scale), the "truth" used for dB error budgets.
+ *
+ * Built with -ffp-contract=off: the reference builds its audio kernels with
+ * -mno-fma (src/modules/CMakeLists.txt:105-124), so a*b+c is rounded twice.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+enum { ORC_BORDER_ZERO = 0, ORC_BORDER_CLAMP = 1, ORC_BORDER_REFLECT = 2 };
+enum { ORC_MEL_SLANEY = 0, ORC_MEL_HTK = 1 };
+
+static int orc_threads(int n) { return n > 0 ? n : 1; }
+
+/* ------------------------------------------------------------------ */
+/* pre-emphasis  (cpu/kernel/pre_emphasis_filter.cpp:27-75)            */
+/* y[0] = x[0] - c*border ; y[t] = x[t] - c*x[t-1]                      */
+/* ------------------------------------------------------------------ */
+ORC_API void orc_pre_emphasis_f32(const float *src, long srcStride, float *dst, long dstStride,
+                                  const int *len, const float *coeff, int B, int border, int nthreads)
+{
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        float *y = dst + (long)b * dstStride;
+        int L = len[b];
+        float c = coeff[b];
+        if (L <= 0) continue;
+        /* border read: CLAMP -> x[0], REFLECT -> x[1], ZERO -> 0 (:46-48) */
+        float bval = (border == ORC_BORDER_CLAMP) ? x[0] : (border == ORC_BORDER_REFLECT) ? x[1] : 0.0f;
+        float p = c * bval;
+        y[0] = x[0] - p;
+        for (int t = 1; t < L; t++) {
+            float q = c * x[t - 1];
+            y[t] = x[t] - q;
+        }
+    }
+}
+
+ORC_API void orc_pre_emphasis_f64(const float *src, long srcStride, double *dst, long dstStride,
+                                  const int *len, const float *coeff, int B, int border)
+{
+    for (int b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        double *y = dst + (long)b * dstStride;
+        int L = len[b];
+        double c = coeff[b];
+        if (L <= 0) continue;
+        double bval = (border == ORC_BORDER_CLAMP) ? x[0] : (border == ORC_BORDER_REFLECT) ? x[1] : 0.0;
+        y[0] = (double)x[0] - c * bval;
+        for (int t = 1; t < L; t++) y[t] = (double)x[t] - c * (double)x[t - 1];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* window / framing  (cpu/kernel/spectrogram.cpp:36-69)                */
+/* ------------------------------------------------------------------ */
+/* Hann with a half-sample shift: w[t] = 0.5*(1 - cos(2*pi*(t+0.5)/W)), FP64 -> FP32 (:36-44) */
+ORC_API void orc_hann_f32(int W, float *out)
+{
+    double a = (2.0 * M_PI) / W;
+    for (int t = 0; t < W; t++) out[t] = (float)(0.5 * (1.0 - cos(a * (t + 0.5))));
+}
+
+ORC_API void orc_hann_f64(int W, double *out)
+{
+    double a = (2.0 * M_PI) / W;
+    for (int t = 0; t < W; t++) out[t] = 0.5 * (1.0 - cos(a * (t + 0.5)));
+}
+
+/* F = L/H + 1 (centered) or (L-W)/H + 1 (:47-52) */
+ORC_API int orc_num_windows(int L, int W, int H, int center)
+{
+    if (!center) L -= W;
+    return L / H + 1;
+}
+
+/* mirror without repeating the edge sample, iterated; ranges shorter than 2 map to hi-1 (:55-69) */
+ORC_API int orc_idx_reflect(int loc, int lo, int hi)
+{
+    if (hi - lo < 2) return hi - 1;
+    for (;;) {
+        if (loc < lo) loc = 2 * lo - loc;
+        else if (loc >= hi) loc = 2 * hi - 2 - loc;
+        else break;
+    }
+    return loc;
+}
+
+/* ------------------------------------------------------------------ */
+/* split real FFT: N real -> N/2+1 complex, generated for float/double */
+/* ------------------------------------------------------------------ */
+#define ORC_DEFINE_RFFT(T, SFX)                                                              \
+typedef struct {                                                                           \
+    int n, m;            /* n real points, m = n/2 complex points */                      \
+    int *rev;            /* bit reversal of m */                                          \
+    T *wr, *wi;          /* twiddles of the m-point complex FFT, exp(-2 pi i j/m) */      \
+    T *pr, *pi;          /* post twiddles exp(-2 pi i k/n), k < m */                      \
+} rfft_##SFX;                                                                              \
+static rfft_##SFX *rfft_plan_##SFX(int n)                                                  \
+{                                                                                          \
+    rfft_##SFX *p = (rfft_##SFX *)calloc(1, sizeof(*p));                                   \
+    p->n = n; p->m = n / 2;                                                                \
+    int m = p->m, lg = 0;                                                                  \
+    while ((1 << lg) < m) lg++;                                                            \
+    p->rev = (int *)malloc(sizeof(int) * m);                                               \
+    for (int i = 0; i < m; i++) {                                                          \
+        int r = 0;                                                                         \
+        for (int b = 0; b < lg; b++) if (i & (1 << b)) r |= 1 << (lg - 1 - b);             \
+        p->rev[i] = r;                                                                     \
+    }                                                                                      \
+    p->wr = (T *)malloc(sizeof(T) * m); p->wi = (T *)malloc(sizeof(T) * m);                \
+    p->pr = (T *)malloc(sizeof(T) * m); p->pi = (T *)malloc(sizeof(T) * m);                \
+    for (int j = 0; j < m; j++) {                                                          \
+        double a = -2.0 * M_PI * j / m;                                                    \
+        p->wr[j] = (T)cos(a); p->wi[j] = (T)sin(a);                                        \
+        double b2 = -2.0 * M_PI * j / n;                                                   \
+        p->pr[j] = (T)cos(b2); p->pi[j] = (T)sin(b2);                                      \
+    }                                                                                      \
+    return p;                                                                              \
+}                                                                                          \
+static void rfft_free_##SFX(rfft_##SFX *p)                                                 \
+{                                                                                          \
+    free(p->rev); free(p->wr); free(p->wi); free(p->pr); free(p->pi); free(p);             \
+}                                                                                          \
+/* in: n reals; out: re[0..m], im[0..m]; scratch zr, zi of m */                            \
+static void rfft_exec_##SFX(const rfft_##SFX *p, const T *in, T *re, T *im, T *zr, T *zi)  \
+{                                                                                          \
+    int m = p->m;                                                                          \
+    for (int i = 0; i < m; i++) { int r = p->rev[i]; zr[r] = in[2 * i]; zi[r] = in[2 * i + 1]; } \
+    for (int len = 2; len <= m; len <<= 1) {                                               \
+        int half = len >> 1, step = m / len;                                               \
+        for (int s = 0; s < m; s += len) {                                                 \
+            for (int j = 0; j < half; j++) {                                               \
+                T wr = p->wr[j * step], wi = p->wi[j * step];                              \
+                int a = s + j, b = a + half;                                               \
+                T tr = zr[b] * wr - zi[b] * wi;                                            \
+                T ti = zr[b] * wi + zi[b] * wr;                                            \
+                zr[b] = zr[a] - tr; zi[b] = zi[a] - ti;                                    \
+                zr[a] = zr[a] + tr; zi[a] = zi[a] + ti;                                    \
+            }                                                                              \
+        }                                                                                  \
+    }                                                                                      \
+    /* X[k] = E[k] + W^k O[k]; E = (Z[k]+conj Z[m-k])/2, O = (Z[k]-conj Z[m-k])/(2i) */     \
+    re[0] = zr[0] + zi[0]; im[0] = 0;                                                      \
+    re[m] = zr[0] - zi[0]; im[m] = 0;                                                      \
+    for (int k = 1; k < m; k++) {                                                          \
+        T ar = zr[k], ai = zi[k], br = zr[m - k], bi = -zi[m - k];                         \
+        T er = (T)0.5 * (ar + br), ei = (T)0.5 * (ai + bi);                                \
+        T orr = (T)0.5 * (ai - bi), oi = (T)-0.5 * (ar - br);                              \
+        T wr = p->pr[k], wi = p->pi[k];                                                    \
+        re[k] = er + (orr * wr - oi * wi);                                                 \
+        im[k] = ei + (orr * wi + oi * wr);                                                 \
+    }                                                                                      \
+}
+
+ORC_DEFINE_RFFT(float, f32)
+ORC_DEFINE_RFFT(double, f64)
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* ------------------------------------------------------------------ */
+/* spectrogram  (cpu/kernel/spectrogram.cpp:71-251)                    */
+/* ------------------------------------------------------------------ */
+/* dst layout: NFT -> dst[k*hStride + f]; NTF -> dst[f*hStride + k]      */
+ORC_API void orc_spectrogram_f32(const float *src, long srcStride, const int *len, int B,
+                                 float *dst, long dstNStride, long hStride, int layoutNFT,
+                                 int center, int reflect, const float *window, int nfft, int power,
+                                 int W, int H, int nthreads)
+{
+    if (nfft == 0) nfft = W;
+    const int K = nfft / 2 + 1;
+    const int off = center ? W / 2 : 0;
+    float *win = (float *)malloc(sizeof(float) * W);
+    if (window) memcpy(win, window, sizeof(float) * W);
+    else orc_hann_f32(W, win);
+    const int real_impl = is_pow2(nfft) && nfft >= 4;
+    /* the frame sits at offset (N-W)/2 of a zeroed N buffer (:197-212); it only rotates phase */
+    const int inStart = W < nfft ? (nfft - W) / 2 : 0;
+    const int Wc = W < nfft ? W : nfft;
+    float *cosT = NULL, *sinT = NULL;
+    if (!real_impl) {
+        cosT = (float *)malloc(sizeof(float) * nfft);
+        sinT = (float *)malloc(sizeof(float) * nfft);
+        for (int j = 0; j < nfft; j++) {
+            cosT[j] = (float)cos(2.0 * M_PI * j / nfft);
+            sinT[j] = (float)sin(2.0 * M_PI * j / nfft);
+        }
+    }
+    rfft_f32 *plan = real_impl ? rfft_plan_f32(nfft) : NULL;
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        float *y = dst + (long)b * dstNStride;
+        const int L = len[b];
+        const int F = orc_num_windows(L, W, H, center);
+        float *buf = (float *)calloc((size_t)nfft, sizeof(float));
+        float *frame = (float *)malloc(sizeof(float) * W);
+        float *re = (float *)malloc(sizeof(float) * (nfft + 2));
+        float *im = (float *)malloc(sizeof(float) * (nfft + 2));
+        float *zr = (float *)malloc(sizeof(float) * (nfft + 2));
+        float *zi = (float *)malloc(sizeof(float) * (nfft + 2));
+        for (int f = 0; f < F; f++) {
+            int s = f * H - off;
+            if (s < 0 || s + W > L) {
+                for (int t = 0; t < W; t++) {
+                    int i = s + t;
+                    if (reflect) frame[t] = win[t] * x[orc_idx_reflect(i, 0, L)];
+                    else frame[t] = (i >= 0 && i < L) ? win[t] * x[i] : 0.0f;
+                }
+            } else {
+                for (int t = 0; t < W; t++) frame[t] = win[t] * x[s + t];
+            }
+            memset(buf, 0, sizeof(float) * nfft);
+            for (int t = 0; t < Wc; t++) buf[inStart + t] = frame[t];
+            if (real_impl) {
+                rfft_exec_f32(plan, buf, re, im, zr, zi);
+            } else {
+                for (int k = 0; k < K; k++) {
+                    float sr = 0, si = 0;
+                    for (int t = 0; t < nfft; t++) {
+                        int j = (int)(((long)k * t) % nfft);
+                        sr += buf[t] * cosT[j];
+                        si -= buf[t] * sinT[j];
+                    }
+                    re[k] = sr; im[k] = si;
+                }
+            }
+            /* power == 1 -> |X|, any other value -> re^2 + im^2 (:217-242) */
+            for (int k = 0; k < K; k++) {
+                float v = (power == 1) ? sqrtf(re[k] * re[k] + im[k] * im[k])
+                                       : re[k] * re[k] + im[k] * im[k];
+                if (layoutNFT) y[(long)k * hStride + f] = v;
+                else y[(long)f * hStride + k] = v;
+            }
+        }
+        free(buf); free(frame); free(re); free(im); free(zr); free(zi);
+    }
+    if (plan) rfft_free_f32(plan);
+    free(cosT); free(sinT); free(win);
+}
+
+/* FP64 oracle spectrogram: input is the (already pre-emphasised) signal in double. */
+ORC_API void orc_spectrogram_f64(const double *src, long srcStride, const int *len, int B,
+                                 double *dst, long dstNStride, long hStride, int layoutNFT,
+                                 int center, int reflect, const double *window, int nfft, int power,
+                                 int W, int H, int nthreads)
+{
+    if (nfft == 0) nfft = W;
+    const int K = nfft / 2 + 1;
+    const int off = center ? W / 2 : 0;
+    double *win = (double *)malloc(sizeof(double) * W);
+    if (window) memcpy(win, window, sizeof(double) * W);
+    else orc_hann_f64(W, win);
+    const int real_impl = is_pow2(nfft) && nfft >= 4;
+    const int Wc = W < nfft ? W : nfft;
+    rfft_f64 *plan = real_impl ? rfft_plan_f64(nfft) : NULL;
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const double *x = src + (long)b * srcStride;
+        double *y = dst + (long)b * dstNStride;
+        const int L = len[b];
+        const int F = orc_num_windows(L, W, H, center);
+        double *buf = (double *)calloc((size_t)nfft, sizeof(double));
+        double *re = (double *)malloc(sizeof(double) * (nfft + 2));
+        double *im = (double *)malloc(sizeof(double) * (nfft + 2));
+        double *zr = (double *)malloc(sizeof(double) * (nfft + 2));
+        double *zi = (double *)malloc(sizeof(double) * (nfft + 2));
+        for (int f = 0; f < F; f++) {
+            int s = f * H - off;
+            memset(buf, 0, sizeof(double) * nfft);
+            for (int t = 0; t < Wc; t++) {
+                int i = s + t;
+                double v;
+                if (s < 0 || s + W > L) {
+                    if (reflect) v = win[t] * x[orc_idx_reflect(i, 0, L)];
+                    else v = (i >= 0 && i < L) ? win[t] * x[i] : 0.0;
+                } else v = win[t] * x[i];
+                buf[t] = v;   /* phase origin is irrelevant for |X| */
+            }
+            if (real_impl) rfft_exec_f64(plan, buf, re, im, zr, zi);
+            else {
+                for (int k = 0; k < K; k++) {
+                    double sr = 0, si = 0;
+                    for (int t = 0; t < nfft; t++) {
+                        double a = 2.0 * M_PI * (double)(((long)k * t) % nfft) / nfft;
+                        sr += buf[t] * cos(a);
+                        si -= buf[t] * sin(a);
+                    }
+                    re[k] = sr; im[k] = si;
+                }
+            }
+            for (int k = 0; k < K; k++) {
+                double p2 = re[k] * re[k] + im[k] * im[k];
+                double v = (power == 1) ? sqrt(p2) : p2;
+                if (layoutNFT) y[(long)k * hStride + f] = v;
+                else y[(long)f * hStride + k] = v;
+            }
+        }
+        free(buf); free(re); free(im); free(zr); free(zi);
+    }
+    if (plan) rfft_free_f64(plan);
+    free(win);
+}
+
+/* ------------------------------------------------------------------ */
+/* mel scales  (api/rppdefs.h:795-851) -- FP32 member functions         */
+/* ------------------------------------------------------------------ */
+static const float SL_FSP = 66.666667f;
+static const float SL_MIN_LOG_HZ = 1000.0f;
+static const float SL_STEP_LOG = 0.068751777f;
+static const float SL_INV_MIN_LOG_HZ = 0.001f;
+
+static float slaney_min_log_mel(void) { return (SL_MIN_LOG_HZ - 0.0f) / SL_FSP; }
+static float slaney_hz_to_mel_f(float hz)
+{
+    const float invStepLog = 1.0f / SL_STEP_LOG, invFsp = 1.0f / SL_FSP;
+    if (hz >= SL_MIN_LOG_HZ) return slaney_min_log_mel() + logf(hz * SL_INV_MIN_LOG_HZ) * invStepLog;
+    return (hz - 0.0f) * invFsp;
+}
+static float slaney_mel_to_hz_f(float mel)
+{
+    if (mel >= slaney_min_log_mel()) return SL_MIN_LOG_HZ * expf(SL_STEP_LOG * (mel - slaney_min_log_mel()));
+    return 0.0f + mel * SL_FSP;
+}
+static float htk_hz_to_mel_f(float hz) { return 1127.0f * logf(1.0f + (hz / 700.0f)); }
+static float htk_mel_to_hz_f(float mel) { return 700.0f * (expf(mel / 1127.0f) - 1.0f); }
+
+static float hz_to_mel_f(int formula, float hz) { return formula == ORC_MEL_HTK ? htk_hz_to_mel_f(hz) : slaney_hz_to_mel_f(hz); }
+static float mel_to_hz_f(int formula, float mel) { return formula == ORC_MEL_HTK ? htk_mel_to_hz_f(mel) : slaney_mel_to_hz_f(mel); }
+
+/* double-precision mel scale for the FP64 oracle (same formulas, exact constants) */
+static double hz_to_mel_d(int formula, double hz)
+{
+    if (formula == ORC_MEL_HTK) return 1127.0 * log(1.0 + hz / 700.0);
+    const double fsp = 200.0 / 3.0, minLogHz = 1000.0, minLogMel = minLogHz / fsp, stepLog = log(6.4) / 27.0;
+    return hz >= minLogHz ? minLogMel + log(hz / minLogHz) / stepLog : hz / fsp;
+}
+static double mel_to_hz_d(int formula, double mel)
+{
+    if (formula == ORC_MEL_HTK) return 700.0 * (exp(mel / 1127.0) - 1.0);
+    const double fsp = 200.0 / 3.0, minLogHz = 1000.0, minLogMel = minLogHz / fsp, stepLog = log(6.4) / 27.0;
+    return mel >= minLogMel ? minLogHz * exp(stepLog * (mel - minLogMel)) : mel * fsp;
+}
+
+/*
+ * Mel tables as the HOST path builds them (cpu/kernel/mel_filter_bank.cpp:55-112):
+ * weightsDown[k] (f32), intervals[k] (filter index whose up-slope bin k is on, -1 if unused),
+ * normFactors[m] (f32, 1 unless normalize), and the bin range [binStart, binEnd).
+ */
+ORC_API void orc_mel_tables_f32(int numBins, float sampleRate, float maxFreqVal, float minFreq,
+                                int formula, int M, int normalize,
+                                float *weightsDown, int *intervals, float *normFactors, int *binRange)
+{
+    float maxFreq = (maxFreqVal == 0) ? sampleRate / 2 : maxFreqVal;
+    double melLow = hz_to_mel_f(formula, minFreq);
+    double melHigh = hz_to_mel_f(formula, maxFreq);
+    double melStep = (melHigh - melLow) / (M + 1);
+    int nfft = (numBins - 1) * 2;
+    double hzStep = (double)sampleRate / nfft;
+    double invHzStep = 1.0 / hzStep;
+    int binStart = (int)ceil(minFreq * invHzStep);
+    int binEnd = (int)ceil(maxFreq * invHzStep);
+    if (binEnd > numBins) binEnd = numBins;
+    for (int m = 0; m < M; m++) normFactors[m] = 1.0f;
+    for (int k = 0; k < numBins; k++) { weightsDown[k] = 0.0f; intervals[k] = -1; }
+    int bin = binStart;
+    double mel0 = melLow, mel1 = melLow + melStep;
+    double fIter = bin * hzStep;
+    for (int iv = 0; iv < M + 1; iv++, mel0 = mel1, mel1 += melStep) {
+        double f0 = mel_to_hz_f(formula, (float)mel0);
+        double f1 = mel_to_hz_f(formula, (float)(iv == M ? melHigh : mel1));
+        double slope = 1. / (f1 - f0);
+        if (normalize && iv < M) {
+            double f2 = mel_to_hz_f(formula, (float)(mel1 + melStep));
+            normFactors[iv] = (float)(2.0 / (f2 - f0));
+        }
+        for (; bin < binEnd && fIter < f1; bin++, fIter = bin * hzStep) {
+            weightsDown[bin] = (float)((f1 - fIter) * slope);
+            intervals[bin] = iv;
+        }
+    }
+    binRange[0] = binStart;
+    binRange[1] = binEnd;
+}
+
+/*
+ * FP32 mel filter bank (cpu/kernel/mel_filter_bank.cpp:114-193).  srcDims[2b] = numBins,
+ * srcDims[2b+1] = frames.  Each output element accumulates, bin by ascending bin, w*src
+ * rounded then added (no FMA), starting from 0.
+ */
+ORC_API void orc_mel_f32(const float *src, long srcNStride, long srcHStride, const int *srcDims, int B,
+                         float *dst, long dstNStride, long dstHStride,
+                         float maxFreqVal, float minFreq, int formula, int M, float sampleRate,
+                         int normalize, int nthreads)
+{
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const int numBins = srcDims[2 * b];
+        const int F = srcDims[2 * b + 1];
+        float *wd = (float *)malloc(sizeof(float) * numBins);
+        int *iv = (int *)malloc(sizeof(int) * numBins);
+        float *nf = (float *)malloc(sizeof(float) * M);
+        int range[2];
+        orc_mel_tables_f32(numBins, sampleRate, maxFreqVal, minFreq, formula, M, normalize, wd, iv, nf, range);
+        const float *xs = src + (long)b * srcNStride;
+        float *y = dst + (long)b * dstNStride;
+        long zeroCols = F + 8 < dstHStride ? F + 8 : dstHStride;
+        for (int m = 0; m < M; m++)
+            for (long f = 0; f < zeroCols; f++) y[m * dstHStride + f] = 0.0f;
+        for (int k = range[0]; k < range[1]; k++) {
+            const float *row = xs + (long)k * srcHStride;
+            int up = iv[k];
+            int down = up - 1;
+            float wDown = wd[k];
+            float wUp = 1.0f - wd[k];
+            if (down >= 0) {
+                float w = normalize ? wDown * nf[down] : wDown;
+                float *o = y + (long)down * dstHStride;
+                for (int f = 0; f < F; f++) { float p = row[f] * w; o[f] = o[f] + p; }
+            }
+            if (up >= 0 && up < M) {
+                float w = normalize ? wUp * nf[up] : wUp;
+                float *o = y + (long)up * dstHStride;
+                for (int f = 0; f < F; f++) { float p = row[f] * w; o[f] = o[f] + p; }
+            }
+        }
+        free(wd); free(iv); free(nf);
+    }
+}
+
+/* FP64 triangular mel filter bank from the double mel scale (same filter construction). */
+ORC_API void orc_mel_f64(const double *src, long srcNStride, long srcHStride, const int *srcDims, int B,
+                         double *dst, long dstNStride, long dstHStride,
+                         double maxFreqVal, double minFreq, int formula, int M, double sampleRate,
+                         int normalize, int nthreads)
+{
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const int numBins = srcDims[2 * b];
+        const int F = srcDims[2 * b + 1];
+        double maxFreq = maxFreqVal == 0 ? sampleRate / 2 : maxFreqVal;
+        double melLow = hz_to_mel_d(formula, minFreq), melHigh = hz_to_mel_d(formula, maxFreq);
+        double melStep = (melHigh - melLow) / (M + 1);
+        int nfft = (numBins - 1) * 2;
+        double hzStep = sampleRate / nfft;
+        double *fp = (double *)malloc(sizeof(double) * (M + 2));
+        for (int i = 0; i < M + 2; i++)
+            fp[i] = mel_to_hz_d(formula, i == M + 1 ? melHigh : melLow + i * melStep);
+        const double *xs = src + (long)b * srcNStride;
+        double *y = dst + (long)b * dstNStride;
+        for (int m = 0; m < M; m++) {
+            double norm = normalize ? 2.0 / (fp[m + 2] - fp[m]) : 1.0;
+            for (int f = 0; f < F; f++) y[m * dstHStride + f] = 0.0;
+            for (int k = 0; k < numBins; k++) {
+                double fk = k * hzStep;
+                if (fk < minFreq || fk >= maxFreq) continue;   /* bin range [ceil(fmin/hz), ceil(fmax/hz)) */
+                double w = 0.0;
+                if (fk >= fp[m] && fk < fp[m + 1]) w = (fk - fp[m]) / (fp[m + 1] - fp[m]);
+                else if (fk >= fp[m + 1] && fk < fp[m + 2]) w = (fp[m + 2] - fk) / (fp[m + 2] - fp[m + 1]);
+                if (w == 0.0) continue;
+                w *= norm;
+                const double *row = xs + (long)k * srcHStride;
+                for (int f = 0; f < F; f++) y[m * dstHStride + f] += w * row[f];
+            }
+        }
+        free(fp);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* to_decibels  (cpu/kernel/to_decibels.cpp:27-99)                      */
+/* out = (mult*log10(2)) * log2(max(minRatio, x/ref)); dims[2b]=width,  */
+/* dims[2b+1]=height (RpptImagePatch order); width 1 -> 1-D of height.  */
+/* ------------------------------------------------------------------ */
+ORC_API void orc_to_db_f32(const float *src, long srcNStride, long srcHStride,
+                           float *dst, long dstNStride, long dstHStride, const unsigned *dims, int B,
+                           float cutOffDB, float multiplier, float referenceMagnitude, int nthreads)
+{
+    float minRatio = (float)pow(10.0, (double)(cutOffDB / multiplier));
+    if (minRatio == 0.0f) minRatio = nextafterf(0.0f, 1.0f);
+    const float log10Factor = 0.3010299956639812f;
+    const float mult = multiplier * log10Factor;
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcNStride;
+        float *y = dst + (long)b * dstNStride;
+        unsigned width = dims[2 * b], height = dims[2 * b + 1];
+        float ref = referenceMagnitude;
+        if (referenceMagnitude == 0.0f) {
+            ref = -FLT_MAX;
+            if (width == 1) { for (unsigned i = 0; i < height; i++) if (x[i] > ref) ref = x[i]; }
+            else for (unsigned r = 0; r < height; r++)
+                for (unsigned c = 0; c < width; c++) if (x[r * srcHStride + c] > ref) ref = x[r * srcHStride + c];
+        }
+        float inv = (ref != 0.0f) ? (1.f / ref) : 1.0f;
+        if (width == 1) {
+            for (unsigned i = 0; i < height; i++) {
+                float v = x[i] * inv;
+                y[i] = mult * log2f(v > minRatio ? v : minRatio);
+            }
+        } else {
+            for (unsigned r = 0; r < height; r++)
+                for (unsigned c = 0; c < width; c++) {
+                    float v = x[r * srcHStride + c] * inv;
+                    y[r * dstHStride + c] = mult * log2f(v > minRatio ? v : minRatio);
+                }
+        }
+    }
+}
+
+ORC_API void orc_to_db_f64(const double *src, long srcNStride, long srcHStride,
+                           double *dst, long dstNStride, long dstHStride, const unsigned *dims, int B,
+                           double cutOffDB, double multiplier, double referenceMagnitude)
+{
+    double minRatio = pow(10.0, cutOffDB / multiplier);
+    for (int b = 0; b < B; b++) {
+        const double *x = src + (long)b * srcNStride;
+        double *y = dst + (long)b * dstNStride;
+        unsigned width = dims[2 * b], height = dims[2 * b + 1];
+        double ref = referenceMagnitude;
+        if (referenceMagnitude == 0.0) {
+            ref = -DBL_MAX;
+            for (unsigned r = 0; r < height; r++)
+                for (unsigned c = 0; c < width; c++) {
+                    double v = width == 1 ? x[r] : x[r * srcHStride + c];
+                    if (v > ref) ref = v;
+                }
+        }
+        double inv = ref != 0.0 ? 1.0 / ref : 1.0;
+        for (unsigned r = 0; r < height; r++)
+            for (unsigned c = 0; c < width; c++) {
+                long si = width == 1 ? r : r * srcHStride + c, di = width == 1 ? r : r * dstHStride + c;
+                double v = x[si] * inv;
+                y[di] = multiplier * log10(v > minRatio ? v : minRatio);
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* down-mixing  (cpu/kernel/down_mixing.cpp:28-122)                     */
+/* mono[t] = sum_c x[t*C+c] * (1/C); lanes of 4 (C<8) / 8 (C>=8) summed  */
+/* pairwise like a horizontal add, remaining channels added after.      */
+/* ------------------------------------------------------------------ */
+/* __m128 horizontal add: (v0+v1) + (v2+v3) */
+static float hsum4(const float *v) { return (v[0] + v[1]) + (v[2] + v[3]); }
+/* __m256 horizontal add: ((v0+v4)+(v2+v6)) + ((v1+v5)+(v3+v7)) */
+static float hsum8(const float *v)
+{
+    float s0 = v[0] + v[4], s1 = v[1] + v[5], s2 = v[2] + v[6], s3 = v[3] + v[7];
+    return (s0 + s2) + (s1 + s3);
+}
+
+ORC_API void orc_down_mix_f32(const float *src, long srcStride, float *dst, long dstStride,
+                              const int *srcDims, int B, int normalizeWeights, int nthreads)
+{
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        float *y = dst + (long)b * dstStride;
+        int S = srcDims[2 * b], C = srcDims[2 * b + 1];
+        if (C == 1) { memcpy(y, x, sizeof(float) * (size_t)S); continue; }
+        float *w = (float *)malloc(sizeof(float) * C);
+        for (int c = 0; c < C; c++) w[c] = 1.f / C;
+        if (normalizeWeights) {
+            float sum = 0.0f;
+            for (int c = 0; c < C; c++) sum += w[c];
+            float inv = (float)(1.0 / sum);
+            for (int c = 0; c < C; c++) w[c] *= inv;
+        }
+        int lanes = C > 7 ? 8 : 4;
+        int aligned = (C / lanes) * lanes;
+        for (long t = 0; t < S; t++) {
+            const float *xt = x + t * C;
+            float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int c = 0;
+            for (; c < aligned; c += lanes)
+                for (int l = 0; l < lanes; l++) { float p = xt[c + l] * w[c + l]; acc[l] = acc[l] + p; }
+            float s;
+            if (lanes == 8) s = hsum8(acc);
+            else s = hsum4(acc);
+            for (; c < C; c++) { float p = xt[c] * w[c]; s = s + p; }
+            y[t] = s;
+        }
+        free(w);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* resample  (cpu/kernel/resample.cpp:27-146, window rppdefs.h:751-790)  */
+/* ------------------------------------------------------------------ */
+typedef struct { float scale, center; int lobes, lookupSize; const float *lut; } orc_window;
+
+static float win_eval(const orc_window *w, float x)
+{
+    float locRaw = x * w->scale + w->center;
+    int locFloor = (int)floorf(locRaw);
+    float weight = locRaw - (float)locFloor;
+    if (locFloor > w->lookupSize - 2) locFloor = w->lookupSize - 2;
+    if (locFloor < 0) locFloor = 0;
+    float cur = w->lut[locFloor], nxt = w->lut[locFloor + 1];
+    return cur + weight * (nxt - cur);
+}
+
+/* the 4-wide path truncates instead of flooring and does not clamp the index */
+static float win_eval_trunc(const orc_window *w, float x)
+{
+    float locRaw = x * w->scale + w->center;
+    int locT = (int)locRaw;
+    float weight = locRaw - (float)locT;
+    float cur = w->lut[locT], nxt = w->lut[locT + 1];
+    return cur + weight * (nxt - cur);
+}
+
+ORC_API void orc_resample_f32(const float *src, long srcStride, float *dst, long dstStride,
+                              const float *inRate, const float *outRate, const int *srcDims, int B,
+                              float wScale, float wCenter, int lobes, int lookupSize, const float *lut,
+                              int nthreads)
+{
+    orc_window w = {wScale, wCenter, lobes, lookupSize, lut};
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        float *y = dst + (long)b * dstStride;
+        float ir = inRate[b], orr = outRate[b];
+        int L = srcDims[2 * b], C = srcDims[2 * b + 1];
+        if (ir == orr) { memcpy(y, x, sizeof(float) * (size_t)L * C); continue; }
+        float lf = (float)L * orr;
+        int outEnd = (int)ceilf(lf / ir);
+        const int block = 256;
+        double scale = (double)ir / orr;
+        float fscale = (float)scale;
+        float *acc = (float *)malloc(sizeof(float) * (C > 0 ? C : 1));
+        for (int ob = 0; ob < outEnd; ob += block) {
+            int be = ob + block < outEnd ? ob + block : outEnd;
+            double inBlockRaw = ob * scale;
+            int inBlockRounded = (int)inBlockRaw;
+            float inPos = (float)(inBlockRaw - inBlockRounded);
+            const float *inBlock = x + (long)inBlockRounded * C;
+            for (int op = ob; op < be; op++, inPos += fscale) {
+                int xc = (int)ceilf(inPos);
+                int loc0 = xc - lobes, loc1 = xc + lobes;
+                if (loc0 + inBlockRounded < 0) loc0 = -inBlockRounded;
+                if (loc1 + inBlockRounded > L) loc1 = L - inBlockRounded;
+                if (C == 1) {
+                    int li = loc0;
+                    float locBegin = (float)li - inPos;
+                    float lane[4] = {0, 0, 0, 0};
+                    float pos[4] = {locBegin + 0.f, locBegin + 1.f, locBegin + 2.f, locBegin + 3.f};
+                    for (; li + 3 < loc1; li += 4) {
+                        for (int l = 0; l < 4; l++) {
+                            float wv = win_eval_trunc(&w, pos[l]);
+                            float p = inBlock[li + l] * wv;
+                            lane[l] = lane[l] + p;
+                            pos[l] = pos[l] + 4.0f;
+                        }
+                    }
+                    /* (a0+a2, a1+a3) then their sum, as the shuffle reduction does */
+                    float s02 = lane[0] + lane[2], s13 = lane[1] + lane[3];
+                    float a = s02 + s13;
+                    float xx = (float)li - inPos;
+                    for (; li < loc1; li++, xx += 1.0f) {
+                        float p = inBlock[li] * win_eval(&w, xx);
+                        a = a + p;
+                    }
+                    y[op] = a;
+                } else {
+                    for (int c = 0; c < C; c++) acc[c] = 0.0f;
+                    float xx = (float)loc0 - inPos;
+                    for (int li = loc0; li < loc1; li++, xx += 1.0f) {
+                        float wv = win_eval(&w, xx);
+                        for (int c = 0; c < C; c++) { float p = inBlock[(long)li * C + c] * wv; acc[c] = acc[c] + p; }
+                    }
+                    for (int c = 0; c < C; c++) y[(long)op * C + c] = acc[c];
+                }
+            }
+        }
+        free(acc);
+    }
+}
+
+/* Resampling LUT as the reference test suite builds it (rpp_test_suite_audio.h:390-430): */
+/* Hann-windowed sinc, lookupSize = coeffs + 5, lut[0] = lut[coeffs+1..] = 0.            */
+ORC_API void orc_windowed_sinc_lut(int coeffs, int lobes, float *lut, float *outScale, float *outCenter)
+{
+    float scale = 2.0f * lobes / (coeffs - 1);
+    float scaleEnv = 2.0f / coeffs;
+    int center = (int)((coeffs - 1) * 0.5f);
+    for (int i = 0; i < coeffs + 5; i++) lut[i] = 0.0f;
+    for (int i = 0; i < coeffs; i++) {
+        float x = (i - center) * scale;
+        float yv = (i - center) * scaleEnv;
+        float xp = (float)((double)x * M_PI);
+        float s = (fabsf(xp) < 1e-5f) ? (float)(1.f - (xp * xp * 0.16666667)) : sinf(xp) / xp;
+        double h = 0.5 * (1 + cos((double)yv * M_PI));
+        lut[i + 1] = (float)(s * h);
+    }
+    *outCenter = (float)(center + 1);
+    *outScale = 1 / scale;
+}
+
+ORC_API int orc_num_threads_available(void)
+{
+#ifdef _OPENMP
+    return omp_get_num_procs();
+#else
+    return 1;
+#endif
+}

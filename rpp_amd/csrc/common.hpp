@@ -1,0 +1,120 @@
+// common.hpp -- internal definitions shared by the gfx950 audio kernels and the C-ABI layer.
+//
+// Every translation unit is compiled with -ffp-contract=off: a*b+c is rounded twice unless
+// the code asks for fmaf() explicitly.  This keeps the FP32 stages on the same rounding
+// points as RPP's HOST kernels, which are built with -mno-fma (src/modules/CMakeLists.txt:105-124).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rpp_audio.h"
+
+#define RPPA_EXPORT extern "C" __attribute__((visibility("default")))
+
+// HIP runtime failures become RPP_ERROR (the reference calls exit(-1), rppdefs.h:69-75).
+#define RPPA_HIP(call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            fprintf(stderr, "rpp_audio: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                    \
+            return RPP_ERROR;                                                               \
+        }                                                                                   \
+    } while (0)
+
+namespace rppa {
+
+constexpr int kWave = 64;
+
+// Device-resident, lazily generated tables for one (nfft, windowLength, window) triple.
+// basis[t * cols + c]: window-folded DFT columns (see tables.hip for the packing).
+struct DftTables {
+    int nfft = 0, W = 0, cols = 0;
+    float *basis = nullptr;        // [W][cols] FP32
+    uint16_t *basisHi = nullptr;   // [cols][Wp] bf16 or fp16 hi part, column-major (K contiguous)
+    uint16_t *basisLo = nullptr;   // [cols][Wp] lo part
+    int Wp = 0;                    // W padded to the MFMA K step
+    int splitKind = -1;            // which 16-bit format basisHi/Lo hold (-1 none)
+};
+
+// Mel tables for one parameter set, built on the GPU (tables.hip).
+struct MelTables {
+    int numBins = 0, M = 0;
+    int binStart = 0, binEnd = 0;
+    float *weightsDown = nullptr;   // [numBins]
+    int *intervals = nullptr;       // [numBins]  filter whose rising slope bin k is on, -1 unused
+    float *normFactors = nullptr;   // [M]
+    int *filterBins = nullptr;      // [M][2] first bin / one-past-last bin of filter m (gather form)
+    float *dense = nullptr;         // [M][numBins] dense normalised weights (FP32)
+};
+
+}  // namespace rppa
+
+// The opaque handle of api/rpp.h.
+struct rppHandle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t batchSize = 1;
+    int numCU = 256;
+    int numXCD = 8;
+
+    // grow-only device workspace
+    void *ws = nullptr;
+    size_t wsBytes = 0;
+
+    // small pinned staging buffer for host-built metadata
+    void *pinned = nullptr;
+    size_t pinnedBytes = 0;
+
+    std::map<std::tuple<int, int, uint64_t>, rppa::DftTables> dft;
+    std::map<std::string, rppa::MelTables> mel;
+
+    RpptLogMelTiles tiles{0, 0};
+};
+
+namespace rppa {
+
+// Returns a device pointer of at least `bytes` (grow-only); nullptr on failure or when the
+// stream is being captured and the pool would have to grow.
+void *workspace(rppHandle *h, size_t bytes);
+
+// Window-folded DFT basis for (nfft, W, window); window == nullptr means RPP's Hann.
+// splitKind: -1 FP32 only, 0 bf16 hi/lo, 1 fp16 hi/lo (scaled).
+const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *window, int splitKind);
+
+// Mel tables built the way the reference builds them (cpu/kernel/mel_filter_bank.cpp:55-112).
+const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, float maxFreq, float minFreq,
+                                int formula, int M, int normalize);
+
+inline unsigned ceil_div(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// RPP's frame count (cpu/kernel/spectrogram.cpp:47-52).
+__host__ __device__ inline int num_windows(int length, int W, int H, bool center)
+{
+    if (!center) length -= W;
+    return length / H + 1;
+}
+
+// RPP's reflect index (cpu/kernel/spectrogram.cpp:55-69): mirror without repeating the edge.
+__host__ __device__ inline int idx_reflect(int loc, int lo, int hi)
+{
+    if (hi - lo < 2) return hi - 1;
+    for (;;) {
+        if (loc < lo) loc = 2 * lo - loc;
+        else if (loc >= hi) loc = 2 * hi - 2 - loc;
+        else break;
+    }
+    return loc;
+}
+
+}  // namespace rppa

@@ -1,0 +1,103 @@
+// handle.hip -- the rppHandle_t lifecycle of api/rpp.h (src/modules/handle_api.cpp:29-83).
+//
+// A handle owns: the stream all work is enqueued on, a grow-only device workspace, a small
+// pinned staging buffer and the per-parameter caches of GPU-built tables.  Unlike the
+// reference (handle_hip.cpp:170-235, 1.49 GB device + 33 MB pinned up front), nothing is
+// allocated until an entry point needs it.
+#include "common.hpp"
+
+namespace rppa {
+
+void *workspace(rppHandle *h, size_t bytes)
+{
+    if (bytes <= h->wsBytes) return h->ws;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        fprintf(stderr, "rpp_audio: workspace must grow to %zu bytes while the stream is being captured; "
+                        "run the call once before capture\n", bytes);
+        return nullptr;
+    }
+    if (h->ws) {
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+        (void)hipFree(h->ws);
+        h->ws = nullptr;
+        h->wsBytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    if (hipMalloc(&h->ws, want) != hipSuccess) return nullptr;
+    h->wsBytes = want;
+    return h->ws;
+}
+
+}  // namespace rppa
+
+RPPA_EXPORT rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u numThreads, void *stream,
+                                  RppBackend backend)
+{
+    (void)numThreads;
+    if (!handle) return rppStatusBadParm;
+    if (backend != RPP_HIP_BACKEND) {
+        fprintf(stderr, "rpp_audio: only RPP_HIP_BACKEND handles are served by this library\n");
+        return rppStatusNotImplemented;
+    }
+    rppHandle *h = new (std::nothrow) rppHandle();
+    if (!h) return rppStatusAllocFailed;
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return rppStatusInternalError; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) h->numCU = prop.multiProcessorCount;
+    h->stream = static_cast<hipStream_t>(stream);
+    h->batchSize = nBatchSize ? nBatchSize : 1;
+    *handle = h;
+    return rppStatusSuccess;
+}
+
+RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
+{
+    if (!handle) return rppStatusBadParm;
+    if (backend != RPP_HIP_BACKEND) return rppStatusNotImplemented;
+    (void)hipStreamSynchronize(handle->stream);
+    if (handle->ws) (void)hipFree(handle->ws);
+    if (handle->pinned) (void)hipHostFree(handle->pinned);
+    for (auto &kv : handle->dft) {
+        (void)hipFree(kv.second.basis);
+        if (kv.second.basisHi) (void)hipFree(kv.second.basisHi);
+        if (kv.second.basisLo) (void)hipFree(kv.second.basisLo);
+    }
+    for (auto &kv : handle->mel) {
+        (void)hipFree(kv.second.weightsDown);
+        (void)hipFree(kv.second.intervals);
+        (void)hipFree(kv.second.normFactors);
+        (void)hipFree(kv.second.filterBins);
+        (void)hipFree(kv.second.dense);
+    }
+    delete handle;
+    return rppStatusSuccess;
+}
+
+RPPA_EXPORT rppStatus_t rppSetBatchSize(rppHandle_t handle, size_t batchSize)
+{
+    if (!handle) return rppStatusBadParm;
+    handle->batchSize = batchSize;
+    return rppStatusSuccess;
+}
+
+RPPA_EXPORT rppStatus_t rppGetBatchSize(rppHandle_t handle, size_t *batchSize)
+{
+    if (!handle || !batchSize) return rppStatusBadParm;
+    *batchSize = handle->batchSize;
+    return rppStatusSuccess;
+}
+
+RPPA_EXPORT rppStatus_t rppSetStream(rppHandle_t handle, void *streamID)
+{
+    if (!handle) return rppStatusBadParm;
+    handle->stream = static_cast<hipStream_t>(streamID);
+    return rppStatusSuccess;
+}
+
+RPPA_EXPORT rppStatus_t rppGetStream(rppHandle_t handle, void **streamID)
+{
+    if (!handle || !streamID) return rppStatusBadParm;
+    *streamID = handle->stream;
+    return rppStatusSuccess;
+}

@@ -1,0 +1,275 @@
+// tables.hip -- window, window-folded DFT basis and mel filter tables, generated on the GPU in FP64.
+//
+// DFT basis column packing (nfft = N, K = N/2 + 1 bins, W window samples):
+//   column 0        w[t] * cos(0)              -> bin 0 (DC, real only)
+//   column 1        w[t] * cos(pi t)           -> bin N/2 (Nyquist, real only; zero when N is odd)
+//   columns 2k,2k+1 w[t] * cos(2 pi k t / N), -w[t] * sin(2 pi k t / N)   for 1 <= k <= (N-1)/2
+// so cols = N (+1 when N is odd) and power[k] = C[2k]^2 + C[2k+1]^2, power[0] = C[0]^2,
+// power[N/2] = C[1]^2.  The reference places the frame at offset (N-W)/2 of the FFT buffer
+// (cpu/kernel/spectrogram.cpp:197-212); that only rotates phase, so the basis uses origin 0.
+#include "common.hpp"
+
+#include <cstring>
+
+namespace rppa {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// RPP's Hann with a half-sample shift (cpu/kernel/spectrogram.cpp:36-44), FP64 -> FP32.
+__global__ void hann_kernel(int W, float *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= W) return;
+    double a = (2.0 * kPi) / W;
+    out[t] = static_cast<float>(0.5 * (1.0 - cos(a * (t + 0.5))));
+}
+
+__device__ inline uint16_t f32_to_bf16_rne(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(r >> 16);
+}
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ inline uint16_t f32_to_f16_rne(float v)
+{
+    _Float16 h = static_cast<_Float16>(v);
+    uint16_t r;
+    __builtin_memcpy(&r, &h, 2);
+    return r;
+}
+__device__ inline float f16_to_f32(uint16_t u)
+{
+    _Float16 h;
+    __builtin_memcpy(&h, &u, 2);
+    return static_cast<float>(h);
+}
+
+// One thread per (t, c).  basis[t][c] in FP32; optional split copies, column-major [c][Wp].
+__global__ void dft_basis_kernel(int N, int W, int cols, int Wp, const float *__restrict__ win,
+                                 float *__restrict__ basis, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                                 int splitKind, float splitScale)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.y;
+    if (c >= cols || t >= Wp) return;
+    double v = 0.0;
+    if (t < W) {
+        double w = static_cast<double>(win[t]);
+        int k;
+        bool isSin = false;
+        if (c == 0) k = 0;
+        else if (c == 1) k = (N % 2 == 0) ? N / 2 : -1;
+        else { k = c >> 1; isSin = (c & 1) != 0; }
+        if (k >= 0) {
+            long long kt = (static_cast<long long>(k) * t) % N;
+            double th = 2.0 * kPi * static_cast<double>(kt) / N;
+            v = isSin ? -w * sin(th) : w * cos(th);
+        }
+        basis[static_cast<size_t>(t) * cols + c] = static_cast<float>(v);
+    }
+    if (splitKind >= 0) {
+        double s = v * splitScale;
+        uint16_t h, l;
+        if (splitKind == 0) {
+            h = f32_to_bf16_rne(static_cast<float>(s));
+            l = f32_to_bf16_rne(static_cast<float>(s - bf16_to_f32(h)));
+        } else {
+            h = f32_to_f16_rne(static_cast<float>(s));
+            l = f32_to_f16_rne(static_cast<float>(s - f16_to_f32(h)));
+        }
+        hi[static_cast<size_t>(c) * Wp + t] = h;
+        lo[static_cast<size_t>(c) * Wp + t] = l;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mel scales as RPP defines them: FP32 member functions (api/rppdefs.h:795-851).  logf/expf are
+// evaluated as FP64 log/exp rounded once to FP32 (the correctly rounded FP32 result).
+// ---------------------------------------------------------------------------------------------
+__device__ inline float logf_cr(float x) { return static_cast<float>(log(static_cast<double>(x))); }
+__device__ inline float expf_cr(float x) { return static_cast<float>(exp(static_cast<double>(x))); }
+
+struct MelScale {
+    int formula;  // 0 Slaney, 1 HTK
+    __device__ float minLogMel() const { return (1000.0f - 0.0f) / 66.666667f; }
+    __device__ float hz_to_mel(float hz) const
+    {
+        if (formula == 1) return 1127.0f * logf_cr(1.0f + (hz / 700.0f));
+        const float invStepLog = 1.0f / 0.068751777f, invFsp = 1.0f / 66.666667f;
+        if (hz >= 1000.0f) return minLogMel() + logf_cr(hz * 0.001f) * invStepLog;
+        return (hz - 0.0f) * invFsp;
+    }
+    __device__ float mel_to_hz(float mel) const
+    {
+        if (formula == 1) return 700.0f * (expf_cr(mel / 1127.0f) - 1.0f);
+        if (mel >= minLogMel()) return 1000.0f * expf_cr(0.068751777f * (mel - minLogMel()));
+        return 0.0f + mel * 66.666667f;
+    }
+};
+
+// Single-thread table build following the reference's loop exactly
+// (cpu/kernel/mel_filter_bank.cpp:55-112): FP64 mel points accumulated by repeated addition,
+// FP32 scale functions, FP64 slopes rounded to FP32 weights.
+__global__ void mel_tables_kernel(int numBins, float sampleRate, float maxFreqVal, float minFreq, int formula,
+                                  int M, int normalize, float *weightsDown, int *intervals, float *normFactors,
+                                  int *filterBins, float *dense, int *binRange)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MelScale ms{formula};
+    float maxFreq = (maxFreqVal == 0) ? sampleRate / 2 : maxFreqVal;
+    double melLow = ms.hz_to_mel(minFreq);
+    double melHigh = ms.hz_to_mel(maxFreq);
+    double melStep = (melHigh - melLow) / (M + 1);
+    int nfft = (numBins - 1) * 2;
+    double hzStep = static_cast<double>(sampleRate) / nfft;
+    double invHzStep = 1.0 / hzStep;
+    int binStart = static_cast<int>(ceil(minFreq * invHzStep));
+    int binEnd = static_cast<int>(ceil(maxFreq * invHzStep));
+    if (binEnd > numBins) binEnd = numBins;
+    for (int m = 0; m < M; m++) normFactors[m] = 1.0f;
+    for (int k = 0; k < numBins; k++) { weightsDown[k] = 0.0f; intervals[k] = -1; }
+    int bin = binStart;
+    double mel0 = melLow, mel1 = melLow + melStep;
+    double fIter = bin * hzStep;
+    for (int iv = 0; iv < M + 1; iv++, mel0 = mel1, mel1 += melStep) {
+        double f0 = ms.mel_to_hz(static_cast<float>(mel0));
+        double f1 = ms.mel_to_hz(static_cast<float>(iv == M ? melHigh : mel1));
+        double slope = 1. / (f1 - f0);
+        if (normalize && iv < M) {
+            double f2 = ms.mel_to_hz(static_cast<float>(mel1 + melStep));
+            normFactors[iv] = static_cast<float>(2.0 / (f2 - f0));
+        }
+        for (; bin < binEnd && fIter < f1; bin++, fIter = bin * hzStep) {
+            weightsDown[bin] = static_cast<float>((f1 - fIter) * slope);
+            intervals[bin] = iv;
+        }
+    }
+    binRange[0] = binStart;
+    binRange[1] = binEnd;
+    // gather ranges and the dense normalised matrix
+    for (int m = 0; m < M; m++) { filterBins[2 * m] = 0; filterBins[2 * m + 1] = 0; }
+    for (int m = 0; m < M; m++) {
+        int first = -1, last = -1;
+        for (int k = binStart; k < binEnd; k++) {
+            int up = intervals[k];
+            if (up == m || up - 1 == m) {
+                if (first < 0) first = k;
+                last = k;
+            }
+        }
+        if (first >= 0) { filterBins[2 * m] = first; filterBins[2 * m + 1] = last + 1; }
+    }
+    for (int i = 0; i < M * numBins; i++) dense[i] = 0.0f;
+    for (int k = binStart; k < binEnd; k++) {
+        int up = intervals[k], down = up - 1;
+        float wDown = weightsDown[k], wUp = 1.0f - weightsDown[k];
+        if (down >= 0) dense[down * numBins + k] = normalize ? wDown * normFactors[down] : wDown;
+        if (up >= 0 && up < M) dense[up * numBins + k] = normalize ? wUp * normFactors[up] : wUp;
+    }
+}
+
+uint64_t fnv1a(const void *p, size_t n)
+{
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+}  // namespace
+
+const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *window, int splitKind)
+{
+    uint64_t key = window ? fnv1a(window, sizeof(float) * W) : 0;
+    auto k = std::make_tuple(nfft, W, key);
+    auto it = h->dft.find(k);
+    if (it != h->dft.end() && (splitKind < 0 || it->second.splitKind == splitKind)) return &it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
+
+    DftTables t;
+    if (it != h->dft.end()) {
+        t = it->second;  // reuse the FP32 basis, (re)build the split copy
+    } else {
+        t.nfft = nfft;
+        t.W = W;
+        t.cols = nfft + (nfft & 1);
+        t.Wp = (W + 31) / 32 * 32;
+    }
+    float *win = nullptr;
+    if (hipMalloc(&win, sizeof(float) * W) != hipSuccess) return nullptr;
+    if (window) {
+        if (hipMemcpyAsync(win, window, sizeof(float) * W, hipMemcpyHostToDevice, h->stream) != hipSuccess) return nullptr;
+    } else {
+        hipLaunchKernelGGL(hann_kernel, dim3(ceil_div(W, 256)), dim3(256), 0, h->stream, W, win);
+    }
+    if (!t.basis && hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * t.cols) != hipSuccess) return nullptr;
+    if (splitKind >= 0) {
+        if (t.basisHi) { (void)hipFree(t.basisHi); (void)hipFree(t.basisLo); }
+        size_t n = static_cast<size_t>(t.cols) * t.Wp;
+        if (hipMalloc(&t.basisHi, n * 2) != hipSuccess || hipMalloc(&t.basisLo, n * 2) != hipSuccess) return nullptr;
+        t.splitKind = splitKind;
+    }
+    float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
+    hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(t.cols, 256), splitKind >= 0 ? t.Wp : W), dim3(256), 0,
+                       h->stream, nfft, W, t.cols, t.Wp, win, t.basis, t.basisHi, t.basisLo, splitKind, splitScale);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+    (void)hipFree(win);
+    h->dft[k] = t;
+    return &h->dft[k];
+}
+
+const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, float maxFreq, float minFreq,
+                                int formula, int M, int normalize)
+{
+    char keyb[160];
+    snprintf(keyb, sizeof(keyb), "%d/%a/%a/%a/%d/%d/%d", numBins, sampleRate, maxFreq, minFreq, formula, M, normalize);
+    std::string key(keyb);
+    auto it = h->mel.find(key);
+    if (it != h->mel.end()) return &it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
+
+    MelTables t;
+    t.numBins = numBins;
+    t.M = M;
+    int *range = nullptr;
+    if (hipMalloc(&t.weightsDown, sizeof(float) * numBins) != hipSuccess ||
+        hipMalloc(&t.intervals, sizeof(int) * numBins) != hipSuccess ||
+        hipMalloc(&t.normFactors, sizeof(float) * M) != hipSuccess ||
+        hipMalloc(&t.filterBins, sizeof(int) * 2 * M) != hipSuccess ||
+        hipMalloc(&t.dense, sizeof(float) * static_cast<size_t>(M) * numBins) != hipSuccess ||
+        hipMalloc(&range, sizeof(int) * 2) != hipSuccess)
+        return nullptr;
+    hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(64), 0, h->stream, numBins, sampleRate, maxFreq, minFreq,
+                       formula, M, normalize, t.weightsDown, t.intervals, t.normFactors, t.filterBins, t.dense, range);
+    int hr[2] = {0, 0};
+    if (hipMemcpyAsync(hr, range, sizeof(hr), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return nullptr;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+    (void)hipFree(range);
+    t.binStart = hr[0];
+    t.binEnd = hr[1];
+    h->mel[key] = t;
+    return &h->mel[key];
+}
+
+}  // namespace rppa
+
+// Debug/test export: copies the mel tables of a parameter set to host arrays.
+RPPA_EXPORT RppStatus rppa_debug_mel_tables(rppHandle_t handle, int numBins, float sampleRate, float maxFreq,
+                                            float minFreq, int formula, int M, int normalize, float *weightsDown,
+                                            int *intervals, float *normFactors, int *binRange)
+{
+    if (!handle) return RPP_ERROR_INVALID_ARGUMENTS;
+    const rppa::MelTables *t = rppa::get_mel_tables(handle, numBins, sampleRate, maxFreq, minFreq, formula, M, normalize);
+    if (!t) return RPP_ERROR;
+    RPPA_HIP(hipMemcpy(weightsDown, t->weightsDown, sizeof(float) * numBins, hipMemcpyDeviceToHost));
+    RPPA_HIP(hipMemcpy(intervals, t->intervals, sizeof(int) * numBins, hipMemcpyDeviceToHost));
+    RPPA_HIP(hipMemcpy(normFactors, t->normFactors, sizeof(float) * M, hipMemcpyDeviceToHost));
+    binRange[0] = t->binStart;
+    binRange[1] = t->binEnd;
+    return RPP_SUCCESS;
+}

@@ -1,0 +1,129 @@
+"""GPU parity of the spectrogram and the 4-call chain against the FP32 oracle (the restatement of
+RPP HOST pinned to the reference's goldens) across the parameter space of the API.
+
+Bar: spectrogram |err| / frame max <= 1e-5; mel power <= 1e-4 relative (north-star tolerance for
+the plain FP32 path); dB outputs within 4.5e-4 dB (= 1e-4 relative in power).
+"""
+import numpy as np
+import pytest
+import torch
+
+from rpp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def frame_rel_err(out, ref, frames, nft=True):
+    worst = 0.0
+    for b, F in enumerate(frames):
+        if F <= 0:
+            continue
+        a = out[b, :, :F] if nft else out[b, :F, :].T
+        r = ref[b, :, :F] if nft else ref[b, :F, :].T
+        a, r = a.astype(np.float64), r.astype(np.float64)
+        fm = np.maximum(r.max(axis=0), 1e-30)
+        worst = max(worst, float((np.abs(a - r) / fm).max()))
+    return worst
+
+
+SPEC_CASES = [
+    # nfft, W, H, center, reflect, power, layout
+    (512, 400, 160, True, True, 2, "NFT"),
+    (512, 400, 160, True, True, 1, "NFT"),
+    (512, 400, 160, True, True, 2, "NTF"),
+    (512, 320, 160, False, True, 2, "NFT"),
+    (512, 400, 160, True, False, 2, "NFT"),
+    (1024, 1024, 256, True, True, 2, "NFT"),
+    (256, 256, 100, True, True, 2, "NTF"),
+    (400, 400, 160, True, True, 2, "NFT"),     # non-power-of-two nfft
+    (401, 300, 128, True, True, 2, "NFT"),     # odd nfft
+]
+
+
+@pytest.mark.parametrize("case", SPEC_CASES, ids=lambda c: "-".join(map(str, c)))
+def test_spectrogram_params(gpu, handle, orc, case):
+    nfft, W, H, center, reflect, power, layout = case
+    rng = np.random.default_rng(nfft + W + H)
+    L = np.array([1, 2, W - 1, W, W + 1, 3 * H + 7, 5000, 12345], np.int32)
+    S = int(L.max()) + 8
+    x = rng.uniform(-1, 1, (len(L), S)).astype(np.float32)
+    ref = orc.spectrogram_f32(x, L, nfft, W, H, center, reflect, power, layout)
+    frames = np.array([orc.num_windows(int(l), W, H, center) for l in L])
+    out = torch.full(ref.shape, -1.0, device="cuda")
+    gpu.spectrogram(handle, dev(x), out, dev(L), nfft=nfft, window_length=W, window_step=H, center=center,
+                    reflect=reflect, power=power, layout=gpu.NFT if layout == "NFT" else gpu.NTF)
+    o = out.cpu().numpy()
+    assert frame_rel_err(o, ref, frames, layout == "NFT") <= 1e-5
+    # nothing written beyond each clip's frames
+    for b, F in enumerate(frames):
+        F = max(F, 0)
+        tail = o[b, :, F:] if layout == "NFT" else o[b, F:, :]
+        assert np.all(tail == -1.0)
+
+
+def test_spectrogram_user_window(gpu, handle, orc):
+    rng = np.random.default_rng(5)
+    W, nfft, H = 400, 512, 160
+    win = rng.uniform(0.1, 1.0, W).astype(np.float32)
+    L = np.array([16000, 8000], np.int32)
+    x = rng.uniform(-1, 1, (2, 16008)).astype(np.float32)
+    ref = orc.spectrogram_f32(x, L, nfft, W, H, window=win)
+    out = torch.zeros(ref.shape, device="cuda")
+    gpu.spectrogram(handle, dev(x), out, dev(L), nfft=nfft, window_length=W, window_step=H, window=win)
+    frames = [orc.num_windows(int(l), W, H, True) for l in L]
+    assert frame_rel_err(out.cpu().numpy(), ref, frames) <= 1e-5
+
+
+def test_mel_tables_match_reference(gpu, handle, orc):
+    """GPU-built tables equal the reference algorithm's (oracle restatement, pinned to the goldens)."""
+    import ctypes as C
+    for (nb, sr, fmax, fmin, formula, M, norm) in [(257, 16000.0, 8000.0, 0.0, 0, 80, 1),
+                                                   (513, 44100.0, 22050.0, 0.0, 0, 128, 1),
+                                                   (257, 16000.0, 7600.0, 125.0, 1, 64, 0)]:
+        wd = np.zeros(nb, np.float32)
+        iv = np.zeros(nb, np.int32)
+        nf = np.zeros(M, np.float32)
+        rg = np.zeros(2, np.int32)
+        st = gpu.lib().rppa_debug_mel_tables(handle.h, nb, sr, fmax, fmin, formula, M, norm,
+                                             wd.ctypes.data, iv.ctypes.data, nf.ctypes.data, rg.ctypes.data)
+        assert st == 0
+        rwd, riv, rnf, rrg = orc.mel_tables_f32(nb, sr, fmax, fmin, M, "htk" if formula else "slaney", bool(norm))
+        assert tuple(rg) == rrg
+        assert np.array_equal(iv, riv)
+        lo, hi = rrg
+        ulp = np.abs(wd[lo:hi].view(np.int32) - rwd[lo:hi].view(np.int32))
+        assert ulp.max() <= 1
+        assert np.abs(nf.view(np.int32) - rnf.view(np.int32)).max() <= 1
+
+
+@pytest.mark.parametrize("formula,normalize", [("slaney", True), ("htk", False), ("htk", True)])
+def test_mel_params(gpu, handle, orc, formula, normalize):
+    rng = np.random.default_rng(7)
+    B, K, S = 3, 257, 96
+    spec = (rng.uniform(0, 1, (B, K, S)) ** 4 * 100).astype(np.float32)
+    frames = np.array([96, 1, 50], np.int32)
+    ref = orc.mel_f32(spec, frames, 64, 16000.0, 100.0, 7000.0, formula, normalize)
+    out = torch.zeros((B, 64, S), device="cuda")
+    dims = np.stack([np.full(B, K), frames], 1).reshape(-1).astype(np.int32)
+    gpu.mel_filter_bank(handle, dev(spec), out, dev(dims), max_freq=7000.0, min_freq=100.0, formula=formula,
+                        num_filter=64, sample_rate=16000.0, normalize=normalize)
+    o = out.cpu().numpy()
+    # the GPU tables use correctly rounded FP32 log/exp; glibc's logf/expf differ from those by an
+    # ulp at a few HTK points, which moves a weight by ~1e-5 relative (Slaney goldens are exact)
+    for b, F in enumerate(frames):
+        a, r = o[b, :, :F], ref[b, :, :F]
+        assert (np.abs(a - r) / np.maximum(np.abs(r), 1e-30)).max() <= 2e-5
+
+
+def test_chain_workload0(gpu, handle, orc):
+    bt = synth.workload0()
+    db_ref, frames = orc.logmel_f32(bt.x, bt.lengths, nthreads=4)
+    db, fr = gpu.logmel_chain(handle, dev(bt.x), dev(bt.lengths))
+    assert np.array_equal(fr.numpy(), frames)
+    d = db.cpu().numpy()
+    worst = max(np.abs(d[b, :, :F] - db_ref[b, :, :F]).max() for b, F in enumerate(frames))
+    assert worst <= 4.5e-4, worst

@@ -1,0 +1,128 @@
+"""GPU parity of the drop-in rppt_*_gpu entry points (through the C-ABI) against the oracle and
+the reference's own golden vectors (utilities/test_suite/REFERENCE_OUTPUTS_AUDIO, batch 3).
+
+Tolerances:
+  pre-emphasis, to_decibels, mel (same input)  bit-exact (the reference's rounding is reproduced)
+  spectrogram                                  |err| / frame max <= 1e-5 and |err| < 1e-3
+                                               (the reference's own HIP QA cutoff,
+                                               utilities/test_suite/rpp_test_suite_audio.h:72-81)
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def frame_rel_err(out, ref, frames):
+    """max over clips of max |out - ref| / max_k ref[k, f] (NFT arrays [B, K, >=F])."""
+    worst = 0.0
+    for b, F in enumerate(frames):
+        a, r = out[b, :, :F].astype(np.float64), ref[b, :, :F].astype(np.float64)
+        fm = np.maximum(r.max(axis=0), 1e-30)
+        worst = max(worst, float((np.abs(a - r) / fm).max()))
+    return worst
+
+
+def test_pre_emphasis_golden(gpu, handle, golden):
+    x, L = golden["x"], golden["lengths"]
+    y = torch.zeros_like(dev(x))
+    gpu.pre_emphasis_filter(handle, dev(x), y, dev(L), dev(np.full(3, 0.97, np.float32)), "clamp")
+    y = y.cpu().numpy()
+    for b, n in enumerate(L):
+        assert np.array_equal(y[b, :n], golden["pre_emphasis"][b, :n]), b
+
+
+@pytest.mark.parametrize("border", ["zero", "clamp", "reflect"])
+def test_pre_emphasis_borders_ragged(gpu, handle, orc, border):
+    rng = np.random.default_rng(1)
+    L = np.array([1, 2, 3, 7, 8, 9, 1023, 1025, 4000], np.int32)
+    x = rng.uniform(-1, 1, (len(L), 4008)).astype(np.float32)
+    coeff = rng.uniform(0.5, 0.99, len(L)).astype(np.float32)
+    y = torch.full((len(L), 4008), 7.0, device="cuda")
+    gpu.pre_emphasis_filter(handle, dev(x), y, dev(L), dev(coeff), border)
+    ref = orc.pre_emphasis_f32(x, L, coeff, border)
+    y = y.cpu().numpy()
+    for b, n in enumerate(L):
+        assert np.array_equal(y[b, :n], ref[b, :n]), (border, b)
+        assert np.all(y[b, n:] == 7.0), "wrote past the clip length"
+
+
+def test_spectrogram_golden(gpu, handle, golden):
+    x, L, F = golden["x"], golden["lengths"], golden["frames"]
+    g = golden["spectrogram"]
+    out = torch.zeros((3, 257, g.shape[2]), device="cuda")
+    gpu.spectrogram(handle, dev(x), out, dev(L), nfft=512, window_length=320, window_step=160)
+    o = out.cpu().numpy()
+    assert frame_rel_err(o, g, F) <= 1e-5
+    for b, f in enumerate(F):
+        assert np.abs(o[b, :, :f] - g[b, :, :f]).max() < 1e-3
+
+
+def test_mel_golden(gpu, handle, golden):
+    spec, F = golden["spectrogram"], golden["frames"]
+    out = torch.zeros((3, 80, spec.shape[2]), device="cuda")
+    dims = np.stack([np.full(3, 257), F], 1).reshape(-1).astype(np.int32)
+    gpu.mel_filter_bank(handle, dev(spec), out, dev(dims), max_freq=8000.0, min_freq=0.0, num_filter=80,
+                        sample_rate=16000.0, normalize=True)
+    o, g = out.cpu().numpy(), golden["mel"]
+    for b, f in enumerate(F):
+        a, r = o[b, :, :f], g[b, :, :f]
+        rel = np.abs(a - r) / np.maximum(np.abs(r), 1e-30)
+        assert rel.max() <= 1e-6, (b, rel.max())
+        assert np.mean(a == r) > 0.999, "tables differ from the reference's"
+
+
+def test_to_decibels_golden(gpu, handle, golden):
+    x, L = golden["x"], golden["lengths"]
+    y = torch.zeros_like(dev(x))
+    dims = np.stack([np.ones(3), L], 1).reshape(-1).astype(np.uint32)
+    gpu.to_decibels(handle, dev(x), y, dev(dims.view(np.int32)), cutoff_db=float(np.float32(np.log(1e-20))),
+                    multiplier=float(np.float32(np.log(10))), reference_magnitude=1.0)
+    y = y.cpu().numpy()
+    for b, n in enumerate(L):
+        assert np.array_equal(y[b, :n], golden["to_decibels"][b, :n]), b
+
+
+@pytest.mark.parametrize("ref", [1.0, 0.0, 2.5])
+def test_to_decibels_2d_ref(gpu, handle, orc, ref):
+    rng = np.random.default_rng(2)
+    B, M, S = 4, 80, 130
+    x = (rng.uniform(0, 1, (B, M, S)) ** 6 * 50).astype(np.float32)
+    widths = np.array([130, 1, 64, 7], np.uint32)
+    heights = np.array([80, 80, 3, 80], np.uint32)
+    dims = np.stack([widths, heights], 1).reshape(-1).astype(np.uint32)
+    y = torch.zeros((B, M, S), device="cuda")
+    src_desc = gpu.desc_2d(B, M, S)
+    gpu.to_decibels(handle, dev(x), y, dev(dims.view(np.int32)), cutoff_db=-80.0, multiplier=10.0,
+                    reference_magnitude=ref, src_desc=src_desc, dst_desc=gpu.desc_2d(B, M, S))
+    y = y.cpu().numpy()
+    r = orc.to_db_f32(x, widths, heights, -80.0, 10.0, ref)
+    for b in range(B):
+        if widths[b] == 1:
+            a = y[b].reshape(-1)[:heights[b]]
+            e = r[b].reshape(-1)[:heights[b]]
+        else:
+            a, e = y[b, :heights[b], :widths[b]], r[b, :heights[b], :widths[b]]
+        assert np.array_equal(a, e), b
+
+
+def test_status_codes(gpu, handle):
+    x = torch.zeros((2, 64), device="cuda")
+    L = dev(np.array([64, 64], np.int32))
+    bad = gpu.desc_2d(2, 33, 5, layout=gpu.NHW)
+    assert gpu.spectrogram(handle, x, torch.zeros((2, 33, 5), device="cuda"), L, nfft=64, window_length=32,
+                           window_step=16, dst_desc=bad, check_status=False) == -10
+    d = gpu.desc_2d(2, 33, 5)
+    assert gpu.mel_filter_bank(handle, torch.zeros((2, 33, 5), device="cuda"), torch.zeros((2, 8, 5), device="cuda"),
+                               dev(np.array([33, 5, 33, 5], np.int32)), max_freq=9000.0, num_filter=8,
+                               sample_rate=16000.0, src_desc=d, check_status=False) == -2
+    assert gpu.to_decibels(handle, x, x, dev(np.array([1, 64, 1, 64], np.int32)), multiplier=0.0,
+                           check_status=False) == -4
