@@ -1,0 +1,193 @@
+"""Benchmark of the batched log-mel path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp16x3] [--workload 1]
+
+Workload (default 1): batch 512 mono float32 16 kHz clips, lengths log-uniform 0.5-20 s,
+N(0, 0.1^2) noise plus random chirps; pre-emphasis 0.97 (clamp), centered reflect-padded Hann
+frames W=400 hop 160 nfft 512, power 2, 80 Slaney mels 0-8 kHz normalised, dB (x10, ref 1, -80 dB).
+Inputs are resident in HBM before timing starts.
+
+A step is one pass of the hot path over the whole batch: the fused persistent launch
+(rppt_logmel_fused_gpu, planner + kernel) replayed from a hipGraph.  Printed as one JSON line:
+value = audio-seconds of the batch * steps / wall time of the timed steps.  Side figures (same
+run): the FP32 fused launch, the plain FP32 drop-in chain per clip (one launch per clip per stage,
+the 1x denominator), the error of each path against the FP64 oracle on a sample, the roofline of
+the fused kernel, and the CPU baseline (the oracle's RPP-HOST-equivalent FP32 chain, OpenMP).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
+MI355X_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "bf16", "fp32"])
+    ap.add_argument("--no-side", action="store_true", help="skip side measurements (profiling runs)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus > 1 is not supported "
+                                   "(clips shard across GPUs with no exchange; see DESIGN.md)"}))
+        return 2
+
+    import torch
+    from rpp_amd import api, synth
+
+    torch.cuda.set_device(0)
+    bt = synth.make(a.workload)
+    feat = bt.params
+    B = len(bt.lengths)
+    H = feat["H"]
+    frames = bt.lengths // H + 1
+    maxF = int(frames.max())
+    audio_s = bt.audio_seconds
+    log(f"workload {a.workload}: B={B} audio={audio_s:.1f}s frames={int(frames.sum())} maxF={maxF}")
+
+    x = torch.from_numpy(bt.x).cuda()
+    L = torch.from_numpy(bt.lengths).cuda()
+    out = torch.empty((B, feat["nmel"], maxF), dtype=torch.float32, device="cuda")
+    params = api.logmel_params(nfft=feat["nfft"], window_length=feat["W"], window_step=H,
+                               sample_rate=feat["sample_rate"], min_freq=feat["fmin"], max_freq=feat["fmax"],
+                               num_filter=feat["nmel"])
+    stream = torch.cuda.Stream()
+    h = api.Handle(B, stream=stream)
+
+    def fused_graph(precision):
+        with torch.cuda.stream(stream):
+            api.logmel_fused(h, x, out, L, params, precision)  # builds tables, sizes workspace
+            stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=stream):
+                api.logmel_fused(h, x, out, L, params, precision)
+        return g
+
+    def time_graph(g, steps, warmup):
+        for _ in range(warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            ev0.record(stream)
+            for _ in range(steps):
+                g.replay()
+            ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, ev0.elapsed_time(ev1) / 1e3
+
+    # ---------------- headline: mixed-precision fused launch under a graph
+    g = fused_graph(a.precision)
+    wall, gpu_s = time_graph(g, a.steps, a.warmup)
+    ms_per_step = wall / a.steps * 1e3
+    value = audio_s * a.steps / wall
+    head = out.clone()
+    log(f"fused {a.precision}: {ms_per_step:.3f} ms/step, {value:,.0f} audio-s/s (events {gpu_s / a.steps * 1e3:.3f} ms)")
+
+    # roofline of the dominant kernel (the fused kernel; the planner is ~us): algorithmic FLOPs
+    # (dense DFT 2*W*2K + dense mel 2*K*M per frame; split-product passes are overhead, not work)
+    K = feat["nfft"] // 2 + 1
+    flop_per_frame = 2 * feat["W"] * 2 * K + 2 * K * feat["nmel"]
+    flops = float(frames.sum()) * flop_per_frame
+    kernel_s = gpu_s / a.steps
+    achieved_tf = flops / kernel_s / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MI355X_PEAK_TFLOPS[a.precision],
+                "unit": "TFLOP/s", "frac": round(achieved_tf / MI355X_PEAK_TFLOPS[a.precision], 4),
+                "traffic": None, "flop_per_launch": flops, "kernel_ms": round(kernel_s * 1e3, 4),
+                "hbm_bytes_algorithmic": int(4 * bt.lengths.sum() + 4 * feat["nmel"] * frames.sum())}
+
+    side = {}
+    cpu = None
+    if not a.no_side:
+        from oracle import oracle
+
+        def max_db_err(o, r, fr):
+            return max(float(np.abs(o[b, :, :F].astype(np.float64) - r[b, :, :F]).max()) for b, F in enumerate(fr))
+        # FP32 fused
+        g32 = fused_graph("fp32")
+        w32, _ = time_graph(g32, max(3, a.steps // 4), 1)
+        side["fp32_fused_audio_s_per_s"] = round(audio_s * max(3, a.steps // 4) / w32, 1)
+        # plain FP32 drop-in chain: one launch per clip per stage (the 1x denominator), and batched
+        L_host = torch.from_numpy(bt.lengths).pin_memory()
+        out32 = torch.empty_like(out)
+        with torch.cuda.stream(stream):
+            for per_clip, key, reps in ((True, "fp32_per_clip_audio_s_per_s", 2), (False, "fp32_chain_batched_audio_s_per_s", 3)):
+                api.logmel_chain_gpu(h, x, out32, L_host, params, per_clip=per_clip)
+                stream.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    api.logmel_chain_gpu(h, x, out32, L_host, params, per_clip=per_clip)
+                stream.synchronize()
+                side[key] = round(audio_s * reps / (time.perf_counter() - t0), 1)
+        side["speedup_vs_fp32_per_clip"] = round(value / side["fp32_per_clip_audio_s_per_s"], 2)
+        # accuracy of the headline output on a sample of clips against the FP64 oracle
+        n_chk = min(B, 32)
+        kw = dict(nfft=feat["nfft"], W=feat["W"], H=H, nmel=feat["nmel"], sample_rate=feat["sample_rate"],
+                  fmax=feat["fmax"])
+        ref64, _, fr64 = oracle.logmel_f64(bt.x[:n_chk], bt.lengths[:n_chk], nthreads=os.cpu_count() or 1, **kw)
+        side["max_db_err_vs_fp64"] = round(max_db_err(head[:n_chk].cpu().numpy(), ref64, fr64), 6)
+        side["max_db_err_sample_clips"] = n_chk
+        # CPU baseline: the oracle's RPP-HOST-equivalent FP32 chain on the host cores
+        nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = min(B, 128)
+        oracle.logmel_f32(bt.x[:8], bt.lengths[:8], nthreads=nthr, **kw)
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.logmel_f32(bt.x[:sample], bt.lengths[:sample], nthreads=nthr, **kw)
+            reps.append(time.perf_counter() - t0)
+        cpu_as = float(bt.lengths[:sample].sum()) / feat["sample_rate"]
+        cpu = {"value": round(cpu_as / float(np.median(reps)), 1), "unit": "audio-s/s", "cores": nthr, "kind": "port",
+               "sample": f"first {sample} clips of the workload ({cpu_as:.0f} audio-s), median of 3 runs"}
+        log(f"side: {side}  cpu: {cpu}")
+
+    line = {
+        "metric": "log-mel audio-sec/sec (mixed-precision fused vs FP32, <=0.05 dB vs FP64)",
+        "value": round(value, 1),
+        "unit": "audio-s/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp16x3": "fp16 hi+lo split, fp32 accumulate", "bf16x3": "bf16 hi+lo split, fp32 accumulate",
+                  "bf16": "bf16, fp32 accumulate", "fp32": "fp32"}[a.precision],
+        "data": "synthetic (seeded SplitMix64)",
+        "config": {"workload": {0: "batch 8 mono 16 kHz 1-3 s", 1: "batch 512 ragged mono 16 kHz 0.5-20 s",
+                                2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],
+                   "batch": B, "audio_seconds": round(audio_s, 2), "frames": int(frames.sum()),
+                   "nfft": feat["nfft"], "window": feat["W"], "hop": H, "mels": feat["nmel"],
+                   "precision": a.precision, "launch": "planner + persistent fused kernel, hipGraph replay"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "side": side,
+    }
+    print(json.dumps(line))
+    h.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

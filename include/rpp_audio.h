@@ -218,6 +218,13 @@ RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
                                 Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
                                 RpptLogMelPrecision precision, rppHandle_t rppHandle);
 
+/* The four drop-in calls behind one entry point, intermediates in the handle's workspace:
+ * perClip != 0 issues them once per clip (one launch per clip per stage), else once for the batch.
+ * srcLengthTensor must be host-readable (host or pinned memory). */
+RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                Rpp32s *srcLengthTensor, const RpptLogMelParams *params, int perClip,
+                                rppHandle_t rppHandle);
+
 /* Tile shape of the fused launch; 0 fields mean "autotuned / default". */
 typedef struct {
     Rpp32s framesPerTile;   /* frames per work item: 32, 64 or 128 */

@@ -66,8 +66,8 @@ def lib():
         l.orc_mel_tables_f32.argtypes = [I, F, F, F, I, I, I, f32p, i32p, f32p, i32p]
         l.orc_mel_f32.argtypes = [f32p, L, L, i32p, I, f32p, L, L, F, F, I, I, F, I, I]
         l.orc_mel_f64.argtypes = [f64p, L, L, i32p, I, f64p, L, L, D, D, I, I, D, I, I]
-        l.orc_to_db_f32.argtypes = [f32p, L, L, f32p, L, L, u32p, I, F, F, F, I]
-        l.orc_to_db_f64.argtypes = [f64p, L, L, f64p, L, L, u32p, I, D, D, D]
+        l.orc_to_db_f32.argtypes = [f32p, L, L, f32p, L, L, u32p, I, F, F, F, I, I]
+        l.orc_to_db_f64.argtypes = [f64p, L, L, f64p, L, L, u32p, I, D, D, D, I]
         l.orc_down_mix_f32.argtypes = [f32p, L, f32p, L, i32p, I, I, I]
         l.orc_resample_f32.argtypes = [f32p, L, f32p, L, f32p, f32p, i32p, I, F, F, I, I, f32p, I]
         l.orc_windowed_sinc_lut.argtypes = [I, I, f32p, C.POINTER(F), C.POINTER(F)]
@@ -226,8 +226,10 @@ def mel_f64(spec, frames, nmel=80, sample_rate=16000.0, fmin=0.0, fmax=0.0, form
     return y
 
 
-def to_db_f32(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0, nthreads=1) -> np.ndarray:
-    """x: [B, rows, hStride] (or [B, stride] for 1-D data with widths == 1)."""
+def to_db_f32(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0, nthreads=1,
+              force2d=False) -> np.ndarray:
+    """x: [B, rows, hStride] (or [B, stride] for 1-D data with widths == 1).  force2d keeps
+    one-column clips 2-D instead of the reference's 1-D reading of them."""
     x = np.ascontiguousarray(x, np.float32)
     B = x.shape[0]
     if x.ndim == 2:
@@ -238,11 +240,11 @@ def to_db_f32(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0, nth
     dims[0::2] = np.asarray(widths, np.uint32)
     dims[1::2] = np.asarray(heights, np.uint32)
     y = np.zeros_like(x)
-    lib().orc_to_db_f32(x, ns, hs, y, ns, hs, dims, B, cutoff_db, multiplier, ref, nthreads)
+    lib().orc_to_db_f32(x, ns, hs, y, ns, hs, dims, B, cutoff_db, multiplier, ref, nthreads, int(force2d))
     return y
 
 
-def to_db_f64(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0) -> np.ndarray:
+def to_db_f64(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0, force2d=False) -> np.ndarray:
     x = np.ascontiguousarray(x, np.float64)
     B = x.shape[0]
     if x.ndim == 2:
@@ -253,7 +255,7 @@ def to_db_f64(x, widths, heights, cutoff_db=-80.0, multiplier=10.0, ref=1.0) -> 
     dims[0::2] = np.asarray(widths, np.uint32)
     dims[1::2] = np.asarray(heights, np.uint32)
     y = np.zeros_like(x)
-    lib().orc_to_db_f64(x, ns, hs, y, ns, hs, dims, B, cutoff_db, multiplier, ref)
+    lib().orc_to_db_f64(x, ns, hs, y, ns, hs, dims, B, cutoff_db, multiplier, ref, int(force2d))
     return y
 
 
@@ -312,6 +314,7 @@ def logmel_f32(x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000
                coeff=0.97, cutoff_db=-80.0, multiplier=10.0, ref=1.0, formula="slaney", normalize=True,
                nthreads=1):
     """RPP-HOST-equivalent chain: pre-emphasis -> spectrogram (NFT) -> mel -> to_decibels.
+    Single-frame clips are converted as 2-D (see to_db_f32's force2d).
 
     Returns (db [B, nmel, maxF], frames[B])."""
     lens = np.asarray(lengths, np.int32)
@@ -319,7 +322,7 @@ def logmel_f32(x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000
     spec = spectrogram_f32(pe, lens, nfft, W, H, True, True, 2, "NFT", nthreads=nthreads)
     frames = np.array([num_windows(int(l), W, H, True) for l in lens], np.int32)
     mel = mel_f32(spec, frames, nmel, sample_rate, fmin, fmax, formula, normalize, nthreads=nthreads)
-    db = to_db_f32(mel, frames, np.full(len(lens), nmel), cutoff_db, multiplier, ref, nthreads)
+    db = to_db_f32(mel, frames, np.full(len(lens), nmel), cutoff_db, multiplier, ref, nthreads, force2d=True)
     return db, frames
 
 
@@ -332,5 +335,5 @@ def logmel_f64(x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000
     spec = spectrogram_f64(pe, lens, nfft, W, H, True, True, 2, "NFT", nthreads=nthreads)
     frames = np.array([num_windows(int(l), W, H, True) for l in lens], np.int32)
     mel = mel_f64(spec, frames, nmel, sample_rate, fmin, fmax, formula, normalize, nthreads=nthreads)
-    db = to_db_f64(mel, frames, np.full(len(lens), nmel), cutoff_db, multiplier, ref)
+    db = to_db_f64(mel, frames, np.full(len(lens), nmel), cutoff_db, multiplier, ref, force2d=True)
     return db, mel, frames

@@ -501,9 +501,12 @@ ORC_API void orc_mel_f64(const double *src, long srcNStride, long srcHStride, co
 /* out = (mult*log10(2)) * log2(max(minRatio, x/ref)); dims[2b]=width,  */
 /* dims[2b+1]=height (RpptImagePatch order); width 1 -> 1-D of height.  */
 /* ------------------------------------------------------------------ */
+/* force2d: treat every clip as rows x columns even when width == 1 (the reference reads a
+ * one-column 2-D input as 1-D data of `height` contiguous samples, cpu/kernel/to_decibels.cpp:76-81,
+ * which mixes rows for a single-frame mel spectrogram; the whole-chain helpers avoid that). */
 ORC_API void orc_to_db_f32(const float *src, long srcNStride, long srcHStride,
                            float *dst, long dstNStride, long dstHStride, const unsigned *dims, int B,
-                           float cutOffDB, float multiplier, float referenceMagnitude, int nthreads)
+                           float cutOffDB, float multiplier, float referenceMagnitude, int nthreads, int force2d)
 {
     float minRatio = (float)pow(10.0, (double)(cutOffDB / multiplier));
     if (minRatio == 0.0f) minRatio = nextafterf(0.0f, 1.0f);
@@ -516,14 +519,15 @@ ORC_API void orc_to_db_f32(const float *src, long srcNStride, long srcHStride,
         float *y = dst + (long)b * dstNStride;
         unsigned width = dims[2 * b], height = dims[2 * b + 1];
         float ref = referenceMagnitude;
+        const int oneD = width == 1 && !force2d;
         if (referenceMagnitude == 0.0f) {
             ref = -FLT_MAX;
-            if (width == 1) { for (unsigned i = 0; i < height; i++) if (x[i] > ref) ref = x[i]; }
+            if (oneD) { for (unsigned i = 0; i < height; i++) if (x[i] > ref) ref = x[i]; }
             else for (unsigned r = 0; r < height; r++)
                 for (unsigned c = 0; c < width; c++) if (x[r * srcHStride + c] > ref) ref = x[r * srcHStride + c];
         }
         float inv = (ref != 0.0f) ? (1.f / ref) : 1.0f;
-        if (width == 1) {
+        if (oneD) {
             for (unsigned i = 0; i < height; i++) {
                 float v = x[i] * inv;
                 y[i] = mult * log2f(v > minRatio ? v : minRatio);
@@ -540,26 +544,27 @@ ORC_API void orc_to_db_f32(const float *src, long srcNStride, long srcHStride,
 
 ORC_API void orc_to_db_f64(const double *src, long srcNStride, long srcHStride,
                            double *dst, long dstNStride, long dstHStride, const unsigned *dims, int B,
-                           double cutOffDB, double multiplier, double referenceMagnitude)
+                           double cutOffDB, double multiplier, double referenceMagnitude, int force2d)
 {
     double minRatio = pow(10.0, cutOffDB / multiplier);
     for (int b = 0; b < B; b++) {
         const double *x = src + (long)b * srcNStride;
         double *y = dst + (long)b * dstNStride;
         unsigned width = dims[2 * b], height = dims[2 * b + 1];
+        const int oneD = width == 1 && !force2d;
         double ref = referenceMagnitude;
         if (referenceMagnitude == 0.0) {
             ref = -DBL_MAX;
             for (unsigned r = 0; r < height; r++)
                 for (unsigned c = 0; c < width; c++) {
-                    double v = width == 1 ? x[r] : x[r * srcHStride + c];
+                    double v = oneD ? x[r] : x[r * srcHStride + c];
                     if (v > ref) ref = v;
                 }
         }
         double inv = ref != 0.0 ? 1.0 / ref : 1.0;
         for (unsigned r = 0; r < height; r++)
             for (unsigned c = 0; c < width; c++) {
-                long si = width == 1 ? r : r * srcHStride + c, di = width == 1 ? r : r * dstHStride + c;
+                long si = oneD ? r : r * srcHStride + c, di = oneD ? r : r * dstHStride + c;
                 double v = x[si] * inv;
                 y[di] = multiplier * log10(v > minRatio ? v : minRatio);
             }

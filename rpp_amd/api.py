@@ -108,6 +108,7 @@ def lib():
             l.rppt_down_mixing_gpu.argtypes = [P, DP, P, DP, P, C.c_bool, P]
         if hasattr(l, "rppt_resample_gpu"):
             l.rppt_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.POINTER(RpptResamplingWindow), P]
+        l.rppt_logmel_chain_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         if hasattr(l, "rppt_logmel_fused_gpu"):
             l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
             l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
@@ -294,6 +295,19 @@ def logmel_fused(h: Handle, src, dst, lengths, params: RpptLogMelParams, precisi
                                      PRECISION[precision], h.h)
     if check_status:
         check("rppt_logmel_fused_gpu", st)
+    return st
+
+
+def logmel_chain_gpu(h: Handle, src, dst, lengths_host, params: RpptLogMelParams, per_clip=False,
+                     check_status=True) -> int:
+    """The 4-call chain through rppt_logmel_chain_gpu; lengths_host: int32 CPU tensor (pinned or not)."""
+    B, S = src.shape
+    sd = desc_1d(B, S)
+    dd = desc_2d(B, dst.shape[1], dst.shape[2])
+    st = lib().rppt_logmel_chain_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), lengths_host.data_ptr(),
+                                     C.byref(params), int(per_clip), h.h)
+    if check_status:
+        check("rppt_logmel_chain_gpu", st)
     return st
 
 

@@ -41,10 +41,14 @@ constexpr int kWave = 64;
 struct DftTables {
     int nfft = 0, W = 0, cols = 0;
     float *basis = nullptr;        // [W][cols] FP32
-    uint16_t *basisHi = nullptr;   // [cols][Wp] bf16 or fp16 hi part, column-major (K contiguous)
-    uint16_t *basisLo = nullptr;   // [cols][Wp] lo part
-    int Wp = 0;                    // W padded to the MFMA K step
-    int splitKind = -1;            // which 16-bit format basisHi/Lo hold (-1 none)
+    // Split copies for the MFMA path, in the operand order of a 32x32x16 MFMA: element (t, c)
+    // at ((t / 8) * colsAlloc + c) * 8 + t % 8, so the 32 lanes of a half-wave that load
+    // 8 consecutive t of 32 consecutive columns read one contiguous 512-byte run.
+    uint16_t *basisHi = nullptr;   // bf16 or fp16 hi part
+    uint16_t *basisLo = nullptr;   // lo part
+    int Wp = 0;                    // W padded to a multiple of 16 (zero rows)
+    int colsAlloc = 0;             // 512 or 1024 (zero columns beyond cols)
+    int splitKind = -1;            // which 16-bit format basisHi/Lo hold (-1 none, 0 bf16, 1 fp16)
 };
 
 // Mel tables for one parameter set, built on the GPU (tables.hip).
@@ -56,6 +60,10 @@ struct MelTables {
     float *normFactors = nullptr;   // [M]
     int *filterBins = nullptr;      // [M][2] first bin / one-past-last bin of filter m (gather form)
     float *dense = nullptr;         // [M][numBins] dense normalised weights (FP32)
+    // bf16 hi/lo split of `dense` in MFMA operand order: element (m, k) at ((k / 8) * Mpad + m) * 8 + k % 8
+    uint16_t *denseHi = nullptr, *denseLo = nullptr;
+    int KP = 0;                     // numBins padded to a multiple of 32
+    int Mpad = 0;                   // M padded to a multiple of 16
 };
 
 }  // namespace rppa
@@ -105,16 +113,16 @@ __host__ __device__ inline int num_windows(int length, int W, int H, bool center
     return length / H + 1;
 }
 
-// RPP's reflect index (cpu/kernel/spectrogram.cpp:55-69): mirror without repeating the edge.
-__host__ __device__ inline int idx_reflect(int loc, int lo, int hi)
+// RPP's reflect index over [0, L) (cpu/kernel/spectrogram.cpp:55-69): mirror about 0 and L-1
+// without repeating the edge, iterated.  That map is periodic with period 2(L-1), so it is
+// evaluated in closed form here; L < 2 maps everything to L-1 as the reference does.
+__host__ __device__ inline int idx_reflect(int loc, int L)
 {
-    if (hi - lo < 2) return hi - 1;
-    for (;;) {
-        if (loc < lo) loc = 2 * lo - loc;
-        else if (loc >= hi) loc = 2 * hi - 2 - loc;
-        else break;
-    }
-    return loc;
+    if (L < 2) return L - 1;
+    const int P = 2 * (L - 1);
+    int m = loc % P;
+    if (m < 0) m += P;
+    return m < L ? m : P - m;
 }
 
 }  // namespace rppa

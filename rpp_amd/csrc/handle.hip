@@ -69,6 +69,8 @@ RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
         (void)hipFree(kv.second.normFactors);
         (void)hipFree(kv.second.filterBins);
         (void)hipFree(kv.second.dense);
+        (void)hipFree(kv.second.denseHi);
+        (void)hipFree(kv.second.denseLo);
     }
     delete handle;
     return rppStatusSuccess;

@@ -1,0 +1,616 @@
+// logmel_fused.hip -- the whole log-mel chain over a ragged batch in one persistent launch.
+//
+// Equivalent to rppt_pre_emphasis_filter_gpu -> rppt_spectrogram_gpu (NFT, power) ->
+// rppt_mel_filter_bank_gpu -> rppt_to_decibels_gpu with matching arguments, without the three
+// HBM intermediates: per work item (FT frames of one clip) the kernel
+//   1. stages the clip segment those frames cover in LDS, applying pre-emphasis and the
+//      reflect/zero padding on the fly (HBM: the raw samples, read once);
+//   2. runs the DFT as a contraction of the implicit frame matrix [FT x W] with the
+//      window-folded basis [W x N] (tables.hip) -- on MFMA (bf16/fp16, split hi+lo,
+//      FP32 accumulation) or on FP32 FMA;
+//   3. forms power from the re/im column pairs into an LDS tile [FT x K];
+//   4. projects onto the mel filters -- MFMA (bf16 split hi+lo) or the exact FP32 gather;
+//   5. converts to dB and stores the NFT output (HBM: the dB values, written once).
+//
+// Work list: a planner kernel turns the clip lengths into a prefix sum of tiles per clip; the
+// persistent grid strides over the tiles.  Everything stays on the device, so the two launches
+// can be captured in a hipGraph and replayed for any batch living in the same buffers.
+//
+// LDS layouts.  Signal segment: sample i of the segment at i + (i / H) * 4 -- a 16-byte skew per
+// hop, so the 32 frames read by a half-wave start in different banks (frames are H apart and
+// H*4 bytes is a multiple of the bank period).  Power tile: [FT][KP + 4].
+#include "common.hpp"
+
+#include <cmath>
+
+namespace rppa {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kThreads = 256;
+constexpr int kSkew = 4;
+
+struct LogMelArgs {
+    const float *src;
+    uint32_t srcStride;
+    const int *lengths;
+    int B;
+    float *dst;
+    uint32_t dstNStride, hStride;
+    float coeff;
+    int border;
+    int N, W, H, center, reflect, power;
+    int K, M;
+    float minRatio, mult, invRef;
+    const int *tileStart;  // [B + 1]
+    // FP32 basis
+    const float *basis;
+    int cols;
+    // split basis
+    const uint16_t *bHi, *bLo;
+    int Wp, colsAlloc;
+    float basisScale;
+    // mel
+    const float *weightsDown;
+    const int *intervals;
+    const float *normFactors;
+    const int *filterBins;
+    int normalize;
+    const uint16_t *mHi, *mLo;
+    int KP, Mpad;
+};
+
+__host__ __device__ __forceinline__ int seg_pos(int i, int H) { return i + (i / H) * kSkew; }
+
+// Pre-emphasised sample at signal index j of a clip of length L, with the spectrogram's padding
+// rule applied to out-of-range indices (the chain pads the pre-emphasised signal).
+__device__ __forceinline__ float pe_sample(const float *x, int L, int j, float c, int border, int reflect)
+{
+    if (L <= 0) return 0.0f;
+    if (j < 0 || j >= L) {
+        if (!reflect) return 0.0f;
+        j = idx_reflect(j, L);
+    }
+    float prev;
+    if (j == 0) prev = (border == CLAMP) ? x[0] : (border == REFLECT && L > 1 ? x[1] : 0.0f);
+    else prev = x[j - 1];
+    float p = c * prev;
+    return x[j] - p;
+}
+
+// Finds the clip that owns `tile` (largest b with tileStart[b] <= tile).
+__device__ __forceinline__ int owner_clip(const int *tileStart, int B, int tile)
+{
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (tileStart[mid] <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float db_fast(float v, const LogMelArgs &a)
+{
+    float r = v * a.invRef;
+    float m = (a.minRatio < r) ? r : a.minRatio;
+    return a.mult * log2f(m);
+}
+
+__device__ __forceinline__ float db_exact(float v, const LogMelArgs &a)
+{
+    float r = v * a.invRef;
+    float m = (a.minRatio < r) ? r : a.minRatio;
+    return a.mult * static_cast<float>(log2(static_cast<double>(m)));
+}
+
+// ------------------------------------------------------------------------- planner
+// Single block: tiles per clip = ceil(F_b / FT) with F_b = min(frames(L_b), hStride); exclusive
+// prefix sum into tileStart[0..B], total in tileStart[B].
+__global__ void __launch_bounds__(1024)
+plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, uint32_t hStride, int W, int H,
+                  int center, int FT, int *__restrict__ tileStart)
+{
+    __shared__ int waveSum[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < B; base += 1024) {
+        const int b = base + tid;
+        int n = 0;
+        if (b < B) {
+            int L = min(lengths[b], static_cast<int>(srcStride));
+            int F = num_windows(max(L, 0), W, H, center != 0);
+            F = max(0, min(F, static_cast<int>(hStride)));
+            n = (F + FT - 1) / FT;
+        }
+        int v = n;  // inclusive wave scan
+        for (int o = 1; o < 64; o <<= 1) {
+            int u = __shfl_up(v, o);
+            if (lane >= o) v += u;
+        }
+        if (lane == 63) waveSum[wave] = v;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; w++) before += waveSum[w];
+        if (b < B) tileStart[b] = carry + before + v - n;
+        __syncthreads();
+        if (tid == 1023) carry += before + v;
+        __syncthreads();
+    }
+    if (tid == 0) tileStart[B] = carry;
+}
+
+// ------------------------------------------------------------------------- shared steps
+// Stage the segment of frames [f0, f0 + FT) into LDS (`span` samples); only the frames below F
+// are read from the clip, the rest of the span is zero.  Returns the block-wide max |sample| when
+// `wantMax` (for the fp16 scaling), else 0.
+template <bool wantMax>
+__device__ float stage_segment(float *seg, const LogMelArgs &a, const float *x, int L, int f0, int F, int span,
+                               float *red)
+{
+    const int off = a.center ? a.W / 2 : 0;
+    const int s0 = f0 * a.H - off;
+    const int nfr = max(0, F - f0);
+    const int valid = nfr > 0 ? min(span, (nfr - 1) * a.H + a.W) : 0;
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < span; i += kThreads) {
+        float v = i < valid ? pe_sample(x, L, s0 + i, a.coeff, a.border, a.reflect) : 0.0f;
+        seg[seg_pos(i, a.H)] = v;
+        if (wantMax) mx = fmaxf(mx, fabsf(v));
+    }
+    if (!wantMax) return 0.0f;
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    float r = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    return r;
+}
+
+// Mel projection + dB by the exact FP32 gather (same order as mel_filter_bank.hip), from the
+// power tile pw[FT][pws].
+__device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, int b, int f0, int F, int FT,
+                                 bool exactLog)
+{
+    const int nOut = a.M * FT;
+    for (int e = threadIdx.x; e < nOut; e += kThreads) {
+        const int m = e / FT, f = e % FT;
+        if (f0 + f >= F) continue;
+        const float norm = a.normFactors[m];
+        const int k0 = a.filterBins[2 * m], k1 = a.filterBins[2 * m + 1];
+        float acc = 0.0f;
+        for (int k = k0; k < k1; k++) {
+            const float wd = a.weightsDown[k];
+            float w = (a.intervals[k] == m) ? 1.0f - wd : wd;
+            if (a.normalize) w = w * norm;
+            const float p = pw[f * pws + k] * w;
+            acc = acc + p;
+        }
+        a.dst[static_cast<size_t>(b) * a.dstNStride + static_cast<size_t>(m) * a.hStride + f0 + f] =
+            exactLog ? db_exact(acc, a) : db_fast(acc, a);
+    }
+}
+
+// ------------------------------------------------------------------------- FP32 kernel
+// FT = 32 frames per tile; each thread owns 4 frames x 16 columns of a 512-column pass; the K
+// loop order (t ascending, fmaf) and every later step match spectrogram.hip / mel_filter_bank.hip /
+// to_decibels.hip exactly, so the output equals the 4-call chain bit for bit.
+constexpr int kFT32 = 32;
+constexpr int kKC = 16;
+constexpr int kCC = 512;
+
+__global__ void __launch_bounds__(kThreads, 1)
+logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int aliasPw)
+{
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    float *seg = lds;
+    float *sB = lds + segFloats;
+    float *pw = aliasPw ? sB : sB + bFloats;
+    const int pws = a.KP + kSkew;
+    const int tid = threadIdx.x;
+    const int cg = tid & 31, fg = tid >> 5;
+    const int T = a.tileStart[a.B];
+    const int nPasses = (a.cols + kCC - 1) / kCC;
+    const int span = (kFT32 - 1) * a.H + a.W;
+    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        const int b = owner_clip(a.tileStart, a.B, tile);
+        const int f0 = (tile - a.tileStart[b]) * kFT32;
+        const int L = min(a.lengths[b], static_cast<int>(a.srcStride));
+        const int F = min(num_windows(max(L, 0), a.W, a.H, a.center != 0), static_cast<int>(a.hStride));
+        const float *x = a.src + static_cast<size_t>(b) * a.srcStride;
+        stage_segment<false>(seg, a, x, L, f0, F, span, nullptr);
+        __syncthreads();
+        for (int pass = 0; pass < nPasses; pass++) {
+            const int cBase = pass * kCC;
+            float acc[4][16];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) acc[i][j] = 0.0f;
+            for (int t0 = 0; t0 < a.W; t0 += kKC) {
+                for (int e = tid; e < kKC * kCC; e += kThreads) {
+                    int tt = e / kCC, cc = e % kCC;
+                    int t = t0 + tt, c = cBase + cc;
+                    sB[e] = (t < a.W && c < a.cols) ? a.basis[static_cast<size_t>(t) * a.cols + c] : 0.0f;
+                }
+                __syncthreads();
+#pragma unroll 4
+                for (int kk = 0; kk < kKC; kk++) {
+                    const int t = t0 + kk;
+                    float av[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        av[i] = (t < a.W) ? seg[seg_pos((fg * 4 + i) * a.H + t, a.H)] : 0.0f;
+                    float bv[16];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) bv[j] = sB[kk * kCC + cg * 16 + j];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 16; j++) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
+                }
+                __syncthreads();
+            }
+            // power of this pass into pw[f][k]
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int f = fg * 4 + i;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const float re = acc[i][2 * j], im = acc[i][2 * j + 1];
+                    const int p = pass * (kCC / 2) + cg * 8 + j;
+                    if (p == 0) {
+                        float v = re * re, nq = im * im;
+                        if (a.power == 1) { v = sqrtf(v); nq = sqrtf(nq); }
+                        pw[f * pws + 0] = v;
+                        if (!(a.N & 1)) pw[f * pws + a.N / 2] = nq;
+                    } else if (p < a.K && !(!(a.N & 1) && p == a.N / 2)) {
+                        float v = re * re + im * im;
+                        if (a.power == 1) v = sqrtf(v);
+                        pw[f * pws + p] = v;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        mel_gather_store(pw, pws, a, b, f0, F, kFT32, true);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------- MFMA kernel
+enum { kPrecBF16x3 = 1, kPrecFP16x3 = 2, kPrecBF16 = 3 };
+
+__device__ __forceinline__ void split_bf16(const float *v, bf16x8 &hi, bf16x8 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        __bf16 h = static_cast<__bf16>(v[j]);
+        hi[j] = h;
+        lo[j] = static_cast<__bf16>(v[j] - static_cast<float>(h));
+    }
+}
+
+__device__ __forceinline__ void split_f16(const float *v, float s, f16x8 &hi, f16x8 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float u = v[j] * s;
+        _Float16 h = static_cast<_Float16>(u);
+        hi[j] = h;
+        lo[j] = static_cast<_Float16>(u - static_cast<float>(h));
+    }
+}
+
+template <int PREC, int FT, int COLS>
+__global__ void __launch_bounds__(kThreads, 2)
+logmel_mfma_kernel(LogMelArgs a)
+{
+    constexpr int CPW = COLS / 4;       // columns per wave
+    constexpr int RT = FT / 32;         // 32-row tiles
+    constexpr int CT = CPW / 32;        // 32-column tiles per wave
+    static_assert(RT * CT <= 8, "accumulator budget");
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ float red[4];
+    float *seg = lds;
+    float *pw = lds;                    // aliases the segment once the DFT is done
+    const int pws = a.KP + kSkew;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int T = a.tileStart[a.B];
+    const int span = (FT - 1) * a.H + a.Wp;
+    const bool evenN = !(a.N & 1);
+    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        const int b = owner_clip(a.tileStart, a.B, tile);
+        const int f0 = (tile - a.tileStart[b]) * FT;
+        const int L = min(a.lengths[b], static_cast<int>(a.srcStride));
+        const int F = min(num_windows(max(L, 0), a.W, a.H, a.center != 0), static_cast<int>(a.hStride));
+        const float *x = a.src + static_cast<size_t>(b) * a.srcStride;
+
+        float segScale = 1.0f;
+        if (PREC == kPrecFP16x3) {
+            float mx = stage_segment<true>(seg, a, x, L, f0, F, span, red);
+            if (mx > 0.0f) segScale = ldexpf(1.0f, 10 - ilogbf(mx));
+        } else {
+            stage_segment<false>(seg, a, x, L, f0, F, span, nullptr);
+        }
+        __syncthreads();
+
+        // ---- DFT on MFMA 32x32x16: acc[rt][ct] = frames (rt) x columns (ct) of this wave
+        f32x16 acc[RT][CT];
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rt][ct][r] = 0.0f;
+
+        const int colBase = wave * CPW + (lane & 31);
+        for (int k0 = 0; k0 < a.Wp; k0 += 16) {
+            const int tq = k0 + 8 * (lane >> 5);
+            bf16x8 ahb[RT], alb[RT];
+            f16x8 ahf[RT], alf[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                const int p = seg_pos((rt * 32 + (lane & 31)) * a.H + tq, a.H);
+                const float4 v0 = *reinterpret_cast<const float4 *>(&seg[p]);
+                const float4 v1 = *reinterpret_cast<const float4 *>(&seg[p + 4]);
+                const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                if (PREC == kPrecFP16x3) split_f16(v, segScale, ahf[rt], alf[rt]);
+                else split_bf16(v, ahb[rt], alb[rt]);
+            }
+            const size_t rowOff = static_cast<size_t>((k0 >> 3) + (lane >> 5)) * a.colsAlloc;
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const size_t o = (rowOff + colBase + ct * 32) * 8;
+                const u16x8 bh = *reinterpret_cast<const u16x8 *>(a.bHi + o);
+                if (PREC == kPrecFP16x3) {
+                    const u16x8 bl = *reinterpret_cast<const u16x8 *>(a.bLo + o);
+                    const f16x8 bhf = __builtin_bit_cast(f16x8, bh), blf = __builtin_bit_cast(f16x8, bl);
+#pragma unroll
+                    for (int rt = 0; rt < RT; rt++) {
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[rt], bhf, acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[rt], blf, acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alf[rt], bhf, acc[rt][ct], 0, 0, 0);
+                    }
+                } else {
+                    const bf16x8 bhb = __builtin_bit_cast(bf16x8, bh);
+                    if (PREC == kPrecBF16x3) {
+                        const u16x8 bl = *reinterpret_cast<const u16x8 *>(a.bLo + o);
+                        const bf16x8 blb = __builtin_bit_cast(bf16x8, bl);
+#pragma unroll
+                        for (int rt = 0; rt < RT; rt++) {
+                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], bhb, acc[rt][ct], 0, 0, 0);
+                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], blb, acc[rt][ct], 0, 0, 0);
+                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alb[rt], bhb, acc[rt][ct], 0, 0, 0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int rt = 0; rt < RT; rt++)
+                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], bhb, acc[rt][ct], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the segment is dead: pw may overwrite it
+
+        // ---- power: even lane of each (2k, 2k+1) column pair
+        const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const int c = colBase + ct * 32;
+                const int p = c >> 1;
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const float v = acc[rt][ct][r] * unscale;
+                    const float partner = __shfl_xor(v, 1);
+                    if (!(lane & 1)) {
+                        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        if (p == 0) {
+                            float dc = v * v, nq = partner * partner;
+                            if (a.power == 1) { dc = sqrtf(dc); nq = sqrtf(nq); }
+                            pw[row * pws] = dc;
+                            if (evenN) pw[row * pws + a.N / 2] = nq;
+                        } else if (p < a.K && !(evenN && p == a.N / 2)) {
+                            float pv = v * v + partner * partner;
+                            if (a.power == 1) pv = sqrtf(pv);
+                            pw[row * pws + p] = pv;
+                        }
+                    }
+                }
+            }
+        }
+        // zero the padding bins [K, KP) that the mel contraction reads
+        for (int e = tid; e < FT * (a.KP - a.K); e += kThreads) {
+            const int f = e / (a.KP - a.K), k = a.K + e % (a.KP - a.K);
+            pw[f * pws + k] = 0.0f;
+        }
+        __syncthreads();
+
+        // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16, split hi+lo
+        const int nMT = a.Mpad / 16, nFT = FT / 16, nTiles = nMT * nFT;
+        for (int tt = wave; tt < nTiles; tt += 4) {
+            const int mt = tt % nMT, ft = tt / nMT;
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int fr = ft * 16 + (lane & 15);
+            for (int k0 = 0; k0 < a.KP; k0 += 32) {
+                const size_t o = (static_cast<size_t>((k0 >> 3) + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
+                const bf16x8 wh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(a.mHi + o));
+                const bf16x8 wl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(a.mLo + o));
+                const int q = fr * pws + k0 + 8 * (lane >> 4);
+                const float4 v0 = *reinterpret_cast<const float4 *>(&pw[q]);
+                const float4 v1 = *reinterpret_cast<const float4 *>(&pw[q + 4]);
+                const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                bf16x8 ph, pl;
+                split_bf16(v, ph, pl);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, ph, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, pl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, ph, c, 0, 0, 0);
+            }
+            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
+            if (f0 + fr < F) {
+                float *out = a.dst + static_cast<size_t>(b) * a.dstNStride + f0 + fr;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m = mt * 16 + 4 * (lane >> 4) + r;
+                    if (m < a.M) out[static_cast<size_t>(m) * a.hStride] = db_fast(c[r], a);
+                }
+            }
+        }
+        __syncthreads();  // pw is reused by the next tile's segment
+    }
+}
+
+// ------------------------------------------------------------------------- dispatch
+struct Launch {
+    int FT = 32, COLS = 512;
+};
+
+}  // namespace
+}  // namespace rppa
+
+using namespace rppa;
+
+RPPA_EXPORT RppStatus rppt_logmel_set_tiles(rppHandle_t h, const RpptLogMelTiles *t)
+{
+    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (t->framesPerTile != 0 && t->framesPerTile != 32 && t->framesPerTile != 64) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (t->blocksPerCU < 0 || t->blocksPerCU > 8) return RPP_ERROR_INVALID_ARGUMENTS;
+    h->tiles = *t;
+    return RPP_SUCCESS;
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_get_tiles(rppHandle_t h, RpptLogMelTiles *t)
+{
+    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
+    *t = h->tiles;
+    return RPP_SUCCESS;
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                            RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                            const RpptLogMelParams *p, RpptLogMelPrecision precision,
+                                            rppHandle_t h)
+{
+    if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
+    if (dstDescPtr->layout != NFT) return RPP_ERROR_INVALID_DST_LAYOUT;
+    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (!srcPtr || !dstPtr || !srcLengthTensor) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (!p->multiplier) return RPP_ERROR_ZERO_DIVISION;
+    const int nfft = p->nfft ? p->nfft : p->windowLength;
+    if (p->windowLength <= 0 || p->windowStep <= 0 || nfft < 2 || p->windowLength > nfft || p->numFilter <= 0)
+        return RPP_ERROR_INVALID_ARGUMENTS;
+    if (p->maxFreq < 0 || p->maxFreq > p->sampleRate / 2 || p->minFreq < 0 || p->minFreq > p->sampleRate / 2)
+        return RPP_ERROR_INVALID_ARGUMENTS;
+    if (!(p->referenceMagnitude > 0.0f)) return RPP_ERROR_NOT_IMPLEMENTED;  // per-clip max needs a second pass
+    if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
+    const unsigned B = srcDescPtr->n;
+    if (B == 0) return RPP_SUCCESS;
+    if (dstDescPtr->h < static_cast<unsigned>(p->numFilter)) return RPP_ERROR_INVALID_DST_DIMS;
+    const bool mfma = precision != RPPT_LOGMEL_FP32;
+    const int K = nfft / 2 + 1;
+    const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
+    if (mfma && ((p->windowStep % 8) != 0 || nfft > 1024)) return RPP_ERROR_NOT_IMPLEMENTED;
+
+    const DftTables *dt = get_dft_tables(h, nfft, p->windowLength, nullptr, splitKind);
+    const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
+                                         p->numFilter, p->normalize ? 1 : 0);
+    if (!dt || !mt) return RPP_ERROR;
+    int *tileStart = static_cast<int *>(workspace(h, sizeof(int) * (B + 1)));
+    if (!tileStart) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+
+    LogMelArgs a{};
+    a.src = static_cast<const float *>(srcPtr);
+    a.srcStride = srcDescPtr->strides.nStride;
+    a.lengths = srcLengthTensor;
+    a.B = static_cast<int>(B);
+    a.dst = static_cast<float *>(dstPtr);
+    a.dstNStride = dstDescPtr->strides.nStride;
+    a.hStride = dstDescPtr->strides.hStride;
+    a.coeff = p->preEmphasisCoeff;
+    a.border = static_cast<int>(p->borderType);
+    a.N = nfft;
+    a.W = p->windowLength;
+    a.H = p->windowStep;
+    a.center = p->centerWindows;
+    a.reflect = p->reflectPadding;
+    a.power = p->power;
+    a.K = K;
+    a.M = p->numFilter;
+    float minRatio = static_cast<float>(std::pow(10.0, static_cast<double>(p->cutOffDB / p->multiplier)));
+    if (minRatio == 0.0f) minRatio = std::nextafter(0.0f, 1.0f);
+    a.minRatio = minRatio;
+    a.mult = p->multiplier * 0.3010299956639812f;
+    a.invRef = 1.f / p->referenceMagnitude;
+    a.tileStart = tileStart;
+    a.basis = dt->basis;
+    a.cols = dt->cols;
+    a.bHi = dt->basisHi;
+    a.bLo = dt->basisLo;
+    a.Wp = dt->Wp;
+    a.colsAlloc = dt->colsAlloc;
+    a.basisScale = splitKind == 1 ? 4096.0f : 1.0f;
+    a.weightsDown = mt->weightsDown;
+    a.intervals = mt->intervals;
+    a.normFactors = mt->normFactors;
+    a.filterBins = mt->filterBins;
+    a.normalize = p->normalize ? 1 : 0;
+    a.mHi = mt->denseHi;
+    a.mLo = mt->denseLo;
+    a.KP = mt->KP;
+    a.Mpad = mt->Mpad;
+
+    // tile shape
+    int FT;
+    if (!mfma) FT = 32;
+    else if (dt->colsAlloc > 512) FT = 32;
+    else FT = (h->tiles.framesPerTile == 64) ? 64 : 32;
+    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 2;
+    const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
+
+    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, srcLengthTensor, a.B, a.srcStride,
+                       a.hStride, a.W, a.H, a.center, FT, tileStart);
+    const int pws = a.KP + kSkew;
+    if (!mfma) {
+        const int span = (kFT32 - 1) * a.H + a.W;
+        const int segFloats = (seg_pos(span - 1, a.H) + 1 + 3) / 4 * 4;
+        const int bFloats = kKC * kCC;
+        const int pwFloats = kFT32 * pws;
+        const int passes = (a.cols + kCC - 1) / kCC;
+        const int alias = passes == 1 ? 1 : 0;
+        const size_t bytes = sizeof(float) * (segFloats + (alias ? std::max(bFloats, pwFloats) : bFloats + pwFloats));
+        if (bytes > 160 * 1024) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
+        hipLaunchKernelGGL(logmel_fp32_kernel, dim3(grid), dim3(kThreads), bytes, h->stream, a, segFloats, bFloats,
+                           pwFloats, alias);
+    } else {
+        const int span = (FT - 1) * a.H + a.Wp;
+        const int segFloats = seg_pos(span - 1, a.H) + 1 + 8;
+        const int pwFloats = FT * pws;
+        const size_t bytes = sizeof(float) * std::max(segFloats, pwFloats);
+        if (bytes > 160 * 1024) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
+        const int cols = dt->colsAlloc;
+#define RPPA_LAUNCH(PR)                                                                                     \
+        do {                                                                                                \
+            if (FT == 64 && cols == 512)                                                                     \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 512>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+            else if (cols == 512)                                                                            \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 512>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+            else                                                                                             \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 1024>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+        } while (0)
+        if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
+        else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);
+        else RPPA_LAUNCH(kPrecBF16);
+#undef RPPA_LAUNCH
+    }
+    RPPA_HIP(hipGetLastError());
+    return RPP_SUCCESS;
+}

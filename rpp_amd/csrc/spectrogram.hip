@@ -44,7 +44,7 @@ __device__ inline float sample_at(const float *x, int L, int i, int reflect)
 {
     if (i >= 0 && i < L) return x[i];
     if (L <= 0) return 0.0f;
-    if (reflect) return x[idx_reflect(i, 0, L)];
+    if (reflect) return x[idx_reflect(i, L)];
     return 0.0f;
 }
 

@@ -47,16 +47,18 @@ __device__ inline float f16_to_f32(uint16_t u)
     return static_cast<float>(h);
 }
 
-// One thread per (t, c).  basis[t][c] in FP32; optional split copies, column-major [c][Wp].
-__global__ void dft_basis_kernel(int N, int W, int cols, int Wp, const float *__restrict__ win,
+// One thread per (t, c) of the padded [Wp][colsAlloc] grid.  basis[t][c] (t < W, c < cols) in FP32;
+// optional split copies in MFMA operand order (see common.hpp).
+__global__ void dft_basis_kernel(int N, int W, int cols, int Wp, int colsAlloc, const float *__restrict__ win,
                                  float *__restrict__ basis, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
                                  int splitKind, float splitScale)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int t = blockIdx.y;
-    if (c >= cols || t >= Wp) return;
+    int cMax = splitKind >= 0 ? colsAlloc : cols;
+    if (c >= cMax || t >= (splitKind >= 0 ? Wp : W)) return;
     double v = 0.0;
-    if (t < W) {
+    if (t < W && c < cols) {
         double w = static_cast<double>(win[t]);
         int k;
         bool isSin = false;
@@ -80,9 +82,25 @@ __global__ void dft_basis_kernel(int N, int W, int cols, int Wp, const float *__
             h = f32_to_f16_rne(static_cast<float>(s));
             l = f32_to_f16_rne(static_cast<float>(s - f16_to_f32(h)));
         }
-        hi[static_cast<size_t>(c) * Wp + t] = h;
-        lo[static_cast<size_t>(c) * Wp + t] = l;
+        size_t o = (static_cast<size_t>(t / 8) * colsAlloc + c) * 8 + (t % 8);
+        hi[o] = h;
+        lo[o] = l;
     }
+}
+
+// bf16 hi/lo split of the dense mel matrix in MFMA operand order (see common.hpp).
+__global__ void mel_split_kernel(const float *__restrict__ dense, int M, int numBins, int Mpad, int KP,
+                                 uint16_t *__restrict__ hi, uint16_t *__restrict__ lo)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (m >= Mpad || k >= KP) return;
+    float v = (m < M && k < numBins) ? dense[static_cast<size_t>(m) * numBins + k] : 0.0f;
+    uint16_t h = f32_to_bf16_rne(v);
+    uint16_t l = f32_to_bf16_rne(v - bf16_to_f32(h));
+    size_t o = (static_cast<size_t>(k / 8) * Mpad + m) * 8 + (k % 8);
+    hi[o] = h;
+    lo[o] = l;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -197,7 +215,8 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
         t.nfft = nfft;
         t.W = W;
         t.cols = nfft + (nfft & 1);
-        t.Wp = (W + 31) / 32 * 32;
+        t.Wp = (W + 15) / 16 * 16;
+        t.colsAlloc = t.cols <= 512 ? 512 : 1024;
     }
     float *win = nullptr;
     if (hipMalloc(&win, sizeof(float) * W) != hipSuccess) return nullptr;
@@ -209,13 +228,15 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
     if (!t.basis && hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * t.cols) != hipSuccess) return nullptr;
     if (splitKind >= 0) {
         if (t.basisHi) { (void)hipFree(t.basisHi); (void)hipFree(t.basisLo); }
-        size_t n = static_cast<size_t>(t.cols) * t.Wp;
+        size_t n = static_cast<size_t>(t.colsAlloc) * t.Wp;
         if (hipMalloc(&t.basisHi, n * 2) != hipSuccess || hipMalloc(&t.basisLo, n * 2) != hipSuccess) return nullptr;
         t.splitKind = splitKind;
     }
     float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
-    hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(t.cols, 256), splitKind >= 0 ? t.Wp : W), dim3(256), 0,
-                       h->stream, nfft, W, t.cols, t.Wp, win, t.basis, t.basisHi, t.basisLo, splitKind, splitScale);
+    hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(splitKind >= 0 ? t.colsAlloc : t.cols, 256),
+                       splitKind >= 0 ? t.Wp : W), dim3(256), 0,
+                       h->stream, nfft, W, t.cols, t.Wp, t.colsAlloc, win, t.basis, t.basisHi, t.basisLo, splitKind,
+                       splitScale);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
     (void)hipFree(win);
     h->dft[k] = t;
@@ -246,6 +267,12 @@ const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, flo
         return nullptr;
     hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(64), 0, h->stream, numBins, sampleRate, maxFreq, minFreq,
                        formula, M, normalize, t.weightsDown, t.intervals, t.normFactors, t.filterBins, t.dense, range);
+    t.KP = (numBins + 31) / 32 * 32;
+    t.Mpad = (M + 15) / 16 * 16;
+    size_t ns = static_cast<size_t>(t.KP) * t.Mpad;
+    if (hipMalloc(&t.denseHi, ns * 2) != hipSuccess || hipMalloc(&t.denseLo, ns * 2) != hipSuccess) return nullptr;
+    hipLaunchKernelGGL(mel_split_kernel, dim3(ceil_div(t.Mpad, 64), t.KP), dim3(64), 0, h->stream, t.dense, M,
+                       numBins, t.Mpad, t.KP, t.denseHi, t.denseLo);
     int hr[2] = {0, 0};
     if (hipMemcpyAsync(hr, range, sizeof(hr), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return nullptr;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;

@@ -1,0 +1,162 @@
+"""GPU parity of the fused persistent log-mel launch (rppt_logmel_fused_gpu) through the C-ABI.
+
+  FP32 mode   bit-identical to the 4-call drop-in chain (same rounding points); against the FP32
+              oracle (the RPP HOST restatement) 1e-4 relative in mel power with a floor at 1e-6 of
+              the frame's loudest band (helpers.fp32_bar)
+  fp16x3      max |dB - FP64 oracle dB| <= 0.05 dB over every output (values clamp at -80 dB)
+  bf16x3      error reported against the FP64 oracle; meets 0.05 dB on workload 0 only (a two-term
+              bf16 split keeps ~16 mantissa bits), so its check on workloads 1/2 is the reported figure
+  bf16        one product, reported only
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import fp32_bar, max_db_err, max_rel_power_err
+from rpp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+DB_TOL_MIXED = 0.05
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def params_for(gpu, feat):
+    return gpu.logmel_params(nfft=feat["nfft"], window_length=feat["W"], window_step=feat["H"],
+                             sample_rate=feat["sample_rate"], min_freq=feat["fmin"], max_freq=feat["fmax"],
+                             num_filter=feat["nmel"])
+
+
+def run_fused(gpu, h, x, L, feat, precision, hstride=None, fill=np.nan):
+    frames = L // feat["H"] + 1
+    hs = int(frames.max()) if hstride is None else hstride
+    out = torch.full((len(L), feat["nmel"], hs), float(fill), device="cuda")
+    gpu.logmel_fused(h, dev(x), out, dev(L), params_for(gpu, feat), precision)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), frames
+
+
+def test_fused_fp32_equals_chain(gpu, handle):
+    bt = synth.workload0()
+    L = bt.lengths
+    db_chain, fr = gpu.logmel_chain(handle, dev(bt.x), dev(L))
+    out, frames = run_fused(gpu, handle, bt.x, L, bt.params, "fp32")
+    d = db_chain.cpu().numpy()
+    for b, F in enumerate(frames):
+        assert np.array_equal(out[b, :, :F], d[b, :, :F]), b
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_fused_mixed_workload0_vs_fp64(gpu, handle, orc, precision):
+    bt = synth.workload0()
+    ref, _, frames = orc.logmel_f64(bt.x, bt.lengths, nthreads=8)
+    out, fr = run_fused(gpu, handle, bt.x, bt.lengths, bt.params, precision)
+    err = max_db_err(out, ref, frames)
+    print(f"{precision} workload0 max |dB err| vs FP64 = {err:.3e}")
+    assert err <= DB_TOL_MIXED
+
+
+def check_against_oracles(orc, out, x, L, precision, name, **kw):
+    """FP32 mode: fp32_bar against the FP32 oracle; fp16x3: <= 0.05 dB from the FP64 oracle;
+    bf16x3: reported.  All figures are printed."""
+    ref64, _, frames = orc.logmel_f64(x, L, nthreads=8, **kw)
+    ref32, _ = orc.logmel_f32(x, L, nthreads=8, **kw)
+    e64 = max_db_err(out, ref64, frames)
+    e32 = max_db_err(out, ref32, frames)
+    bar = fp32_bar(out, ref32, frames)
+    rel64 = max_rel_power_err(out, ref64, frames)
+    print(f"{precision} {name}: max |dB err| vs FP64 = {e64:.3e} (max rel power {rel64:.2e}), "
+          f"vs FP32 oracle = {e32:.3e} (fp32 bar {bar:.3f})")
+    if precision == "fp32":
+        assert bar <= 1.0
+    elif precision == "fp16x3":
+        assert e64 <= DB_TOL_MIXED
+    return e64
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+def test_fused_workload1_subset(gpu, handle, orc, precision):
+    bt = synth.workload1(batch=24)
+    out, fr = run_fused(gpu, handle, bt.x, bt.lengths, bt.params, precision)
+    check_against_oracles(orc, out, bt.x, bt.lengths, precision, "workload1[:24]")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+def test_fused_workload2_subset(gpu, handle, orc, precision):
+    bt = synth.workload2(batch=6)
+    p = bt.params
+    kw = dict(nfft=p["nfft"], W=p["W"], H=p["H"], nmel=p["nmel"], sample_rate=p["sample_rate"], fmax=p["fmax"])
+    out, fr = run_fused(gpu, handle, bt.x, bt.lengths, p, precision)
+    check_against_oracles(orc, out, bt.x, bt.lengths, precision, "workload2[:6]", **kw)
+
+
+def test_fused_bf16_single_reports(gpu, handle, orc):
+    bt = synth.workload0(batch=4)
+    ref, _, frames = orc.logmel_f64(bt.x, bt.lengths)
+    out, fr = run_fused(gpu, handle, bt.x, bt.lengths, bt.params, "bf16")
+    err = max_db_err(out, ref, frames)
+    print(f"bf16 (one product) workload0 max |dB err| vs FP64 = {err:.3e}")
+    assert np.isfinite(err)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
+def test_fused_ragged_edges(gpu, handle, orc, precision):
+    """Tiny and boundary lengths; cells beyond each clip's frames stay untouched."""
+    rng = np.random.default_rng(11)
+    L = np.array([1, 2, 159, 160, 399, 400, 401, 10239, 10240, 20481, 3], np.int32)
+    x = np.zeros((len(L), int(L.max()) + 8), np.float32)
+    for b, n in enumerate(L):
+        x[b, :n] = rng.uniform(-1, 1, n)
+    feat = synth.FEATURES_16K
+    hs = int((L // 160 + 1).max()) + 5
+    out, frames = run_fused(gpu, handle, x, L, feat, precision, hstride=hs, fill=-12345.0)
+    ref32, _ = orc.logmel_f32(x, L)
+    ref64, _, _ = orc.logmel_f64(x, L)
+    for b, F in enumerate(frames):
+        assert np.all(out[b, :, F:] == -12345.0), b
+    if precision == "fp32":
+        assert fp32_bar(out, ref32, frames) <= 1.0
+    else:
+        assert max_db_err(out, ref64, frames) <= DB_TOL_MIXED
+
+
+def test_fused_graph_replay(gpu, orc):
+    """The planner + persistent kernel captured in a graph give the eager result on every replay."""
+    bt = synth.workload0()
+    x, L = dev(bt.x), dev(bt.lengths)
+    frames = bt.lengths // 160 + 1
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        h = gpu.Handle(8, stream=s)
+        out = torch.zeros((8, 80, int(frames.max())), device="cuda")
+        p = params_for(gpu, bt.params)
+        gpu.logmel_fused(h, x, out, L, p, "bf16x3")  # warm-up: builds tables, sizes workspace
+        s.synchronize()
+        eager = out.clone()
+        g = torch.cuda.CUDAGraph()
+        out.zero_()
+        with torch.cuda.graph(g, stream=s):
+            gpu.logmel_fused(h, x, out, L, p, "bf16x3")
+    for i in range(4):
+        out.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for b, F in enumerate(frames):
+            assert torch.equal(out[b, :, :F], eager[b, :, :F]), (i, b)
+    h.close()
+
+
+def test_fused_status_codes(gpu, handle):
+    x = torch.zeros((2, 64), device="cuda")
+    L = dev(np.array([64, 64], np.int32))
+    out = torch.zeros((2, 80, 1), device="cuda")
+    p = gpu.logmel_params(multiplier=0.0)
+    assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -4
+    p = gpu.logmel_params(reference_magnitude=0.0)
+    assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -6
+    p = gpu.logmel_params(max_freq=9000.0)
+    assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -2
