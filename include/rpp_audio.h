@@ -218,6 +218,14 @@ RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
                                 Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
                                 RpptLogMelPrecision precision, rppHandle_t rppHandle);
 
+/* Packed variant for serving: clip b's samples at srcPtr + srcOffsets[b] (srcLengthTensor[b] of
+ * them), its dB output as a dense [numFilter][frames_b] block at dstPtr + dstOffsets[b], with
+ * frames_b = L/H + 1 (centered) or (L-W)/H + 1.  Offsets and lengths in device or pinned memory. */
+RppStatus rppt_logmel_fused_packed_gpu(const Rpp32f *srcPtr, const long long *srcOffsets, const Rpp32s *srcLengthTensor,
+                                       Rpp32s batchSize, Rpp32f *dstPtr, const long long *dstOffsets,
+                                       const RpptLogMelParams *params, RpptLogMelPrecision precision,
+                                       rppHandle_t rppHandle);
+
 /* The four drop-in calls behind one entry point, intermediates in the handle's workspace:
  * perClip != 0 issues them once per clip (one launch per clip per stage), else once for the batch.
  * srcLengthTensor must be host-readable (host or pinned memory). */
@@ -232,6 +240,36 @@ typedef struct {
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);
 RppStatus rppt_logmel_get_tiles(rppHandle_t rppHandle, RpptLogMelTiles *tiles);
+
+/* ======================= serving (no counterpart in the reference) ======================= */
+/* A continuous-batching request loop over rppt_logmel_fused_packed_gpu on the current device:
+ * host threads submit clips, a scheduler thread batches them onto a pool of device slots (each a
+ * stream with device-resident buffers) and stamps each request's completion time. */
+typedef struct rppaServer rppaServer;
+typedef struct {
+    int maxBatch;             /* clips per batch (upper bound) */
+    int deadlineUs;           /* longest a clip waits for its batch to close while a slot is free */
+    long long memBudgetBytes; /* device bytes for all slots (0: 80% of free HBM, at most 8 GiB) */
+    int numSlots;             /* concurrent batches (streams), 1..16 */
+    int eager;                /* close a batch as soon as a slot is idle */
+    int keepOutputs;          /* keep dB rows for rppa_server_wait (else timings only) */
+} rppaServeConfig;
+typedef struct {
+    long long submitted, completed, refused, batches;
+    double p50Us, p90Us, p99Us, meanUs, maxUs;   /* arrival -> result in host memory */
+    double meanBatchClips, audioSamples, gpuBusySeconds;
+    long long slotCapacitySamples;
+} rppaServeStats;
+RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLogMelPrecision precision,
+                             const rppaServeConfig *config, rppaServer **server);
+RppStatus rppa_server_destroy(rppaServer *server);
+/* copies `length` host samples; RPP_ERROR_NOT_ENOUGH_MEMORY when the clip can never fit a slot */
+RppStatus rppa_server_submit(rppaServer *server, const float *samples, int length, long long *id);
+/* blocks until `id` is done; out receives [numFilter][frames] dB rows when outputs are kept */
+RppStatus rppa_server_wait(rppaServer *server, long long id, float *out, long long outCapacity, int *frames,
+                           double *latencyUs);
+RppStatus rppa_server_drain(rppaServer *server, int timeoutMs);
+RppStatus rppa_server_stats(rppaServer *server, rppaServeStats *stats);
 
 #ifdef __cplusplus
 }

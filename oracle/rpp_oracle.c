@@ -349,6 +349,9 @@ static float slaney_mel_to_hz_f(float mel)
     if (mel >= slaney_min_log_mel()) return SL_MIN_LOG_HZ * expf(SL_STEP_LOG * (mel - slaney_min_log_mel()));
     return 0.0f + mel * SL_FSP;
 }
+/* HTK (rppdefs.h:812-818), IEEE evaluation.  The reference library is built with -Ofast, whose
+ * rewriting of this expression is not reproduced (Slaney, the path the workloads use, has no
+ * division and matches bit for bit). */
 static float htk_hz_to_mel_f(float hz) { return 1127.0f * logf(1.0f + (hz / 700.0f)); }
 static float htk_mel_to_hz_f(float mel) { return 700.0f * (expf(mel / 1127.0f) - 1.0f); }
 

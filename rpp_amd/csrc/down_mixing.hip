@@ -1,0 +1,84 @@
+// down_mixing.hip -- rppt_down_mixing_gpu (api/rppt_tensor_audio_augmentations.h:188).
+//
+// mono[t] = sum_c x[t*C + c] * w_c with w_c = 1/C (renormalised to unit sum when
+// normalizeWeights), srcDimsTensor[2b] = samples, [2b+1] = channels, interleaved input
+// (cpu/kernel/down_mixing.cpp:28-122).  The summation order is the HOST path's: channels in
+// groups of 4 (C < 8) or 8 (C >= 8) accumulate lane-wise, the lanes are added as its horizontal
+// add does ((l0+l1)+(l2+l3); eight lanes ((l0+l4)+(l2+l6))+((l1+l5)+(l3+l7))), then the leftover
+// channels one by one -- so the output is bit-identical to RPP HOST.  (RPP's HIP kernel ignores
+// normalizeWeights and sums in a different order, hip/kernel/down_mixing.cpp:28-70.)
+// One thread per output sample; HBM-bound: 4*(C+1) bytes per sample.
+#include "common.hpp"
+
+namespace rppa {
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads)
+down_mix_kernel(const float *__restrict__ src, uint32_t srcStride, float *__restrict__ dst, uint32_t dstStride,
+                const int *__restrict__ dims, int normalize)
+{
+    const int b = blockIdx.y;
+    const int C = dims[2 * b + 1];
+    const long long t = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (C <= 0) return;
+    const long long S = min(static_cast<long long>(dims[2 * b]),
+                            min(static_cast<long long>(srcStride) / C, static_cast<long long>(dstStride)));
+    if (t >= S) return;
+    const float *x = src + static_cast<size_t>(b) * srcStride + t * C;
+    float *y = dst + static_cast<size_t>(b) * dstStride;
+    if (C == 1) { y[t] = x[0]; return; }
+    float w = 1.f / C;
+    if (normalize) {
+        float sum = 0.0f;
+        for (int c = 0; c < C; c++) sum = sum + w;
+        float inv = static_cast<float>(1.0 / sum);
+        w = w * inv;
+    }
+    int c = 0;
+    float s;
+    if (C > 7) {
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int aligned = (C / 8) * 8;
+        for (; c < aligned; c += 8) {
+#pragma unroll
+            for (int l = 0; l < 8; l++) { float p = x[c + l] * w; acc[l] = acc[l] + p; }
+        }
+        float s0 = acc[0] + acc[4], s1 = acc[1] + acc[5], s2 = acc[2] + acc[6], s3 = acc[3] + acc[7];
+        s = (s0 + s2) + (s1 + s3);
+    } else {
+        float acc[4] = {0, 0, 0, 0};
+        const int aligned = (C / 4) * 4;
+        for (; c < aligned; c += 4) {
+#pragma unroll
+            for (int l = 0; l < 4; l++) { float p = x[c + l] * w; acc[l] = acc[l] + p; }
+        }
+        s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+    for (; c < C; c++) { float p = x[c] * w; s = s + p; }
+    y[t] = s;
+}
+
+}  // namespace
+}  // namespace rppa
+
+// Validation follows rppt_tensor_audio_augmentations.cpp:416-448.
+RPPA_EXPORT RppStatus rppt_down_mixing_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                           RpptDescPtr dstDescPtr, Rpp32s *srcDimsTensor, bool normalizeWeights,
+                                           rppHandle_t rppHandle)
+{
+    if (!rppHandle || !srcDescPtr || !dstDescPtr) return RPP_ERROR_INVALID_ARGUMENTS;
+    const size_t dims = srcDescPtr->numDims - 1;
+    if (dims != 1 && dims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
+    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (!srcPtr || !dstPtr || !srcDimsTensor) return RPP_ERROR_INVALID_ARGUMENTS;
+    const unsigned B = srcDescPtr->n;
+    if (B == 0) return RPP_SUCCESS;
+    dim3 grid(rppa::ceil_div(dstDescPtr->strides.nStride, rppa::kThreads), B);
+    hipLaunchKernelGGL(rppa::down_mix_kernel, grid, dim3(rppa::kThreads), 0, rppHandle->stream,
+                       static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, static_cast<float *>(dstPtr),
+                       dstDescPtr->strides.nStride, srcDimsTensor, normalizeWeights ? 1 : 0);
+    RPPA_HIP(hipGetLastError());
+    return RPP_SUCCESS;
+}

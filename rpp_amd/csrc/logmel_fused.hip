@@ -48,6 +48,9 @@ struct LogMelArgs {
     int K, M;
     float minRatio, mult, invRef;
     const int *tileStart;  // [B + 1]
+    // packed layout (both null for the [n][nStride] / [n][M][hStride] descriptors): clip b's samples
+    // at src + srcOff[b], its dB rows at dst + dstOff[b] as a dense [M][frames_b] block
+    const long long *srcOff, *dstOff;
     // FP32 basis
     const float *basis;
     int cols;
@@ -66,6 +69,37 @@ struct LogMelArgs {
 };
 
 __host__ __device__ __forceinline__ int seg_pos(int i, int H) { return i + (i / H) * kSkew; }
+
+// Where clip b lives: samples, length, frame count, output base and output row stride.
+struct ClipView {
+    const float *x;
+    float *out;
+    uint32_t hs;
+    int L, F;
+};
+
+__device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
+{
+    ClipView v;
+    if (a.srcOff) {
+        v.L = max(a.lengths[b], 0);
+        v.x = a.src + a.srcOff[b];
+    } else {
+        v.L = max(min(a.lengths[b], static_cast<int>(a.srcStride)), 0);  // never read past the clip's row
+        v.x = a.src + static_cast<size_t>(b) * a.srcStride;
+    }
+    int F = max(num_windows(v.L, a.W, a.H, a.center != 0), 0);
+    if (a.dstOff) {
+        v.out = a.dst + a.dstOff[b];
+        v.hs = static_cast<uint32_t>(F);
+    } else {
+        v.out = a.dst + static_cast<size_t>(b) * a.dstNStride;
+        v.hs = a.hStride;
+        F = min(F, static_cast<int>(a.hStride));
+    }
+    v.F = F;
+    return v;
+}
 
 // Pre-emphasised sample at signal index j of a clip of length L, with the spectrogram's padding
 // rule applied to out-of-range indices (the chain pads the pre-emphasised signal).
@@ -114,7 +148,7 @@ __device__ __forceinline__ float db_exact(float v, const LogMelArgs &a)
 // prefix sum into tileStart[0..B], total in tileStart[B].
 __global__ void __launch_bounds__(1024)
 plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, uint32_t hStride, int W, int H,
-                  int center, int FT, int *__restrict__ tileStart)
+                  int center, int FT, int packed, int *__restrict__ tileStart)
 {
     __shared__ int waveSum[16];
     __shared__ int carry;
@@ -125,9 +159,9 @@ plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, ui
         const int b = base + tid;
         int n = 0;
         if (b < B) {
-            int L = min(lengths[b], static_cast<int>(srcStride));
-            int F = num_windows(max(L, 0), W, H, center != 0);
-            F = max(0, min(F, static_cast<int>(hStride)));
+            int L = packed ? lengths[b] : min(lengths[b], static_cast<int>(srcStride));
+            int F = max(0, num_windows(max(L, 0), W, H, center != 0));
+            if (!packed) F = min(F, static_cast<int>(hStride));
             n = (F + FT - 1) / FT;
         }
         int v = n;  // inclusive wave scan
@@ -175,13 +209,13 @@ __device__ float stage_segment(float *seg, const LogMelArgs &a, const float *x, 
 
 // Mel projection + dB by the exact FP32 gather (same order as mel_filter_bank.hip), from the
 // power tile pw[FT][pws].
-__device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, int b, int f0, int F, int FT,
+__device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, const ClipView &v, int f0, int FT,
                                  bool exactLog)
 {
     const int nOut = a.M * FT;
     for (int e = threadIdx.x; e < nOut; e += kThreads) {
         const int m = e / FT, f = e % FT;
-        if (f0 + f >= F) continue;
+        if (f0 + f >= v.F) continue;
         const float norm = a.normFactors[m];
         const int k0 = a.filterBins[2 * m], k1 = a.filterBins[2 * m + 1];
         float acc = 0.0f;
@@ -192,8 +226,7 @@ __device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, 
             const float p = pw[f * pws + k] * w;
             acc = acc + p;
         }
-        a.dst[static_cast<size_t>(b) * a.dstNStride + static_cast<size_t>(m) * a.hStride + f0 + f] =
-            exactLog ? db_exact(acc, a) : db_fast(acc, a);
+        v.out[static_cast<size_t>(m) * v.hs + f0 + f] = exactLog ? db_exact(acc, a) : db_fast(acc, a);
     }
 }
 
@@ -221,10 +254,8 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
     for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
         const int b = owner_clip(a.tileStart, a.B, tile);
         const int f0 = (tile - a.tileStart[b]) * kFT32;
-        const int L = min(a.lengths[b], static_cast<int>(a.srcStride));
-        const int F = min(num_windows(max(L, 0), a.W, a.H, a.center != 0), static_cast<int>(a.hStride));
-        const float *x = a.src + static_cast<size_t>(b) * a.srcStride;
-        stage_segment<false>(seg, a, x, L, f0, F, span, nullptr);
+        const ClipView v = clip_view(a, b);
+        stage_segment<false>(seg, a, v.x, v.L, f0, v.F, span, nullptr);
         __syncthreads();
         for (int pass = 0; pass < nPasses; pass++) {
             const int cBase = pass * kCC;
@@ -279,7 +310,7 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
             }
             __syncthreads();
         }
-        mel_gather_store(pw, pws, a, b, f0, F, kFT32, true);
+        mel_gather_store(pw, pws, a, v, f0, kFT32, true);
         __syncthreads();
     }
 }
@@ -328,16 +359,14 @@ logmel_mfma_kernel(LogMelArgs a)
     for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
         const int b = owner_clip(a.tileStart, a.B, tile);
         const int f0 = (tile - a.tileStart[b]) * FT;
-        const int L = min(a.lengths[b], static_cast<int>(a.srcStride));
-        const int F = min(num_windows(max(L, 0), a.W, a.H, a.center != 0), static_cast<int>(a.hStride));
-        const float *x = a.src + static_cast<size_t>(b) * a.srcStride;
+        const ClipView v = clip_view(a, b);
 
         float segScale = 1.0f;
         if (PREC == kPrecFP16x3) {
-            float mx = stage_segment<true>(seg, a, x, L, f0, F, span, red);
+            float mx = stage_segment<true>(seg, a, v.x, v.L, f0, v.F, span, red);
             if (mx > 0.0f) segScale = ldexpf(1.0f, 10 - ilogbf(mx));
         } else {
-            stage_segment<false>(seg, a, x, L, f0, F, span, nullptr);
+            stage_segment<false>(seg, a, v.x, v.L, f0, v.F, span, nullptr);
         }
         __syncthreads();
 
@@ -455,12 +484,12 @@ logmel_mfma_kernel(LogMelArgs a)
                 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, ph, c, 0, 0, 0);
             }
             // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
-            if (f0 + fr < F) {
-                float *out = a.dst + static_cast<size_t>(b) * a.dstNStride + f0 + fr;
+            if (f0 + fr < v.F) {
+                float *out = v.out + f0 + fr;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int m = mt * 16 + 4 * (lane >> 4) + r;
-                    if (m < a.M) out[static_cast<size_t>(m) * a.hStride] = db_fast(c[r], a);
+                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c[r], a);
                 }
             }
         }
@@ -469,41 +498,8 @@ logmel_mfma_kernel(LogMelArgs a)
 }
 
 // ------------------------------------------------------------------------- dispatch
-struct Launch {
-    int FT = 32, COLS = 512;
-};
-
-}  // namespace
-}  // namespace rppa
-
-using namespace rppa;
-
-RPPA_EXPORT RppStatus rppt_logmel_set_tiles(rppHandle_t h, const RpptLogMelTiles *t)
+RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
 {
-    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (t->framesPerTile != 0 && t->framesPerTile != 32 && t->framesPerTile != 64) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (t->blocksPerCU < 0 || t->blocksPerCU > 8) return RPP_ERROR_INVALID_ARGUMENTS;
-    h->tiles = *t;
-    return RPP_SUCCESS;
-}
-
-RPPA_EXPORT RppStatus rppt_logmel_get_tiles(rppHandle_t h, RpptLogMelTiles *t)
-{
-    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
-    *t = h->tiles;
-    return RPP_SUCCESS;
-}
-
-RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
-                                            RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
-                                            const RpptLogMelParams *p, RpptLogMelPrecision precision,
-                                            rppHandle_t h)
-{
-    if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
-    if (dstDescPtr->layout != NFT) return RPP_ERROR_INVALID_DST_LAYOUT;
-    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
-    if (!srcPtr || !dstPtr || !srcLengthTensor) return RPP_ERROR_INVALID_ARGUMENTS;
     if (!p->multiplier) return RPP_ERROR_ZERO_DIVISION;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     if (p->windowLength <= 0 || p->windowStep <= 0 || nfft < 2 || p->windowLength > nfft || p->numFilter <= 0)
@@ -512,14 +508,19 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
         return RPP_ERROR_INVALID_ARGUMENTS;
     if (!(p->referenceMagnitude > 0.0f)) return RPP_ERROR_NOT_IMPLEMENTED;  // per-clip max needs a second pass
     if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
-    const unsigned B = srcDescPtr->n;
-    if (B == 0) return RPP_SUCCESS;
-    if (dstDescPtr->h < static_cast<unsigned>(p->numFilter)) return RPP_ERROR_INVALID_DST_DIMS;
+    if (precision != RPPT_LOGMEL_FP32 && ((p->windowStep % 8) != 0 || nfft > 1024)) return RPP_ERROR_NOT_IMPLEMENTED;
+    return RPP_SUCCESS;
+}
+
+// Validated parameters + buffers -> planner and fused kernel on h->stream.
+RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPrecision precision, const float *src,
+                        uint32_t srcStride, const long long *srcOff, const int *lengths, int B, float *dst,
+                        uint32_t dstNStride, uint32_t hStride, const long long *dstOff)
+{
     const bool mfma = precision != RPPT_LOGMEL_FP32;
+    const int nfft = p->nfft ? p->nfft : p->windowLength;
     const int K = nfft / 2 + 1;
     const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
-    if (mfma && ((p->windowStep % 8) != 0 || nfft > 1024)) return RPP_ERROR_NOT_IMPLEMENTED;
-
     const DftTables *dt = get_dft_tables(h, nfft, p->windowLength, nullptr, splitKind);
     const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
                                          p->numFilter, p->normalize ? 1 : 0);
@@ -528,13 +529,15 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     if (!tileStart) return RPP_ERROR_NOT_ENOUGH_MEMORY;
 
     LogMelArgs a{};
-    a.src = static_cast<const float *>(srcPtr);
-    a.srcStride = srcDescPtr->strides.nStride;
-    a.lengths = srcLengthTensor;
-    a.B = static_cast<int>(B);
-    a.dst = static_cast<float *>(dstPtr);
-    a.dstNStride = dstDescPtr->strides.nStride;
-    a.hStride = dstDescPtr->strides.hStride;
+    a.src = src;
+    a.srcStride = srcStride;
+    a.lengths = lengths;
+    a.B = B;
+    a.dst = dst;
+    a.dstNStride = dstNStride;
+    a.hStride = hStride;
+    a.srcOff = srcOff;
+    a.dstOff = dstOff;
     a.coeff = p->preEmphasisCoeff;
     a.border = static_cast<int>(p->borderType);
     a.N = nfft;
@@ -545,6 +548,7 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     a.power = p->power;
     a.K = K;
     a.M = p->numFilter;
+    // dB constants exactly as cpu/kernel/to_decibels.cpp:38-43 computes them
     float minRatio = static_cast<float>(std::pow(10.0, static_cast<double>(p->cutOffDB / p->multiplier)));
     if (minRatio == 0.0f) minRatio = std::nextafter(0.0f, 1.0f);
     a.minRatio = minRatio;
@@ -568,7 +572,7 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    // tile shape
+    // tile shape: frames per work item and persistent blocks per CU
     int FT;
     if (!mfma) FT = 32;
     else if (dt->colsAlloc > 512) FT = 32;
@@ -576,8 +580,8 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 2;
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
-    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, srcLengthTensor, a.B, a.srcStride,
-                       a.hStride, a.W, a.H, a.center, FT, tileStart);
+    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.hStride,
+                       a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart);
     const int pws = a.KP + kSkew;
     if (!mfma) {
         const int span = (kFT32 - 1) * a.H + a.W;
@@ -613,4 +617,64 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     }
     RPPA_HIP(hipGetLastError());
     return RPP_SUCCESS;
+}
+
+}  // namespace
+}  // namespace rppa
+
+using namespace rppa;
+
+RppStatus check_logmel_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
+{
+    return check_params(p, precision);
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_set_tiles(rppHandle_t h, const RpptLogMelTiles *t)
+{
+    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (t->framesPerTile != 0 && t->framesPerTile != 32 && t->framesPerTile != 64) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (t->blocksPerCU < 0 || t->blocksPerCU > 8) return RPP_ERROR_INVALID_ARGUMENTS;
+    h->tiles = *t;
+    return RPP_SUCCESS;
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_get_tiles(rppHandle_t h, RpptLogMelTiles *t)
+{
+    if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
+    *t = h->tiles;
+    return RPP_SUCCESS;
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                            RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                            const RpptLogMelParams *p, RpptLogMelPrecision precision,
+                                            rppHandle_t h)
+{
+    if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
+    if (dstDescPtr->layout != NFT) return RPP_ERROR_INVALID_DST_LAYOUT;
+    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (!srcPtr || !dstPtr || !srcLengthTensor) return RPP_ERROR_INVALID_ARGUMENTS;
+    RppStatus st = check_params(p, precision);
+    if (st != RPP_SUCCESS) return st;
+    const unsigned B = srcDescPtr->n;
+    if (B == 0) return RPP_SUCCESS;
+    if (dstDescPtr->h < static_cast<unsigned>(p->numFilter)) return RPP_ERROR_INVALID_DST_DIMS;
+    return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
+                         srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
+                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr);
+}
+
+RPPA_EXPORT RppStatus rppt_logmel_fused_packed_gpu(const Rpp32f *srcPtr, const long long *srcOffsets,
+                                                   const Rpp32s *srcLengthTensor, Rpp32s batchSize, Rpp32f *dstPtr,
+                                                   const long long *dstOffsets, const RpptLogMelParams *p,
+                                                   RpptLogMelPrecision precision, rppHandle_t h)
+{
+    if (!h || !p || !srcPtr || !srcOffsets || !srcLengthTensor || !dstPtr || !dstOffsets || batchSize < 0)
+        return RPP_ERROR_INVALID_ARGUMENTS;
+    RppStatus st = check_params(p, precision);
+    if (st != RPP_SUCCESS) return st;
+    if (batchSize == 0) return RPP_SUCCESS;
+    return logmel_launch(h, p, precision, srcPtr, 0, srcOffsets, srcLengthTensor, batchSize, dstPtr, 0, 0,
+                         dstOffsets);
 }

@@ -1,0 +1,63 @@
+"""The drop-in boundary, CPU only: librpp_audio.so loads, exports every entry point that
+include/rpp_audio.h declares (unmangled), and the ctypes mirrors of the descriptor structs have
+the reference's layout (api/rppdefs.h:570-649, :751-790).  No compute call is made here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rpp_audio.h")
+LIB = os.path.join(ROOT, "rpp_amd", "librpp_audio.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^(?:RppStatus|rppStatus_t)\s+(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_reference_entry_points():
+    names = declared_functions()
+    for n in ("rppCreate", "rppDestroy", "rppSetStream", "rppGetStream", "rppSetBatchSize", "rppGetBatchSize",
+              "rppt_pre_emphasis_filter_gpu", "rppt_spectrogram_gpu", "rppt_mel_filter_bank_gpu",
+              "rppt_to_decibels_gpu", "rppt_down_mixing_gpu", "rppt_resample_gpu"):
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    import torch  # noqa: F401  -- one HIP runtime for the process
+    assert os.path.exists(LIB), "build with python -m rpp_amd.build"
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    for n in declared_functions():
+        assert n in exported, f"{n} is not an unmangled global text symbol"
+
+
+def test_struct_layouts():
+    from rpp_amd import api
+    assert C.sizeof(api.RpptDesc) == 56
+    assert api.RpptDesc.offsetInBytes.offset == 8 and api.RpptDesc.dataType.offset == 12
+    assert api.RpptDesc.n.offset == 16 and api.RpptDesc.strides.offset == 32 and api.RpptDesc.layout.offset == 48
+    assert C.sizeof(api.RpptImagePatch) == 8
+    assert C.sizeof(api.RpptResamplingWindow) == 96
+    assert api.RpptResamplingWindow.lookupSize.offset == 16
+    assert api.RpptResamplingWindow.lookupPinned.offset == 24
+    assert api.RpptResamplingWindow.pCenter.offset == 64 and api.RpptResamplingWindow.pScale.offset == 80
+    assert (api.NFT, api.NTF, api.F32) == (5, 6, 1)
+    assert api.BORDER == {"zero": 0, "clamp": 1, "reflect": 2} and api.MEL == {"slaney": 0, "htk": 1}
+
+
+def test_create_rejects_host_backend():
+    """Only RPP_HIP_BACKEND handles are served; HOST returns rppStatusNotImplemented without
+    touching the device."""
+    import torch  # noqa: F401
+    lib = C.CDLL(LIB)
+    h = C.c_void_p()
+    lib.rppCreate.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint32, C.c_void_p, C.c_int]
+    assert lib.rppCreate(C.byref(h), 4, 0, None, 0) == -7

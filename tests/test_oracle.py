@@ -1,0 +1,295 @@
+"""Pins the CPU oracle (oracle/rpp_oracle.c) -- CPU only.
+
+1. the reference's own golden vectors (utilities/test_suite/REFERENCE_OUTPUTS_AUDIO, committed as
+   tests/golden/rpp_qa_audio.npz by tests/golden/make_golden.py);
+2. RPP's own HOST kernels compiled from the reference sources (oracle/_ref; skipped where absent);
+3. what the mathematics fixes: numpy FP64 FFT, Parseval, impulses, closed forms, symmetry,
+   partition of unity, brute-force index maps.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def valid_mask(shape, lengths):
+    m = np.zeros(shape, bool)
+    for b, n in enumerate(lengths):
+        m[b, :n] = True
+    return m
+
+
+# ------------------------------------------------------------------ golden vectors (batch 3 QA)
+def test_golden_pre_emphasis(golden):
+    x, L = golden["x"], golden["lengths"]
+    y = O.pre_emphasis_f32(x, L, 0.97, "clamp")
+    m = valid_mask(x.shape, L)
+    assert np.array_equal(y[m], golden["pre_emphasis"][m])
+
+
+def test_golden_to_decibels(golden):
+    x, L = golden["x"], golden["lengths"]
+    y = O.to_db_f32(x, np.ones(3), L, np.float32(np.log(1e-20)), np.float32(np.log(10)), 1.0)
+    m = valid_mask(x.shape, L)
+    assert np.array_equal(y[m], golden["to_decibels"][m])
+
+
+def test_golden_spectrogram(golden):
+    """FFTS (prebuilt, not runnable here) vs the project FFT: tolerance, as RPP's own HIP QA has."""
+    x, L, F = golden["x"], golden["lengths"], golden["frames"]
+    sp = O.spectrogram_f32(x, L, 512, 320, 160, True, True, 2, "NFT", hstride=int(F.max()))
+    g = golden["spectrogram"]
+    for b, f in enumerate(F):
+        a, r = sp[b, :, :f].astype(np.float64), g[b, :, :f].astype(np.float64)
+        assert np.abs(a - r).max() < 1e-3
+        assert (np.abs(a - r) / r.max(axis=0)).max() <= 1e-6
+
+
+def test_golden_mel(golden):
+    mel = O.mel_f32(golden["spectrogram"], golden["frames"], 80, 16000.0, 0.0, 8000.0)
+    for b, f in enumerate(golden["frames"]):
+        assert np.array_equal(mel[b, :, :f], golden["mel"][b, :, :f])
+
+
+def test_golden_resample(golden):
+    x, L = golden["x"], golden["lengths"]
+    lut, sc, ce = O.windowed_sinc_lut(1025, 16)
+    out_rate = np.float32(16000 * 1.15)
+    y = O.resample_f32(x, L, 16000.0, out_rate, lut, sc, ce, 16, out_stride=golden["resample"].shape[1])
+    for b, n in enumerate(L):
+        k = O.resample_out_length(int(n), 16000.0, float(out_rate))
+        assert np.array_equal(y[b, :k], golden["resample"][b, :k])
+
+
+# ------------------------------------------------------------------ mathematics
+def test_pre_emphasis_closed_form():
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (3, 64)).astype(np.float32)
+    L = np.array([64, 10, 1], np.int32)
+    c = np.float32(0.9)
+    y = O.pre_emphasis_f32(x, L, c, "zero")
+    for b, n in enumerate(L):
+        prev = np.concatenate([[np.float32(0)], x[b, :n - 1]]).astype(np.float32)
+        assert np.array_equal(y[b, :n], x[b, :n] - c * prev)
+    assert np.array_equal(O.pre_emphasis_f32(x, L, 0.0, "clamp")[0], x[0])
+    k = np.full((1, 16), 0.5, np.float32)
+    assert np.allclose(O.pre_emphasis_f32(k, [16], c, "clamp")[0], (1 - c) * 0.5, rtol=0, atol=1e-7)
+
+
+def test_hann_properties():
+    for W in (320, 400, 1024, 7):
+        w = O.hann_f64(W)
+        assert np.allclose(w, w[::-1], atol=1e-15)
+        assert abs(w.sum() - W / 2) < 1e-9
+        w32 = O.hann_f32(W)
+        assert np.array_equal(w32, w.astype(np.float32))
+
+
+def test_reflect_and_frames_brute_force():
+    for L in (2, 3, 5, 17):
+        pad = 5 * L
+        ref = np.pad(np.arange(L), pad, mode="reflect")   # mirror without repeating the edge
+        for j in range(-pad, L + pad):
+            assert O.idx_reflect(j, 0, L) == ref[j + pad]
+    assert O.idx_reflect(-3, 0, 1) == 0
+    for L in (0, 1, 159, 160, 161, 48000):
+        assert O.num_windows(L, 400, 160, True) == L // 160 + 1
+        assert O.num_windows(L, 400, 160, False) == int((L - 400) / 160) + 1
+
+
+def frames_numpy(x, L, W, H, center=True, reflect=True):
+    """Independent framing: numpy reflect padding, then strided frames."""
+    sig = x[:L].astype(np.float64)
+    off = W // 2 if center else 0
+    F = (L if center else L - W) // H + 1
+    pad = off + F * H + W
+    padded = np.pad(sig, pad, mode="reflect" if reflect else "constant")
+    return np.stack([padded[pad - off + f * H: pad - off + f * H + W] for f in range(F)])
+
+
+@pytest.mark.parametrize("nfft,W,H", [(512, 400, 160), (1024, 1024, 256), (400, 400, 160), (401, 300, 128)])
+def test_spectrogram_f64_matches_numpy_fft(nfft, W, H):
+    rng = np.random.default_rng(nfft)
+    L = np.array([5000, 777], np.int32)
+    x = rng.uniform(-1, 1, (2, 5008))
+    spec = O.spectrogram_f64(x, L, nfft, W, H)
+    w = O.hann_f64(W)
+    for b, n in enumerate(L):
+        fr = frames_numpy(x[b], int(n), W, H) * w
+        ref = np.abs(np.fft.rfft(fr, n=nfft, axis=1)) ** 2
+        got = spec[b, :, :len(fr)].T
+        assert np.allclose(got, ref, rtol=1e-9, atol=1e-9 * ref.max())
+
+
+def test_spectrogram_parseval_and_impulse():
+    nfft, W, H = 512, 512, 160
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(1, 4000))
+    spec = O.spectrogram_f64(x, [4000], nfft, W, H)
+    w = O.hann_f64(W)
+    fr = frames_numpy(x[0], 4000, W, H) * w
+    # Parseval for the one-sided power spectrum of a real frame
+    P = spec[0, :, :len(fr)]
+    two_sided = P[0] + P[-1] + 2 * P[1:-1].sum(axis=0)
+    assert np.allclose(two_sided / nfft, (fr ** 2).sum(axis=1), rtol=1e-10)
+    # an impulse at sample t0 of a frame gives a flat spectrum w[t]^2
+    imp = np.zeros((1, 4000))
+    imp[0, 2000] = 1.0
+    s = O.spectrogram_f64(imp, [4000], nfft, W, H, center=False)
+    f = 10  # frame 10 starts at 1600 and holds the impulse at t = 400
+    assert np.allclose(s[0, :, f], w[400] ** 2, rtol=1e-12)
+
+
+def test_spectrogram_f32_close_to_f64():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (2, 16000)).astype(np.float32)
+    L = np.array([16000, 9000], np.int32)
+    s32 = O.spectrogram_f32(x, L).astype(np.float64)
+    s64 = O.spectrogram_f64(x.astype(np.float64), L)
+    for b, n in enumerate(L):
+        F = int(n) // 160 + 1
+        a, r = s32[b, :, :F], s64[b, :, :F]
+        assert (np.abs(a - r) / r.max(axis=0)).max() < 1e-6
+
+
+def test_mel_filter_properties():
+    """Unnormalised filters are a partition of unity between the first and last centre
+    frequencies; normalised (Slaney) filters have unit area in Hz."""
+    K, sr, M = 257, 16000.0, 40
+    eye = np.eye(K)[None].repeat(1, 0).transpose(0, 1, 2).astype(np.float64)  # [1, K, K]: column k = bin k
+    W_un = O.mel_f64(eye, [K], M, sr, 0.0, 8000.0, "slaney", False)[0]      # [M, K]
+    W_n = O.mel_f64(eye, [K], M, sr, 0.0, 8000.0, "slaney", True)[0]
+    hz = np.arange(K) * sr / 512
+    interior = (hz > 400) & (hz < 6000)
+    assert np.allclose(W_un[:, interior].sum(axis=0), 1.0, atol=1e-12)
+    area = W_n.sum(axis=1) * (sr / 512)
+    assert np.allclose(area[5:-5], 1.0, rtol=0.05)
+    # the FP32 restatement agrees with the FP64 construction (RPP's rounded FP32 Slaney constants
+    # 66.666667f / 0.068751777f move the corners by ~1e-6 relative)
+    W32 = O.mel_f32(eye.astype(np.float32), [K], M, sr, 0.0, 8000.0, "slaney", True)[0]
+    assert np.abs(W32 - W_n).max() <= 5e-6 * W_n.max()
+
+
+def test_to_db_closed_form():
+    rng = np.random.default_rng(5)
+    x = (rng.uniform(0, 1, (2, 10, 20)) ** 8).astype(np.float32)
+    y = O.to_db_f32(x, [20, 20], [10, 10], -80.0, 10.0, 1.0)
+    ref = 10 * np.log10(np.maximum(x.astype(np.float64), 1e-8))
+    assert np.abs(y - ref).max() < 1e-4
+    y0 = O.to_db_f32(x, [20, 20], [10, 10], -80.0, 10.0, 0.0)   # ref = per-clip max -> max maps to 0 dB
+    assert np.allclose(y0.reshape(2, -1).max(axis=1), 0.0, atol=1e-6)
+
+
+def test_down_mix_closed_form():
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (2, 200)).astype(np.float32)
+    y = O.down_mix_f32(x, [100, 100], [2, 2])
+    w = np.float32(0.5)
+    assert np.array_equal(y[:, :100], (np.float32(0) + x[:, 0::2] * w) + x[:, 1::2] * w)
+
+
+def test_resample_identity_and_dc():
+    lut, sc, ce = O.windowed_sinc_lut(1025, 16)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, (1, 4000)).astype(np.float32)
+    assert np.array_equal(O.resample_f32(x, [4000], 16000.0, 16000.0, lut, sc, ce, 16)[0], x[0])
+    dc = np.ones((1, 48000), np.float32)
+    y = O.resample_f32(dc, [48000], 48000.0, 16000.0, lut, sc, ce, 16)[0]
+    mid = y[1000:15000]
+    # integer-spaced taps of a (windowed) sinc sum to ~1, so a constant passes at unit gain
+    # (the reference does not widen the kernel when downsampling)
+    assert np.ptp(mid) < 1e-5 and abs(mid.mean() - 1.0) < 1e-3
+
+
+# ------------------------------------------------------------------ RPP's own HOST kernels
+needs_ref = pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built (needs /root/reference)")
+
+
+@needs_ref
+@pytest.mark.parametrize("border", ["zero", "clamp", "reflect"])
+def test_ref_pre_emphasis(border):
+    r = O.ref_lib()
+    rng = np.random.default_rng(8)
+    L = np.array([1, 2, 9, 17, 1000, 4093], np.int32)
+    x = rng.uniform(-1, 1, (len(L), 4096)).astype(np.float32)
+    c = rng.uniform(0.5, 1, len(L)).astype(np.float32)
+    y = np.zeros_like(x)
+    assert r.ref_pre_emphasis(x, 4096, y, 4096, L, c, len(L), O.BORDER[border], 2) == 0
+    o = O.pre_emphasis_f32(x, L, c, border)
+    m = valid_mask(x.shape, L)
+    assert np.array_equal(o[m], y[m])
+
+
+@needs_ref
+@pytest.mark.parametrize("formula,normalize,fmin,fmax,M", [("slaney", True, 0.0, 8000.0, 80),
+                                                           ("slaney", False, 100.0, 7000.0, 40),
+                                                           ("htk", True, 20.0, 8000.0, 64)])
+def test_ref_mel(formula, normalize, fmin, fmax, M):
+    r = O.ref_lib()
+    rng = np.random.default_rng(9)
+    B, K, S = 3, 257, 120
+    spec = (rng.uniform(0, 1, (B, K, S)) ** 4 * 100).astype(np.float32)
+    frames = np.array([120, 1, 77], np.int32)
+    dims = np.stack([np.full(B, K), frames], 1).reshape(-1).astype(np.int32)
+    y = np.zeros((B, M, S), np.float32)
+    assert r.ref_mel_filter_bank(spec, K, S, K * S, dims, y, S, M * S, B, fmax, fmin, O.MEL[formula], M,
+                                 16000.0, int(normalize), 2) == 0
+    o = O.mel_f32(spec, frames, M, 16000.0, fmin, fmax, formula, normalize)
+    for b, F in enumerate(frames):
+        if formula == "slaney":
+            assert np.array_equal(o[b, :, :F], y[b, :, :F])
+        else:
+            # the reference is built with -Ofast, which re-evaluates the HTK scale's divisions and
+            # log/exp differently from IEEE FP32; the filter corners move by ~1e-5 relative
+            r = np.abs(o[b, :, :F] - y[b, :, :F]) / np.maximum(np.abs(y[b, :, :F]), 1e-30)
+            assert r.max() <= 3e-4
+
+
+@needs_ref
+@pytest.mark.parametrize("ref", [1.0, 0.0])
+def test_ref_to_decibels(ref):
+    r = O.ref_lib()
+    rng = np.random.default_rng(10)
+    B, Hh, S = 3, 80, 64
+    x = (rng.uniform(0, 1, (B, Hh, S)) ** 6 * 30).astype(np.float32)
+    widths = np.array([64, 1, 33], np.uint32)
+    heights = np.array([80, 80, 7], np.uint32)
+    dims = np.stack([widths, heights], 1).reshape(-1).astype(np.uint32)
+    y = np.zeros_like(x)
+    assert r.ref_to_decibels(x, S, Hh * S, y, S, Hh * S, dims, B, -80.0, 10.0, ref, 2) == 0
+    o = O.to_db_f32(x, widths, heights, -80.0, 10.0, ref)
+    assert np.array_equal(o, y)
+
+
+@needs_ref
+@pytest.mark.parametrize("C", [2, 3, 4, 8, 9])
+def test_ref_down_mixing(C):
+    r = O.ref_lib()
+    rng = np.random.default_rng(11)
+    n = 500
+    x = rng.uniform(-1, 1, (2, n * C)).astype(np.float32)
+    dims = np.array([n, C, n - 3, C], np.int32)
+    y = np.zeros((2, n), np.float32)
+    assert r.ref_down_mixing(x, n * C, C, y, n, dims, 2, 0, 2) == 0
+    o = O.down_mix_f32(x, [n, n - 3], [C, C])
+    assert np.array_equal(o[0], y[0]) and np.array_equal(o[1, :n - 3], y[1, :n - 3])
+
+
+@needs_ref
+@pytest.mark.parametrize("in_rate,out_rate", [(16000.0, 18400.0), (48000.0, 16000.0), (44100.0, 16000.0)])
+def test_ref_resample(in_rate, out_rate):
+    r = O.ref_lib()
+    lut, sc, ce = O.windowed_sinc_lut(4096 - 5, 16)
+    rng = np.random.default_rng(12)
+    L = np.array([9000, 12345], np.int32)
+    x = rng.uniform(-1, 1, (2, 12352)).astype(np.float32)
+    out_len = max(O.resample_out_length(int(n), in_rate, out_rate) for n in L)
+    y = np.zeros((2, out_len), np.float32)
+    ir = np.full(2, in_rate, np.float32)
+    orr = np.full(2, out_rate, np.float32)
+    dims = np.stack([L, np.ones(2, np.int32)], 1).reshape(-1).astype(np.int32)
+    assert r.ref_resample(x, 12352, y, out_len, ir, orr, dims, 2, sc, ce, 16, 4096 - 5, len(lut), lut, 2) == 0
+    o = O.resample_f32(x, L, in_rate, out_rate, lut, sc, ce, 16, out_stride=out_len)
+    for b, n in enumerate(L):
+        k = O.resample_out_length(int(n), in_rate, out_rate)
+        assert np.array_equal(o[b, :k], y[b, :k])
