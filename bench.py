@@ -35,6 +35,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def serve_measurement(api, synth, params, precision, seconds, rate=4000.0):
+    """Served stream: Poisson arrivals at `rate` clips/s, lengths log-uniform 0.25-30 s at 16 kHz,
+    through the continuous-batching server (2 ms batch deadline, 4 device slots, admission by
+    memory).  Latency = arrival -> dB rows in host memory.  Then a saturation burst."""
+    from rpp_amd.serve import Server
+    t_arr, lengths = synth.arrivals(rate, seconds)
+    pool = [synth.clip_for_request(i, 480000) for i in range(16)]
+    res = {"offered_clips_per_s": rate, "seconds": seconds, "requests": int(len(t_arr)),
+           "offered_audio_s_per_s": round(float(lengths.sum()) / 16000.0 / seconds, 1)}
+    with Server(params, precision, max_batch=64, deadline_us=2000, mem_budget=2 << 30, slots=4, eager=True,
+                keep_outputs=False) as srv:
+        t0 = time.perf_counter()
+        for i, (t, n) in enumerate(zip(t_arr, lengths)):
+            while time.perf_counter() - t0 < t:
+                pass
+            srv.submit(pool[i % len(pool)][:int(n)])
+        srv.drain(120000)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    res.update({"p50_ms": round(st["p50Us"] / 1e3, 3), "p90_ms": round(st["p90Us"] / 1e3, 3),
+                "p99_ms": round(st["p99Us"] / 1e3, 3), "max_ms": round(st["maxUs"] / 1e3, 3),
+                "audio_s_per_s": round(st["audioSamples"] / 16000.0 / wall, 1), "completed": st["completed"],
+                "refused": st["refused"], "batches": st["batches"], "mean_batch_clips": round(st["meanBatchClips"], 2),
+                "gpu_busy_fraction": round(st["gpuBusySeconds"] / wall, 4)})
+    # saturation: a burst of 4000 requests submitted at once, drained by continuous batching
+    n_burst = 4000
+    with Server(params, precision, max_batch=128, deadline_us=2000, mem_budget=8 << 30, slots=4, eager=True,
+                keep_outputs=False) as srv:
+        t0 = time.perf_counter()
+        for i in range(n_burst):
+            srv.submit(pool[i % len(pool)][:int(lengths[i % len(lengths)])])
+        srv.drain(120000)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    res["saturation"] = {"requests": n_burst, "audio_s_per_s": round(st["audioSamples"] / 16000.0 / wall, 1),
+                         "batches": st["batches"], "mean_batch_clips": round(st["meanBatchClips"], 2),
+                         "p50_ms": round(st["p50Us"] / 1e3, 3), "p99_ms": round(st["p99Us"] / 1e3, 3)}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +84,8 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "bf16", "fp32"])
     ap.add_argument("--no-side", action="store_true", help="skip side measurements (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="duration of the served-stream measurement (Poisson arrivals, 4000 clips/s); 0 skips it")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus > 1 is not supported "
@@ -161,6 +203,11 @@ def main() -> int:
                "sample": f"first {sample} clips of the workload ({cpu_as:.0f} audio-s), median of 3 runs"}
         log(f"side: {side}  cpu: {cpu}")
 
+    serving = None
+    if not a.no_side and a.serve_seconds > 0:
+        serving = serve_measurement(api, synth, params, a.precision, a.serve_seconds)
+        log(f"serving: {serving}")
+
     line = {
         "metric": "log-mel audio-sec/sec (mixed-precision fused vs FP32, <=0.05 dB vs FP64)",
         "value": round(value, 1),
@@ -183,6 +230,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "side": side,
+        "serving": serving,
     }
     print(json.dumps(line))
     h.close()

@@ -2,7 +2,8 @@
 the reference's own golden vectors (utilities/test_suite/REFERENCE_OUTPUTS_AUDIO, batch 3).
 
 Tolerances:
-  pre-emphasis, to_decibels, mel (same input)  bit-exact (the reference's rounding is reproduced)
+  pre-emphasis, mel (same input)               bit-exact (the reference's rounding is reproduced)
+  to_decibels                                  bit-exact on the goldens; <= 2 ulp elsewhere (glibc log2f)
   spectrogram                                  |err| / frame max <= 1e-5 and |err| < 1e-3
                                                (the reference's own HIP QA cutoff,
                                                utilities/test_suite/rpp_test_suite_audio.h:72-81)
@@ -112,9 +113,10 @@ def test_to_decibels_2d_ref(gpu, handle, orc, ref):
         else:
             a, e = y[b, :heights[b], :widths[b]], r[b, :heights[b], :widths[b]]
         # log2 on the device is the correctly rounded FP32 value; glibc's log2f (the HOST path's) is
-        # off by one ulp for a few arguments near 1 (the golden vectors above happen to avoid them)
+        # off by one ulp for a few arguments near 1, two after the scaling by the multiplier (the
+        # golden vectors above happen to avoid them)
         ulp = np.abs(a.reshape(-1).view(np.int32).astype(np.int64) - e.reshape(-1).view(np.int32).astype(np.int64))
-        assert ulp.max() <= 1 and np.mean(ulp == 0) > 0.99, b
+        assert ulp.max() <= 2 and np.mean(ulp == 0) > 0.99, b
 
 
 def test_status_codes(gpu, handle):
