@@ -35,6 +35,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def front_end_measurement(api, synth, torch, precision, batch=16, seconds=60.0, steps=5):
+    """Workload 4 (reduced to `batch` clips): stereo int16-range 48 kHz clips, 60 s each ->
+    down-mix -> Kaiser-windowed sinc resample to 16 kHz (16 zero-crossings, 4096-entry table) ->
+    fused log-mel, the three launches replayed from a hipGraph."""
+    bt = synth.workload4(batch=batch, seconds=seconds)
+    lut, sc, ce = synth.kaiser_sinc_lut()
+    x = torch.from_numpy(bt.x).cuda()
+    L = torch.from_numpy(bt.lengths).cuda()
+    lut_d = torch.from_numpy(lut).cuda()
+    p = api.logmel_params()
+    stream = torch.cuda.Stream()
+    h = api.Handle(batch, stream=stream)
+    ws = {}
+    with torch.cuda.stream(stream):
+        api.front_end_logmel(h, x, L, 2, lut_d, sc, ce, 16, 4091, 48000.0, 16000.0, p, precision, ws)
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            api.front_end_logmel(h, x, L, 2, lut_d, sc, ce, 16, 4091, 48000.0, 16000.0, p, precision, ws)
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    audio = float(bt.lengths.sum()) / 48000.0
+    h.close()
+    return {"batch": batch, "seconds_per_clip": seconds, "ms_per_step": round(wall / steps * 1e3, 3),
+            "audio_s_per_s": round(audio * steps / wall, 1)}
+
+
 def serve_measurement(api, synth, params, precision, seconds, rate=4000.0):
     """Served stream: Poisson arrivals at `rate` clips/s, lengths log-uniform 0.25-30 s at 16 kHz,
     through the continuous-batching server (2 ms batch deadline, 4 device slots, admission by
@@ -114,9 +146,13 @@ def main() -> int:
     stream = torch.cuda.Stream()
     h = api.Handle(B, stream=stream)
 
+    tuned = {}
+
     def fused_graph(precision):
         with torch.cuda.stream(stream):
             api.logmel_fused(h, x, out, L, params, precision)  # builds tables, sizes workspace
+            tuned[precision] = api.autotune(h, x, out, L, params, precision, iters=3)
+            log(f"autotune {precision}: tiles {tuned[precision][0]} {tuned[precision][1]:.3f} ms")
             stream.synchronize()
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=stream):
@@ -203,6 +239,9 @@ def main() -> int:
                "sample": f"first {sample} clips of the workload ({cpu_as:.0f} audio-s), median of 3 runs"}
         log(f"side: {side}  cpu: {cpu}")
 
+    if not a.no_side:
+        side["front_end_workload4"] = front_end_measurement(api, synth, torch, a.precision)
+        log(f"front end: {side['front_end_workload4']}")
     serving = None
     if not a.no_side and a.serve_seconds > 0:
         serving = serve_measurement(api, synth, params, a.precision, a.serve_seconds)
@@ -226,7 +265,8 @@ def main() -> int:
                                 2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],
                    "batch": B, "audio_seconds": round(audio_s, 2), "frames": int(frames.sum()),
                    "nfft": feat["nfft"], "window": feat["W"], "hop": H, "mels": feat["nmel"],
-                   "precision": a.precision, "launch": "planner + persistent fused kernel, hipGraph replay"},
+                   "precision": a.precision, "launch": "planner + persistent fused kernel, hipGraph replay",
+                   "tiles": {"frames_per_tile": tuned[a.precision][0][0], "blocks_per_cu": tuned[a.precision][0][1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "side": side,

@@ -239,6 +239,12 @@ typedef struct {
     Rpp32s blocksPerCU;     /* persistent blocks per compute unit */
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);
+/* Autotuner: times every supported tile shape on this batch (`iters` launches each, events on the
+ * handle's stream), keeps the fastest in the handle and reports it. */
+RppStatus rppt_logmel_autotune(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                               Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
+                               RpptLogMelPrecision precision, Rpp32s iters, rppHandle_t rppHandle,
+                               RpptLogMelTiles *chosen, Rpp32f *bestMs);
 RppStatus rppt_logmel_get_tiles(rppHandle_t rppHandle, RpptLogMelTiles *tiles);
 
 /* ======================= serving (no counterpart in the reference) ======================= */

@@ -113,6 +113,8 @@ def lib():
             l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
             l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
             l.rppt_logmel_get_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
+            l.rppt_logmel_autotune.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, C.c_int32, P,
+                                               C.POINTER(RpptLogMelTiles), C.POINTER(C.c_float)]
         l.rppa_debug_mel_tables.argtypes = [P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                                             C.c_int, P, P, P, P]
         _LIB = l
@@ -316,10 +318,52 @@ def set_tiles(h: Handle, frames_per_tile=0, blocks_per_cu=0) -> None:
     check("rppt_logmel_set_tiles", lib().rppt_logmel_set_tiles(h.h, C.byref(t)))
 
 
+def autotune(h: Handle, src, dst, lengths, params: RpptLogMelParams, precision="fp16x3", iters=5):
+    """rppt_logmel_autotune: returns ((framesPerTile, blocksPerCU), best ms per launch)."""
+    B, S = src.shape
+    sd = desc_1d(B, S)
+    dd = desc_2d(B, dst.shape[1], dst.shape[2])
+    t = RpptLogMelTiles()
+    ms = C.c_float()
+    check("rppt_logmel_autotune", lib().rppt_logmel_autotune(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(lengths),
+                                                           C.byref(params), PRECISION[precision], iters, h.h,
+                                                           C.byref(t), C.byref(ms)))
+    return (t.framesPerTile, t.blocksPerCU), ms.value
+
+
 def get_tiles(h: Handle):
     t = RpptLogMelTiles()
     check("rppt_logmel_get_tiles", lib().rppt_logmel_get_tiles(h.h, C.byref(t)))
     return t.framesPerTile, t.blocksPerCU
+
+
+def front_end_logmel(h: Handle, stereo, lengths, channels, lut_dev, scale, center, lobes, coeffs, in_rate, out_rate,
+                     params: RpptLogMelParams, precision="fp16x3", workspace=None):
+    """Workload-4 front end through the drop-in entry points: down-mix (interleaved channels ->
+    mono), resample in_rate -> out_rate with the caller's table, then the fused log-mel launch.
+    stereo [B, L*C] cuda; lengths int32 cuda [B] (frames per channel).  Returns (db, mono, resampled)."""
+    B, S = stereo.shape
+    dev = stereo.device
+    n_mono = S // channels
+    ws = workspace if workspace is not None else {}
+    if "mono" not in ws:
+        ws["mono"] = torch.empty((B, n_mono), dtype=torch.float32, device=dev)
+        n_out = int(-(-n_mono * out_rate // in_rate)) + 8
+        ws["res"] = torch.empty((B, n_out), dtype=torch.float32, device=dev)
+        ws["dims_dm"] = torch.stack([lengths, torch.full_like(lengths, channels)], 1).reshape(-1).contiguous()
+        ws["dims_rs"] = torch.stack([lengths, torch.ones_like(lengths)], 1).reshape(-1).contiguous()
+        ws["in_rate"] = torch.full((B,), in_rate, dtype=torch.float32, device=dev)
+        ws["out_rate"] = torch.full((B,), out_rate, dtype=torch.float32, device=dev)
+        out_len = torch.ceil(lengths.to(torch.float32) * torch.tensor(out_rate, dtype=torch.float32, device=dev)
+                             / torch.tensor(in_rate, dtype=torch.float32, device=dev)).to(torch.int32)
+        ws["res_len"] = out_len
+        F = int(out_len.max().item()) // params.windowStep + 1
+        ws["db"] = torch.empty((B, params.numFilter, F), dtype=torch.float32, device=dev)
+        ws["window"] = make_window(lut_dev, scale, center, lobes, coeffs)
+    down_mixing(h, stereo, ws["mono"], ws["dims_dm"], normalize=False, channels=channels)
+    resample(h, ws["mono"], ws["res"], ws["in_rate"], ws["out_rate"], ws["dims_rs"], ws["window"])
+    logmel_fused(h, ws["res"], ws["db"], ws["res_len"], params, precision)
+    return ws["db"], ws["mono"], ws["res"]
 
 
 def logmel_chain(h: Handle, x, lengths_dev, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000.0, fmin=0.0,

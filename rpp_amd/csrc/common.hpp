@@ -41,14 +41,16 @@ constexpr int kWave = 64;
 struct DftTables {
     int nfft = 0, W = 0, cols = 0;
     float *basis = nullptr;        // [W][cols] FP32
-    // Split copies for the MFMA path, in the operand order of a 32x32x16 MFMA: element (t, c)
-    // at ((t / 8) * colsAlloc + c) * 8 + t % 8, so the 32 lanes of a half-wave that load
-    // 8 consecutive t of 32 consecutive columns read one contiguous 512-byte run.
-    uint16_t *basisHi = nullptr;   // bf16 or fp16 hi part
-    uint16_t *basisLo = nullptr;   // lo part
-    int Wp = 0;                    // W padded to a multiple of 16 (zero rows)
-    int colsAlloc = 0;             // 512 or 1024 (zero columns beyond cols)
-    int splitKind = -1;            // which 16-bit format basisHi/Lo hold (-1 none, 0 bf16, 1 fp16)
+    // Folded split basis for the MFMA path (even nfft and W, symmetric window).  With the frame
+    // centre c = (W-1)/2, e_j = a[c+0.5+j] + a[c-0.5-j] and o_j = a[c+0.5+j] - a[c-0.5-j]
+    // (j < W/2) give X_k = sum_j e_j w_j cos(2 pi k (j+0.5)/N) - i sum_j o_j w_j sin(...): half
+    // the contraction length.  Columns: [0, N/2) cos of bins 0..N/2-1, [N/2, N) sin of the same
+    // bins; bin N/2 (cos term zero) is formed separately from wHalf.  Element (j, c) sits at
+    // ((j / 8) * N + c) * 8 + j % 8 -- the operand order of a 32x32x16 MFMA B fragment.
+    uint16_t *foldHi = nullptr, *foldLo = nullptr;
+    float *wHalf = nullptr;        // [W/2] symmetrised window half, w_j = w[W/2 + j]
+    int Jp = 0;                    // W/2 padded to a multiple of 16 (zero rows)
+    int splitKind = -1;            // format of foldHi/foldLo (-1 none, 0 bf16, 1 fp16 scaled by 4096)
 };
 
 // Mel tables for one parameter set, built on the GPU (tables.hip).

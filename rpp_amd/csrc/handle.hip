@@ -60,8 +60,9 @@ RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
     if (handle->pinned) (void)hipHostFree(handle->pinned);
     for (auto &kv : handle->dft) {
         (void)hipFree(kv.second.basis);
-        if (kv.second.basisHi) (void)hipFree(kv.second.basisHi);
-        if (kv.second.basisLo) (void)hipFree(kv.second.basisLo);
+        if (kv.second.foldHi) (void)hipFree(kv.second.foldHi);
+        if (kv.second.foldLo) (void)hipFree(kv.second.foldLo);
+        if (kv.second.wHalf) (void)hipFree(kv.second.wHalf);
     }
     for (auto &kv : handle->mel) {
         (void)hipFree(kv.second.weightsDown);

@@ -55,8 +55,9 @@ struct LogMelArgs {
     const float *basis;
     int cols;
     // split basis
-    const uint16_t *bHi, *bLo;
-    int Wp, colsAlloc;
+    const uint16_t *bHi, *bLo;      // folded split basis (DftTables::foldHi/foldLo)
+    const float *wHalf;
+    int Wp;                         // folded contraction length, padded to 16
     float basisScale;
     // mel
     const float *weightsDown;
@@ -242,6 +243,7 @@ __global__ void __launch_bounds__(kThreads, 1)
 logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int aliasPw)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ __attribute__((aligned(16))) float sA[kKC][kFT32];
     float *seg = lds;
     float *sB = lds + segFloats;
     float *pw = aliasPw ? sB : sB + bFloats;
@@ -270,14 +272,16 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
                     int t = t0 + tt, c = cBase + cc;
                     sB[e] = (t < a.W && c < a.cols) ? a.basis[static_cast<size_t>(t) * a.cols + c] : 0.0f;
                 }
+                for (int e = tid; e < kKC * kFT32; e += kThreads) {  // the frames' samples of this chunk
+                    const int tt = e / kFT32, ff = e % kFT32, t = t0 + tt;
+                    sA[tt][ff] = (t < a.W) ? seg[seg_pos(ff * a.H + t, a.H)] : 0.0f;
+                }
                 __syncthreads();
 #pragma unroll 4
                 for (int kk = 0; kk < kKC; kk++) {
-                    const int t = t0 + kk;
                     float av[4];
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        av[i] = (t < a.W) ? seg[seg_pos((fg * 4 + i) * a.H + t, a.H)] : 0.0f;
+                    for (int i = 0; i < 4; i++) av[i] = sA[kk][fg * 4 + i];
                     float bv[16];
 #pragma unroll
                     for (int j = 0; j < 16; j++) bv[j] = sB[kk * kCC + cg * 16 + j];
@@ -339,14 +343,51 @@ __device__ __forceinline__ void split_f16(const float *v, float s, f16x8 &hi, f1
     }
 }
 
-template <int PREC, int FT, int COLS>
-__global__ void __launch_bounds__(kThreads, 2)
+template <int PREC>
+__device__ __forceinline__ void split8(const float *v, float s, u16x8 &hi, u16x8 &lo)
+{
+    if (PREC == kPrecFP16x3) {
+        f16x8 h, l;
+        split_f16(v, s, h, l);
+        hi = __builtin_bit_cast(u16x8, h);
+        lo = __builtin_bit_cast(u16x8, l);
+    } else {
+        bf16x8 h, l;
+        split_bf16(v, h, l);
+        hi = __builtin_bit_cast(u16x8, h);
+        lo = __builtin_bit_cast(u16x8, l);
+    }
+}
+
+// acc += A * B over the split operands: hi*hi + hi*lo + lo*hi (one product for kPrecBF16).
+template <int PREC>
+__device__ __forceinline__ void mfma3(f32x16 &acc, const u16x8 &ah, const u16x8 &al, const u16x8 &bh, const u16x8 &bl)
+{
+    if (PREC == kPrecFP16x3) {
+        const f16x8 a0 = __builtin_bit_cast(f16x8, ah), a1 = __builtin_bit_cast(f16x8, al);
+        const f16x8 b0 = __builtin_bit_cast(f16x8, bh), b1 = __builtin_bit_cast(f16x8, bl);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, acc, 0, 0, 0);
+    } else {
+        const bf16x8 a0 = __builtin_bit_cast(bf16x8, ah), a1 = __builtin_bit_cast(bf16x8, al);
+        const bf16x8 b0 = __builtin_bit_cast(bf16x8, bh), b1 = __builtin_bit_cast(bf16x8, bl);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+        if (PREC == kPrecBF16x3) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
+        }
+    }
+}
+
+// FT frames per tile; CTH 32-bin tiles per wave (each a cos tile and a sin tile), so the four
+// waves cover 4*CTH*32 = N/2 bins.  Folded DFT (see DftTables): contraction length W/2.
+template <int PREC, int FT, int CTH>
+__global__ void __launch_bounds__(kThreads, ((FT == 64 || CTH >= 4) ? 1 : 2))
 logmel_mfma_kernel(LogMelArgs a)
 {
-    constexpr int CPW = COLS / 4;       // columns per wave
-    constexpr int RT = FT / 32;         // 32-row tiles
-    constexpr int CT = CPW / 32;        // 32-column tiles per wave
-    static_assert(RT * CT <= 8, "accumulator budget");
+    constexpr int RT = FT / 32;         // 32-frame row tiles
+    static_assert(RT * 2 * CTH <= 8, "accumulator budget");
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[4];
     float *seg = lds;
@@ -354,8 +395,11 @@ logmel_mfma_kernel(LogMelArgs a)
     const int pws = a.KP + kSkew;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int T = a.tileStart[a.B];
-    const int span = (FT - 1) * a.H + a.Wp;
-    const bool evenN = !(a.N & 1);
+    const int half = a.W / 2;
+    const int span = (FT - 1) * a.H + a.W;
+    const int halfN = a.N / 2;
+    const int binBase = wave * (CTH * 32) + (lane & 31);     // cos column == bin; sin column = halfN + bin
+    const int jLane = 8 * (lane >> 5);
     for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
         const int b = owner_clip(a.tileStart, a.B, tile);
         const int f0 = (tile - a.tileStart[b]) * FT;
@@ -363,97 +407,123 @@ logmel_mfma_kernel(LogMelArgs a)
 
         float segScale = 1.0f;
         if (PREC == kPrecFP16x3) {
+            // e = up + down can reach twice the largest sample: keep 2*max*scale below 2^11
             float mx = stage_segment<true>(seg, a, v.x, v.L, f0, v.F, span, red);
-            if (mx > 0.0f) segScale = ldexpf(1.0f, 10 - ilogbf(mx));
+            if (mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));
         } else {
             stage_segment<false>(seg, a, v.x, v.L, f0, v.F, span, nullptr);
         }
         __syncthreads();
 
-        // ---- DFT on MFMA 32x32x16: acc[rt][ct] = frames (rt) x columns (ct) of this wave
-        f32x16 acc[RT][CT];
+        // ---- folded DFT on MFMA 32x32x16 (rolling one-step prefetch of the basis fragments)
+        f32x16 accC[RT][CTH], accS[RT][CTH];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++)
+            for (int ct = 0; ct < CTH; ct++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) acc[rt][ct][r] = 0.0f;
+                for (int r = 0; r < 16; r++) { accC[rt][ct][r] = 0.0f; accS[rt][ct][r] = 0.0f; }
 
-        const int colBase = wave * CPW + (lane & 31);
+        u16x8 bcH[CTH], bcL[CTH], bsH[CTH], bsL[CTH];
+        auto loadB = [&](int k0) {
+            const size_t row = static_cast<size_t>((k0 >> 3) + (lane >> 5)) * a.N;
+#pragma unroll
+            for (int ct = 0; ct < CTH; ct++) {
+                const size_t oc = (row + binBase + ct * 32) * 8, os = oc + static_cast<size_t>(halfN) * 8;
+                bcH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
+                bsH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
+                if (PREC != kPrecBF16) {
+                    bcL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
+                    bsL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
+                }
+            }
+        };
+        loadB(0);
         for (int k0 = 0; k0 < a.Wp; k0 += 16) {
-            const int tq = k0 + 8 * (lane >> 5);
-            bf16x8 ahb[RT], alb[RT];
-            f16x8 ahf[RT], alf[RT];
+            const int j0 = k0 + jLane;
+            u16x8 eH[RT], eL[RT], oH[RT], oL[RT];
 #pragma unroll
             for (int rt = 0; rt < RT; rt++) {
-                const int p = seg_pos((rt * 32 + (lane & 31)) * a.H + tq, a.H);
-                const float4 v0 = *reinterpret_cast<const float4 *>(&seg[p]);
-                const float4 v1 = *reinterpret_cast<const float4 *>(&seg[p + 4]);
-                const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-                if (PREC == kPrecFP16x3) split_f16(v, segScale, ahf[rt], alf[rt]);
-                else split_bf16(v, ahb[rt], alb[rt]);
-            }
-            const size_t rowOff = static_cast<size_t>((k0 >> 3) + (lane >> 5)) * a.colsAlloc;
+                float e[8], o[8];
+                if (j0 < half) {
+                    const int fb = (rt * 32 + (lane & 31)) * a.H;
+                    const int pu = seg_pos(fb + half + j0, a.H), pd = seg_pos(fb + half - 8 - j0, a.H);
+                    const float4 u0 = *reinterpret_cast<const float4 *>(&seg[pu]);
+                    const float4 u1 = *reinterpret_cast<const float4 *>(&seg[pu + 4]);
+                    const float4 d0 = *reinterpret_cast<const float4 *>(&seg[pd]);
+                    const float4 d1 = *reinterpret_cast<const float4 *>(&seg[pd + 4]);
+                    const float up[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+                    const float dn[8] = {d1.w, d1.z, d1.y, d1.x, d0.w, d0.z, d0.y, d0.x};  // reversed
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const size_t o = (rowOff + colBase + ct * 32) * 8;
-                const u16x8 bh = *reinterpret_cast<const u16x8 *>(a.bHi + o);
-                if (PREC == kPrecFP16x3) {
-                    const u16x8 bl = *reinterpret_cast<const u16x8 *>(a.bLo + o);
-                    const f16x8 bhf = __builtin_bit_cast(f16x8, bh), blf = __builtin_bit_cast(f16x8, bl);
-#pragma unroll
-                    for (int rt = 0; rt < RT; rt++) {
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[rt], bhf, acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[rt], blf, acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alf[rt], bhf, acc[rt][ct], 0, 0, 0);
-                    }
+                    for (int i = 0; i < 8; i++) { e[i] = up[i] + dn[i]; o[i] = up[i] - dn[i]; }
                 } else {
-                    const bf16x8 bhb = __builtin_bit_cast(bf16x8, bh);
-                    if (PREC == kPrecBF16x3) {
-                        const u16x8 bl = *reinterpret_cast<const u16x8 *>(a.bLo + o);
-                        const bf16x8 blb = __builtin_bit_cast(bf16x8, bl);
 #pragma unroll
-                        for (int rt = 0; rt < RT; rt++) {
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], bhb, acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], blb, acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alb[rt], bhb, acc[rt][ct], 0, 0, 0);
-                        }
-                    } else {
+                    for (int i = 0; i < 8; i++) { e[i] = 0.0f; o[i] = 0.0f; }
+                }
+                split8<PREC>(e, segScale, eH[rt], eL[rt]);
+                split8<PREC>(o, segScale, oH[rt], oL[rt]);
+            }
+            const bool more = k0 + 16 < a.Wp;
+            const size_t nrow = static_cast<size_t>(((k0 + 16) >> 3) + (lane >> 5)) * a.N;
 #pragma unroll
-                        for (int rt = 0; rt < RT; rt++)
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahb[rt], bhb, acc[rt][ct], 0, 0, 0);
+            for (int ct = 0; ct < CTH; ct++) {
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    mfma3<PREC>(accC[rt][ct], eH[rt], eL[rt], bcH[ct], bcL[ct]);
+                    mfma3<PREC>(accS[rt][ct], oH[rt], oL[rt], bsH[ct], bsL[ct]);
+                }
+                if (more) {  // refill this tile's fragments for the next step while the others run
+                    const size_t oc = (nrow + binBase + ct * 32) * 8, os = oc + static_cast<size_t>(halfN) * 8;
+                    bcH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
+                    bsH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
+                    if (PREC != kPrecBF16) {
+                        bcL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
+                        bsL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
                     }
                 }
             }
         }
+
+        // ---- bin N/2: its cos term vanishes with the half-sample origin; sin term on the VALU
+        float nyq[FT / 4];
+#pragma unroll
+        for (int i = 0; i < FT / 4; i++) {
+            const int f = wave + 4 * i;
+            float sum = 0.0f;
+            for (int j = lane; j < half; j += 64) {
+                const int fb = f * a.H;
+                const float d = seg[seg_pos(fb + half + j, a.H)] - seg[seg_pos(fb + half - 1 - j, a.H)];
+                const float w = (j & 1) ? -a.wHalf[j] : a.wHalf[j];
+                sum = fmaf(d, w, sum);
+            }
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+            nyq[i] = sum;
+        }
         __syncthreads();  // the segment is dead: pw may overwrite it
 
-        // ---- power: even lane of each (2k, 2k+1) column pair
+        // ---- power = cos^2 + sin^2, in lane (cos and sin tiles hold the same bins)
         const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
 #pragma unroll
         for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const int c = colBase + ct * 32;
-                const int p = c >> 1;
+            for (int ct = 0; ct < CTH; ct++) {
+                const int bin = binBase + ct * 32;
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const float v = acc[rt][ct][r] * unscale;
-                    const float partner = __shfl_xor(v, 1);
-                    if (!(lane & 1)) {
-                        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        if (p == 0) {
-                            float dc = v * v, nq = partner * partner;
-                            if (a.power == 1) { dc = sqrtf(dc); nq = sqrtf(nq); }
-                            pw[row * pws] = dc;
-                            if (evenN) pw[row * pws + a.N / 2] = nq;
-                        } else if (p < a.K && !(evenN && p == a.N / 2)) {
-                            float pv = v * v + partner * partner;
-                            if (a.power == 1) pv = sqrtf(pv);
-                            pw[row * pws + p] = pv;
-                        }
-                    }
+                    const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
+                    float p = re * re + im * im;
+                    if (a.power == 1) p = sqrtf(p);
+                    const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    pw[row * pws + bin] = p;
                 }
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < FT / 4; i++) {
+                float p = nyq[i] * nyq[i];
+                if (a.power == 1) p = sqrtf(p);
+                pw[(wave + 4 * i) * pws + halfN] = p;
             }
         }
         // zero the padding bins [K, KP) that the mel contraction reads
@@ -469,19 +539,26 @@ logmel_mfma_kernel(LogMelArgs a)
             const int mt = tt % nMT, ft = tt / nMT;
             f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
             const int fr = ft * 16 + (lane & 15);
+            size_t o = (static_cast<size_t>(lane >> 4) * a.Mpad + mt * 16 + (lane & 15)) * 8;
+            const size_t ostep = static_cast<size_t>(4) * a.Mpad * 8;  // 32 bins further
+            u16x8 wh = *reinterpret_cast<const u16x8 *>(a.mHi + o), wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
             for (int k0 = 0; k0 < a.KP; k0 += 32) {
-                const size_t o = (static_cast<size_t>((k0 >> 3) + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
-                const bf16x8 wh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(a.mHi + o));
-                const bf16x8 wl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(a.mLo + o));
+                const u16x8 ch = wh, cl = wl;
+                if (k0 + 32 < a.KP) {
+                    o += ostep;
+                    wh = *reinterpret_cast<const u16x8 *>(a.mHi + o);
+                    wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
+                }
                 const int q = fr * pws + k0 + 8 * (lane >> 4);
                 const float4 v0 = *reinterpret_cast<const float4 *>(&pw[q]);
                 const float4 v1 = *reinterpret_cast<const float4 *>(&pw[q + 4]);
-                const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                const float pv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
                 bf16x8 ph, pl;
-                split_bf16(v, ph, pl);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, ph, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, pl, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, ph, c, 0, 0, 0);
+                split_bf16(pv, ph, pl);
+                const bf16x8 w0 = __builtin_bit_cast(bf16x8, ch), w1 = __builtin_bit_cast(bf16x8, cl);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph, c, 0, 0, 0);
             }
             // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
             if (f0 + fr < v.F) {
@@ -508,7 +585,11 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
         return RPP_ERROR_INVALID_ARGUMENTS;
     if (!(p->referenceMagnitude > 0.0f)) return RPP_ERROR_NOT_IMPLEMENTED;  // per-clip max needs a second pass
     if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (precision != RPPT_LOGMEL_FP32 && ((p->windowStep % 8) != 0 || nfft > 1024)) return RPP_ERROR_NOT_IMPLEMENTED;
+    // the folded MFMA path: even nfft <= 1024 with N/2 a multiple of 128, window length a multiple of
+    // 16 (its half a multiple of 8), hop a multiple of 8
+    if (precision != RPPT_LOGMEL_FP32 &&
+        ((p->windowStep % 8) != 0 || (p->windowLength % 16) != 0 || nfft > 1024 || (nfft % 256) != 0))
+        return RPP_ERROR_NOT_IMPLEMENTED;
     return RPP_SUCCESS;
 }
 
@@ -557,10 +638,10 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.tileStart = tileStart;
     a.basis = dt->basis;
     a.cols = dt->cols;
-    a.bHi = dt->basisHi;
-    a.bLo = dt->basisLo;
-    a.Wp = dt->Wp;
-    a.colsAlloc = dt->colsAlloc;
+    a.bHi = dt->foldHi;
+    a.bLo = dt->foldLo;
+    a.wHalf = dt->wHalf;
+    a.Wp = dt->Jp;
     a.basisScale = splitKind == 1 ? 4096.0f : 1.0f;
     a.weightsDown = mt->weightsDown;
     a.intervals = mt->intervals;
@@ -575,7 +656,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     // tile shape: frames per work item and persistent blocks per CU
     int FT;
     if (!mfma) FT = 32;
-    else if (dt->colsAlloc > 512) FT = 32;
+    else if (nfft > 512) FT = 32;
     else FT = (h->tiles.framesPerTile == 64) ? 64 : 32;
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 2;
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
@@ -595,20 +676,22 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         hipLaunchKernelGGL(logmel_fp32_kernel, dim3(grid), dim3(kThreads), bytes, h->stream, a, segFloats, bFloats,
                            pwFloats, alias);
     } else {
-        const int span = (FT - 1) * a.H + a.Wp;
+        const int span = (FT - 1) * a.H + a.W;
         const int segFloats = seg_pos(span - 1, a.H) + 1 + 8;
         const int pwFloats = FT * pws;
         const size_t bytes = sizeof(float) * std::max(segFloats, pwFloats);
         if (bytes > 160 * 1024) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
-        const int cols = dt->colsAlloc;
+        const int cth = nfft / 256;  // 32-bin tiles per wave: N/2 bins over 4 waves
 #define RPPA_LAUNCH(PR)                                                                                     \
         do {                                                                                                \
-            if (FT == 64 && cols == 512)                                                                     \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 512>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
-            else if (cols == 512)                                                                            \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 512>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+            if (FT == 64 && cth == 2)                                                                        \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 2>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+            else if (cth == 2)                                                                               \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 2>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+            else if (cth == 4)                                                                               \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 4>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
             else                                                                                             \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 1024>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 1>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
         } while (0)
         if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
         else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);
@@ -663,6 +746,49 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
                          srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
                          dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr);
+}
+
+// Autotuner: times every tile shape the launch supports on the caller's batch (events on the
+// handle's stream, `iters` launches each after one warm-up) and keeps the fastest in the handle.
+RPPA_EXPORT RppStatus rppt_logmel_autotune(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                           RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                           const RpptLogMelParams *p, RpptLogMelPrecision precision, Rpp32s iters,
+                                           rppHandle_t h, RpptLogMelTiles *chosen, Rpp32f *bestMs)
+{
+    if (!h || !p || iters <= 0) return RPP_ERROR_INVALID_ARGUMENTS;
+    const int nfft = p->nfft ? p->nfft : p->windowLength;
+    std::vector<RpptLogMelTiles> cands;
+    const int fts[2] = {32, 64};
+    for (int ft : fts) {
+        if (ft == 64 && (precision == RPPT_LOGMEL_FP32 || nfft > 512)) continue;
+        for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
+    }
+    hipEvent_t e0, e1;
+    RPPA_HIP(hipEventCreate(&e0));
+    RPPA_HIP(hipEventCreate(&e1));
+    const RpptLogMelTiles saved = h->tiles;
+    float best = 1e30f;
+    RpptLogMelTiles bestT = saved;
+    for (const RpptLogMelTiles &c : cands) {
+        h->tiles = c;
+        RppStatus st = rppt_logmel_fused_gpu(srcPtr, srcDescPtr, dstPtr, dstDescPtr, srcLengthTensor, p, precision, h);
+        if (st != RPP_SUCCESS) { h->tiles = saved; return st; }
+        RPPA_HIP(hipEventRecord(e0, h->stream));
+        for (int i = 0; i < iters; i++)
+            (void)rppt_logmel_fused_gpu(srcPtr, srcDescPtr, dstPtr, dstDescPtr, srcLengthTensor, p, precision, h);
+        RPPA_HIP(hipEventRecord(e1, h->stream));
+        RPPA_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        RPPA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= iters;
+        if (ms < best) { best = ms; bestT = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->tiles = bestT;
+    if (chosen) *chosen = bestT;
+    if (bestMs) *bestMs = best;
+    return RPP_SUCCESS;
 }
 
 RPPA_EXPORT RppStatus rppt_logmel_fused_packed_gpu(const Rpp32f *srcPtr, const long long *srcOffsets,

@@ -47,45 +47,60 @@ __device__ inline float f16_to_f32(uint16_t u)
     return static_cast<float>(h);
 }
 
-// One thread per (t, c) of the padded [Wp][colsAlloc] grid.  basis[t][c] (t < W, c < cols) in FP32;
-// optional split copies in MFMA operand order (see common.hpp).
-__global__ void dft_basis_kernel(int N, int W, int cols, int Wp, int colsAlloc, const float *__restrict__ win,
-                                 float *__restrict__ basis, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
-                                 int splitKind, float splitScale)
+// One thread per (t, c): basis[t][c] = window-folded DFT column c at sample t, FP64 -> FP32.
+__global__ void dft_basis_kernel(int N, int W, int cols, const float *__restrict__ win, float *__restrict__ basis)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int t = blockIdx.y;
-    int cMax = splitKind >= 0 ? colsAlloc : cols;
-    if (c >= cMax || t >= (splitKind >= 0 ? Wp : W)) return;
+    if (c >= cols || t >= W) return;
+    double w = static_cast<double>(win[t]);
+    int k;
+    bool isSin = false;
+    if (c == 0) k = 0;
+    else if (c == 1) k = (N % 2 == 0) ? N / 2 : -1;
+    else { k = c >> 1; isSin = (c & 1) != 0; }
     double v = 0.0;
-    if (t < W && c < cols) {
-        double w = static_cast<double>(win[t]);
-        int k;
-        bool isSin = false;
-        if (c == 0) k = 0;
-        else if (c == 1) k = (N % 2 == 0) ? N / 2 : -1;
-        else { k = c >> 1; isSin = (c & 1) != 0; }
-        if (k >= 0) {
-            long long kt = (static_cast<long long>(k) * t) % N;
-            double th = 2.0 * kPi * static_cast<double>(kt) / N;
-            v = isSin ? -w * sin(th) : w * cos(th);
-        }
-        basis[static_cast<size_t>(t) * cols + c] = static_cast<float>(v);
+    if (k >= 0) {
+        long long kt = (static_cast<long long>(k) * t) % N;
+        double th = 2.0 * kPi * static_cast<double>(kt) / N;
+        v = isSin ? -w * sin(th) : w * cos(th);
     }
-    if (splitKind >= 0) {
-        double s = v * splitScale;
-        uint16_t h, l;
-        if (splitKind == 0) {
-            h = f32_to_bf16_rne(static_cast<float>(s));
-            l = f32_to_bf16_rne(static_cast<float>(s - bf16_to_f32(h)));
-        } else {
-            h = f32_to_f16_rne(static_cast<float>(s));
-            l = f32_to_f16_rne(static_cast<float>(s - f16_to_f32(h)));
-        }
-        size_t o = (static_cast<size_t>(t / 8) * colsAlloc + c) * 8 + (t % 8);
-        hi[o] = h;
-        lo[o] = l;
+    basis[static_cast<size_t>(t) * cols + c] = static_cast<float>(v);
+}
+
+// One thread per (j, c) of the folded split basis (see common.hpp).  The window is symmetrised
+// (mean of w[W/2+j] and w[W/2-1-j], equal for RPP's Hann up to FP32 rounding).  Angles are
+// reduced exactly: k (2j+1) mod 2N over 2N.
+__global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict__ win, uint16_t *__restrict__ hi,
+                                  uint16_t *__restrict__ lo, float *__restrict__ wHalf, int splitKind,
+                                  float splitScale)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (c >= N || j >= Jp) return;
+    const int half = W / 2;
+    double v = 0.0;
+    if (j < half) {
+        const double w = 0.5 * (static_cast<double>(win[half + j]) + static_cast<double>(win[half - 1 - j]));
+        if (c == 0) wHalf[j] = static_cast<float>(w);
+        const bool isSin = c >= N / 2;
+        const long long k = isSin ? c - N / 2 : c;
+        const long long m = (k * (2LL * j + 1)) % (2LL * N);
+        const double th = kPi * static_cast<double>(m) / N;
+        v = isSin ? -w * sin(th) : w * cos(th);
     }
+    const double s = v * splitScale;
+    uint16_t h, l;
+    if (splitKind == 0) {
+        h = f32_to_bf16_rne(static_cast<float>(s));
+        l = f32_to_bf16_rne(static_cast<float>(s - bf16_to_f32(h)));
+    } else {
+        h = f32_to_f16_rne(static_cast<float>(s));
+        l = f32_to_f16_rne(static_cast<float>(s - f16_to_f32(h)));
+    }
+    const size_t o = (static_cast<size_t>(j / 8) * N + c) * 8 + (j % 8);
+    hi[o] = h;
+    lo[o] = l;
 }
 
 // bf16 hi/lo split of the dense mel matrix in MFMA operand order (see common.hpp).
@@ -207,16 +222,16 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
     if (it != h->dft.end() && (splitKind < 0 || it->second.splitKind == splitKind)) return &it->second;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
+    if (splitKind >= 0 && ((nfft & 1) || (W & 1))) return nullptr;  // the fold needs even nfft and W
 
     DftTables t;
-    if (it != h->dft.end()) {
-        t = it->second;  // reuse the FP32 basis, (re)build the split copy
+    const bool fresh = it == h->dft.end();
+    if (!fresh) {
+        t = it->second;
     } else {
         t.nfft = nfft;
         t.W = W;
         t.cols = nfft + (nfft & 1);
-        t.Wp = (W + 15) / 16 * 16;
-        t.colsAlloc = t.cols <= 512 ? 512 : 1024;
     }
     float *win = nullptr;
     if (hipMalloc(&win, sizeof(float) * W) != hipSuccess) return nullptr;
@@ -225,18 +240,23 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
     } else {
         hipLaunchKernelGGL(hann_kernel, dim3(ceil_div(W, 256)), dim3(256), 0, h->stream, W, win);
     }
-    if (!t.basis && hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * t.cols) != hipSuccess) return nullptr;
-    if (splitKind >= 0) {
-        if (t.basisHi) { (void)hipFree(t.basisHi); (void)hipFree(t.basisLo); }
-        size_t n = static_cast<size_t>(t.colsAlloc) * t.Wp;
-        if (hipMalloc(&t.basisHi, n * 2) != hipSuccess || hipMalloc(&t.basisLo, n * 2) != hipSuccess) return nullptr;
-        t.splitKind = splitKind;
+    if (fresh) {
+        if (hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * t.cols) != hipSuccess) return nullptr;
+        hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(t.cols, 256), W), dim3(256), 0, h->stream, nfft, W, t.cols,
+                           win, t.basis);
     }
-    float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
-    hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(splitKind >= 0 ? t.colsAlloc : t.cols, 256),
-                       splitKind >= 0 ? t.Wp : W), dim3(256), 0,
-                       h->stream, nfft, W, t.cols, t.Wp, t.colsAlloc, win, t.basis, t.basisHi, t.basisLo, splitKind,
-                       splitScale);
+    if (splitKind >= 0) {
+        if (t.foldHi) { (void)hipFree(t.foldHi); (void)hipFree(t.foldLo); (void)hipFree(t.wHalf); }
+        t.Jp = (W / 2 + 15) / 16 * 16;
+        const size_t n = static_cast<size_t>(t.Jp) * nfft;
+        if (hipMalloc(&t.foldHi, n * 2) != hipSuccess || hipMalloc(&t.foldLo, n * 2) != hipSuccess ||
+            hipMalloc(&t.wHalf, sizeof(float) * (W / 2)) != hipSuccess)
+            return nullptr;
+        t.splitKind = splitKind;
+        const float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
+        hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft, 256), t.Jp), dim3(256), 0, h->stream, nfft, W,
+                           t.Jp, win, t.foldHi, t.foldLo, t.wHalf, splitKind, splitScale);
+    }
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
     (void)hipFree(win);
     h->dft[k] = t;

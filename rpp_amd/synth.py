@@ -153,6 +153,24 @@ def workload4(batch: int = 64, seconds: float = 60.0, seed: int = SEED0 + 4) -> 
     return Batch(x, L, 48000.0, channels=2, params=dict(FEATURES_16K))
 
 
+def kaiser_sinc_lut(coeffs: int = 4091, lobes: int = 16, beta: float = 8.0):
+    """Resampling table for workload 4: a Kaiser-windowed sinc (16 zero-crossings each side) in the
+    layout RpptResamplingWindow expects (the reference test suite's builder,
+    utilities/test_suite/rpp_test_suite_audio.h:390-430, with the Hann envelope replaced by a
+    Kaiser one): lookupSize = coeffs + 5, lut[1 + i] for i < coeffs, zero elsewhere.  With
+    coeffs = 4091 the table has 4096 entries.  Returns (lut float32, scale, center)."""
+    scale = np.float32(2.0 * lobes / (coeffs - 1))
+    center = int((coeffs - 1) * 0.5)
+    lut = np.zeros(coeffs + 5, np.float32)
+    i = np.arange(coeffs)
+    x = (i - center).astype(np.float64) * scale
+    y = (i - center).astype(np.float64) / center
+    sinc = np.sinc(x)
+    env = np.i0(beta * np.sqrt(np.clip(1.0 - y * y, 0.0, 1.0))) / np.i0(beta)
+    lut[1:coeffs + 1] = (sinc * env).astype(np.float32)
+    return lut, float(np.float32(1.0) / scale), float(center + 1)
+
+
 def make(workload: int, batch: int | None = None, **kw) -> Batch:
     fn = {0: workload0, 1: workload1, 2: workload2, 4: workload4}[workload]
     return fn(batch, **kw) if batch is not None else fn(**kw)
