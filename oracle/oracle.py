@@ -71,6 +71,7 @@ def lib():
         l.orc_down_mix_f32.argtypes = [f32p, L, f32p, L, i32p, I, I, I]
         l.orc_resample_f32.argtypes = [f32p, L, f32p, L, f32p, f32p, i32p, I, F, F, I, I, f32p, I]
         l.orc_windowed_sinc_lut.argtypes = [I, I, f32p, C.POINTER(F), C.POINTER(F)]
+        l.orc_nsr_f32.argtypes = [f32p, L, i32p, I, F, I, F, I, i32p, i32p, I]
         l.orc_num_threads_available.restype = I
         _LIB = l
     return _LIB
@@ -90,7 +91,9 @@ def ref_lib():
         r.ref_to_decibels.argtypes = [f32p, U, U, f32p, U, U, u32p, I, F, F, F, I]
         r.ref_down_mixing.argtypes = [f32p, U, U, f32p, U, i32p, I, I, I]
         r.ref_resample.argtypes = [f32p, U, f32p, U, f32p, f32p, i32p, I, F, F, I, I, I, f32p, I]
-        for fn in ("ref_pre_emphasis", "ref_mel_filter_bank", "ref_to_decibels", "ref_down_mixing", "ref_resample"):
+        r.ref_non_silent_region_detection.argtypes = [f32p, U, i32p, I, F, I, F, I, i32p, i32p, I]
+        for fn in ("ref_pre_emphasis", "ref_mel_filter_bank", "ref_to_decibels", "ref_down_mixing", "ref_resample",
+                   "ref_non_silent_region_detection"):
             getattr(r, fn).restype = I
         _REF = r
     return _REF
@@ -304,6 +307,18 @@ def resample_f32(x, lengths, in_rate, out_rate, lut, scale, center, lobes, chann
     lut = np.ascontiguousarray(lut, np.float32)
     lib().orc_resample_f32(x, S, y, out_stride, ir, orr, dims, B, scale, center, lobes, len(lut), lut, nthreads)
     return y
+
+
+def non_silent_region_f32(x, lengths, cutoff_db=-60.0, window_length=2048, reference_power=0.0,
+                          reset_interval=8192, nthreads=1):
+    """Returns (begin[B], length[B]) int32 (src/modules/tensor/cpu/kernel/non_silent_region_detection.cpp:93-191)."""
+    x = np.ascontiguousarray(x, np.float32)
+    B, S = x.shape
+    lens = np.ascontiguousarray(lengths, np.int32)
+    b = np.zeros(B, np.int32)
+    n = np.zeros(B, np.int32)
+    lib().orc_nsr_f32(x, S, lens, B, cutoff_db, window_length, reference_power, reset_interval, b, n, nthreads)
+    return b, n
 
 
 # ----------------------------------------------------------------------------------------------

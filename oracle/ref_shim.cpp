@@ -96,4 +96,14 @@ int ref_resample(float *src, unsigned srcStride, float *dst, unsigned dstStride,
     return resample_host_tensor(src, &s, dst, &d, inRate, outRate, srcDims, w, rpp::deref(hh.h));
 }
 
+int ref_non_silent_region_detection(float *src, unsigned srcStride, int *len, int B, float cutOffDB,
+                                    int windowLength, float referencePower, int resetInterval, int *begin,
+                                    int *length, int threads)
+{
+    RpptDesc s = desc_1d(B, srcStride, 1);
+    HostHandle hh(B, threads);
+    return non_silent_region_detection_host_tensor(src, &s, len, begin, length, cutOffDB, windowLength,
+                                                   referencePower, resetInterval, rpp::deref(hh.h));
+}
+
 }  // extern "C"

@@ -738,6 +738,62 @@ ORC_API void orc_windowed_sinc_lut(int coeffs, int lobes, float *lut, float *out
     *outScale = 1 / scale;
 }
 
+/* ------------------------------------------------------------------ */
+/* non-silent region detection (src/modules/tensor/cpu/kernel/non_silent_region_detection.cpp:93-191) */
+/* moving mean square over windowLength samples, restarted every resetInterval outputs,    */
+/* then the first/last index whose value reaches base * 10^(cutOffDB/10), base = max MMS   */
+/* (referencePower == 0) or referencePower; the begin is widened by windowLength - 1.       */
+/* ------------------------------------------------------------------ */
+ORC_API void orc_nsr_f32(const float *src, long srcStride, const int *len, int B, float cutOffDB, int windowLength,
+                         float referencePower, int resetInterval, int *outBegin, int *outLength, int nthreads)
+{
+    const float cutOff = powf(10.0f, cutOffDB * 0.1f);
+    int b;
+#pragma omp parallel for num_threads(orc_threads(nthreads)) schedule(dynamic, 1)
+    for (b = 0; b < B; b++) {
+        const float *x = src + (long)b * srcStride;
+        const int L = len[b];
+        float *mms = (float *)calloc((size_t)(L > 0 ? L : 1), sizeof(float));
+        const int resetLength = (resetInterval == -1) ? L : resetInterval;
+        const float meanFactor = 1.0f / windowLength;
+        int windowBegin = -windowLength + 1;
+        for (int outPos = 0; outPos < L;) {
+            float sum = 0.0f;
+            for (int i = windowBegin > 0 ? windowBegin : 0; i < outPos; i++) { float q = x[i] * x[i]; sum = sum + q; }
+            int end = outPos + resetLength < L ? outPos + resetLength : L;
+            for (; outPos < end; outPos++, windowBegin++) {
+                float q = x[outPos] * x[outPos];
+                sum = sum + q;
+                mms[outPos] = sum * meanFactor;
+                if (windowBegin >= 0) { float r = x[windowBegin] * x[windowBegin]; sum = sum - r; }
+            }
+        }
+        float base = referencePower;
+        if (referencePower == 0.0f) {
+            base = mms[0];
+            for (int i = 1; i < L; i++) if (base < mms[i]) base = mms[i];
+        }
+        const float cut = base * cutOff;
+        int endIdx = L, beginIdx = L, dBegin, dLen;
+        for (int i = 0; i < endIdx; i++) if (mms[i] >= cut) { beginIdx = i; break; }
+        if (beginIdx == endIdx) { dBegin = 0; dLen = 0; }
+        else {
+            for (int i = endIdx - 1; i >= beginIdx; i--) if (mms[i] >= cut) { endIdx = i; break; }
+            dBegin = beginIdx;
+            dLen = endIdx - beginIdx + 1;
+        }
+        if (dBegin != 0 && dLen != 0) {
+            int nb = dBegin - (windowLength - 1);
+            if (nb < 0) nb = 0;
+            dLen += dBegin - nb;
+            dBegin = nb;
+        }
+        outBegin[b] = dBegin;
+        outLength[b] = dLen;
+        free(mms);
+    }
+}
+
 ORC_API int orc_num_threads_available(void)
 {
 #ifdef _OPENMP

@@ -293,3 +293,41 @@ def test_ref_resample(in_rate, out_rate):
     for b, n in enumerate(L):
         k = O.resample_out_length(int(n), in_rate, out_rate)
         assert np.array_equal(o[b, :k], y[b, :k])
+
+
+# ------------------------------------------------------------------ non-silent region detection
+def test_golden_non_silent_region(golden):
+    """Golden outputs hard-coded in utilities/test_suite/rpp_test_suite_audio.h:64-69 for the QA
+    parameters of HOST/Tensor_audio_host.cpp:196-206."""
+    b, n = O.non_silent_region_f32(golden["x"], golden["lengths"], -60.0, 2048, 0.0, 8192)
+    assert list(zip(b, n)) == [(0, 35840), (0, 33680), (0, 34160)]
+
+
+def test_non_silent_region_closed_form():
+    """A constant burst in silence: the moving mean square is non-zero from the burst's first sample
+    to windowLength - 1 samples past its last, and the begin is then widened by windowLength - 1,
+    so the region is [8000 - 99, 9000 + 99).  An all-zero clip has base 0 and cut-off 0, which
+    every sample meets: the whole clip is reported (the reference's >= comparison)."""
+    x = np.zeros((2, 20000), np.float32)
+    x[0, 8000:9000] = 0.5
+    b, n = O.non_silent_region_f32(x, [20000, 20000], -60.0, 100, 0.0, 4096)
+    assert (b[0], n[0]) == (8000 - 99, 1000 + 99 + 99)
+    assert (b[1], n[1]) == (0, 20000)
+
+
+@needs_ref
+@pytest.mark.parametrize("ref_power,reset", [(0.0, 8192), (0.01, 1000), (0.0, -1)])
+def test_ref_non_silent_region(ref_power, reset):
+    r = O.ref_lib()
+    rng = np.random.default_rng(13)
+    L = np.array([30000, 5000, 20000, 777], np.int32)
+    x = np.zeros((4, 30008), np.float32)
+    for i, l in enumerate(L):
+        env = np.zeros(l)
+        env[l // 4: l // 2] = 1.0
+        x[i, :l] = (rng.normal(size=l) * (0.001 + env)).astype(np.float32)
+    b0 = np.zeros(4, np.int32)
+    n0 = np.zeros(4, np.int32)
+    assert r.ref_non_silent_region_detection(x, 30008, L, 4, -40.0, 512, ref_power, reset, b0, n0, 2) == 0
+    b1, n1 = O.non_silent_region_f32(x, L, -40.0, 512, ref_power, reset)
+    assert np.array_equal(b0, b1) and np.array_equal(n0, n1)
