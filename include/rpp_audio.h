@@ -178,6 +178,15 @@ RppStatus rppt_to_decibels_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t
 RppStatus rppt_down_mixing_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
                                Rpp32s *srcDimsTensor, bool normalizeWeights, rppHandle_t rppHandle);
 
+/* api/rppt_tensor_audio_augmentations.h:86; dispatch rppt_tensor_audio_augmentations.cpp:332-369.
+ * Begin and length of each clip's non-silent region from the moving mean square of the HOST
+ * kernel (src/modules/tensor/cpu/kernel/non_silent_region_detection.cpp:93-191), evaluated in
+ * its order.  srcLengthTensor and both outputs in device or pinned memory; asynchronous. */
+RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, Rpp32s *srcLengthTensor,
+                                               Rpp32s *detectedIndexTensor, Rpp32s *detectionLengthTensor,
+                                               Rpp32f cutOffDB, Rpp32s windowLength, Rpp32f referencePower,
+                                               Rpp32s resetInterval, rppHandle_t rppHandle);
+
 #ifdef __cplusplus
 /* api/rppt_tensor_audio_augmentations.h:310; dispatch rppt_tensor_audio_augmentations.cpp:539-575.
  * Windowed-sinc resampling with the caller's lookup table. */

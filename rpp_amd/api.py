@@ -100,21 +100,16 @@ def lib():
         l.rppt_mel_filter_bank_gpu.argtypes = [P, DP, P, DP, P, C.c_float, C.c_float, C.c_int, C.c_int32,
                                                C.c_float, C.c_bool, P]
         l.rppt_to_decibels_gpu.argtypes = [P, DP, P, DP, P, C.c_float, C.c_float, C.c_float, P]
-        for name in ("rppt_down_mixing_gpu", "rppt_resample_gpu", "rppt_logmel_fused_gpu",
-                     "rppt_logmel_set_tiles", "rppt_logmel_get_tiles"):
-            if hasattr(l, name):
-                pass
-        if hasattr(l, "rppt_down_mixing_gpu"):
-            l.rppt_down_mixing_gpu.argtypes = [P, DP, P, DP, P, C.c_bool, P]
-        if hasattr(l, "rppt_resample_gpu"):
-            l.rppt_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.POINTER(RpptResamplingWindow), P]
+        l.rppt_non_silent_region_detection_gpu.argtypes = [P, DP, P, P, P, C.c_float, C.c_int32, C.c_float,
+                                                           C.c_int32, P]
+        l.rppt_down_mixing_gpu.argtypes = [P, DP, P, DP, P, C.c_bool, P]
+        l.rppt_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.POINTER(RpptResamplingWindow), P]
         l.rppt_logmel_chain_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
-        if hasattr(l, "rppt_logmel_fused_gpu"):
-            l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
-            l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
-            l.rppt_logmel_get_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
-            l.rppt_logmel_autotune.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, C.c_int32, P,
-                                               C.POINTER(RpptLogMelTiles), C.POINTER(C.c_float)]
+        l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
+        l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
+        l.rppt_logmel_get_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
+        l.rppt_logmel_autotune.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, C.c_int32, P,
+                                           C.POINTER(RpptLogMelTiles), C.POINTER(C.c_float)]
         l.rppa_debug_mel_tables.argtypes = [P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                                             C.c_int, P, P, P, P]
         _LIB = l
@@ -250,6 +245,18 @@ def down_mixing(h: Handle, src, dst, src_dims, normalize=False, channels=2, chec
     st = lib().rppt_down_mixing_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(src_dims), normalize, h.h)
     if check_status:
         check("rppt_down_mixing_gpu", st)
+    return st
+
+
+def non_silent_region_detection(h: Handle, src, lengths, begin, length, cutoff_db=-60.0, window_length=2048,
+                                reference_power=0.0, reset_interval=8192, check_status=True) -> int:
+    """rppt_non_silent_region_detection_gpu: begin[b], length[b] (int32 device tensors) of clip b."""
+    B, S = src.shape
+    sd = desc_1d(B, S)
+    st = lib().rppt_non_silent_region_detection_gpu(ptr(src), C.byref(sd), ptr(lengths), ptr(begin), ptr(length),
+                                                    cutoff_db, window_length, reference_power, reset_interval, h.h)
+    if check_status:
+        check("rppt_non_silent_region_detection_gpu", st)
     return st
 
 

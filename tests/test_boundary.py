@@ -23,7 +23,8 @@ def test_header_declares_the_reference_entry_points():
     names = declared_functions()
     for n in ("rppCreate", "rppDestroy", "rppSetStream", "rppGetStream", "rppSetBatchSize", "rppGetBatchSize",
               "rppt_pre_emphasis_filter_gpu", "rppt_spectrogram_gpu", "rppt_mel_filter_bank_gpu",
-              "rppt_to_decibels_gpu", "rppt_down_mixing_gpu", "rppt_resample_gpu"):
+              "rppt_to_decibels_gpu", "rppt_down_mixing_gpu", "rppt_resample_gpu",
+              "rppt_non_silent_region_detection_gpu"):
         assert n in names, n
 
 
