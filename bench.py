@@ -35,6 +35,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def recorded_traffic(workload, precision, tiles):
+    """HBM bytes per launch of the fused kernel for this configuration, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs summarised under profiles/ (a `traffic:` line per configuration,
+    counters already scaled by the copy-kernel calibration), or None when none matches."""
+    import glob
+    key = {"workload": str(workload), "precision": precision, "frames_per_tile": str(tiles[0]),
+           "blocks_per_cu": str(tiles[1])}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if not line.startswith("traffic:"):
+                continue
+            kv = dict(f.split("=", 1) for f in line.split()[1:] if "=" in f)
+            if all(kv.get(k) == v for k, v in key.items()):
+                return int(kv["bytes_per_launch"]), os.path.basename(path)
+    return None
+
+
 def front_end_measurement(api, synth, torch, precision, batch=16, seconds=60.0, steps=5):
     """Workload 4 (reduced to `batch` clips): stereo int16-range 48 kHz clips, 60 s each ->
     down-mix -> Kaiser-windowed sinc resample to 16 kHz (16 zero-crossings, 4096-entry table) ->
@@ -116,6 +133,8 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "bf16", "fp32"])
     ap.add_argument("--no-side", action="store_true", help="skip side measurements (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (known-size kernel for HBM counters)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="duration of the served-stream measurement (Poisson arrivals, 4000 clips/s); 0 skips it")
     a = ap.parse_args()
@@ -128,6 +147,14 @@ def main() -> int:
     from rpp_amd import api, synth
 
     torch.cuda.set_device(0)
+    if a.copy_calibration_mb:
+        n = a.copy_calibration_mb * (1 << 20) // 4
+        src_c = torch.rand(n, device="cuda")
+        dst_c = torch.empty_like(src_c)
+        for _ in range(3):
+            dst_c.copy_(src_c)
+        torch.cuda.synchronize()
+        del src_c, dst_c
     bt = synth.make(a.workload)
     feat = bt.params
     B = len(bt.lengths)
@@ -189,10 +216,14 @@ def main() -> int:
     flops = float(frames.sum()) * flop_per_frame
     kernel_s = gpu_s / a.steps
     achieved_tf = flops / kernel_s / 1e12
+    rec = recorded_traffic(a.workload, a.precision, tuned[a.precision][0])
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MI355X_PEAK_TFLOPS[a.precision],
                 "unit": "TFLOP/s", "frac": round(achieved_tf / MI355X_PEAK_TFLOPS[a.precision], 4),
-                "traffic": None, "flop_per_launch": flops, "kernel_ms": round(kernel_s * 1e3, 4),
+                "traffic": rec[0] if rec else None, "flop_per_launch": flops, "kernel_ms": round(kernel_s * 1e3, 4),
                 "hbm_bytes_algorithmic": int(4 * bt.lengths.sum() + 4 * feat["nmel"] * frames.sum())}
+    if rec:
+        roofline["traffic_source"] = "profiles/" + rec[1]
+        roofline["hbm_gbs_at_traffic"] = round(rec[0] / kernel_s / 1e9, 1)
 
     side = {}
     cpu = None
@@ -225,18 +256,18 @@ def main() -> int:
         ref64, _, fr64 = oracle.logmel_f64(bt.x[:n_chk], bt.lengths[:n_chk], nthreads=os.cpu_count() or 1, **kw)
         side["max_db_err_vs_fp64"] = round(max_db_err(head[:n_chk].cpu().numpy(), ref64, fr64), 6)
         side["max_db_err_sample_clips"] = n_chk
-        # CPU baseline: the oracle's RPP-HOST-equivalent FP32 chain on the host cores
+        # CPU baseline: the oracle's RPP-HOST-equivalent FP32 chain on the host cores, over the
+        # whole batch, repeated until about 10 s of CPU time has been spent
         nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
-        sample = min(B, 128)
         oracle.logmel_f32(bt.x[:8], bt.lengths[:8], nthreads=nthr, **kw)
         reps = []
-        for _ in range(3):
+        t_all = time.perf_counter()
+        while len(reps) < 3 or time.perf_counter() - t_all < 10.0:
             t0 = time.perf_counter()
-            oracle.logmel_f32(bt.x[:sample], bt.lengths[:sample], nthreads=nthr, **kw)
+            oracle.logmel_f32(bt.x, bt.lengths, nthreads=nthr, **kw)
             reps.append(time.perf_counter() - t0)
-        cpu_as = float(bt.lengths[:sample].sum()) / feat["sample_rate"]
-        cpu = {"value": round(cpu_as / float(np.median(reps)), 1), "unit": "audio-s/s", "cores": nthr, "kind": "port",
-               "sample": f"first {sample} clips of the workload ({cpu_as:.0f} audio-s), median of 3 runs"}
+        cpu = {"value": round(audio_s / float(np.median(reps)), 1), "unit": "audio-s/s", "cores": nthr, "kind": "port",
+               "sample": f"the whole batch ({audio_s:.0f} audio-s) x {len(reps)} runs, {sum(reps):.1f} s, median run"}
         log(f"side: {side}  cpu: {cpu}")
 
     if not a.no_side:

@@ -294,21 +294,26 @@ RPPA_EXPORT RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLog
             return RPP_ERROR_NOT_ENOUGH_MEMORY;
         }
     }
-    // build every table and size every workspace before traffic arrives
-    const int warm = static_cast<int>(std::min<size_t>(4096, inCap));
-    const bool canWarm = static_cast<size_t>(frames_of(s, warm)) * s->M <= outCap;
+    // build every table and size every workspace before traffic arrives: one launch per slot with
+    // capB short zero clips, the largest clip count a batch can have (a workspace that grew later
+    // would free its old buffer, which waits for the whole device)
+    const int warmLen = static_cast<int>(std::max<size_t>(4, std::min<size_t>(4096, inCap / capB / 4 * 4)));
+    const bool canWarm = static_cast<size_t>(warmLen) * capB <= inCap &&
+                         static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB <= outCap;
     for (Slot &sl : s->slots) {
         if (!canWarm) break;
-        long long zero = 0;
-        std::memcpy(sl.hMeta, &zero, sizeof(zero));
-        std::memcpy(sl.hMeta + capB, &zero, sizeof(zero));
-        std::memcpy(reinterpret_cast<int *>(sl.hMeta + 2 * capB), &warm, sizeof(warm));
-        std::memset(sl.hIn, 0, sizeof(float) * warm);
-        (void)hipMemcpyAsync(sl.dIn, sl.hIn, sizeof(float) * warm, hipMemcpyHostToDevice, sl.stream);
+        long long *srcOff = sl.hMeta, *dstOff = sl.hMeta + capB;
+        int *len = reinterpret_cast<int *>(sl.hMeta + 2 * capB);
+        for (int b = 0; b < capB; b++) {
+            srcOff[b] = static_cast<long long>(b) * warmLen;
+            dstOff[b] = static_cast<long long>(b) * frames_of(s, warmLen) * s->M;
+            len[b] = warmLen;
+        }
+        (void)hipMemsetAsync(sl.dIn, 0, sizeof(float) * warmLen * capB, sl.stream);
         (void)hipMemcpyAsync(sl.dMeta, sl.hMeta, sizeof(long long) * 2 * capB + sizeof(int) * capB,
                              hipMemcpyHostToDevice, sl.stream);
-        RppStatus st = rppt_logmel_fused_packed_gpu(sl.dIn, sl.dMeta, reinterpret_cast<int *>(sl.dMeta + 2 * capB), 1,
-                                                    sl.dOut, sl.dMeta + capB, params, precision, sl.h);
+        RppStatus st = rppt_logmel_fused_packed_gpu(sl.dIn, sl.dMeta, reinterpret_cast<int *>(sl.dMeta + 2 * capB),
+                                                    capB, sl.dOut, sl.dMeta + capB, params, precision, sl.h);
         (void)hipStreamSynchronize(sl.stream);
         if (st != RPP_SUCCESS) {
             destroy_server(s);
