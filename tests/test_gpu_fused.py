@@ -160,3 +160,26 @@ def test_fused_status_codes(gpu, handle):
     assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -6
     p = gpu.logmel_params(max_freq=9000.0)
     assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -2
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "fp32"])
+def test_fused_grid_and_alignment_invariance(gpu, orc, precision):
+    """Every tile is computed the same wherever it runs: the output does not depend on the number of
+    persistent blocks, and clips whose samples are not 16-byte aligned (a row stride of S + 1, which
+    sends every tile down the generic staging path) give the same bits as aligned ones (the
+    prefetched fast path)."""
+    bt = synth.workload1(batch=24, max_seconds=6.0)
+    L = bt.lengths
+    S = bt.x.shape[1]
+    x1 = np.zeros((len(L), S + 1), np.float32)
+    x1[:, :S] = bt.x
+    outs = []
+    for bpc, x in ((1, bt.x), (3, bt.x), (1, x1)):
+        h = gpu.Handle(len(L))
+        gpu.set_tiles(h, 0, bpc)
+        out, frames = run_fused(gpu, h, x, L, bt.params, precision)
+        h.close()
+        outs.append(out)
+    for b, F in enumerate(frames):
+        assert np.array_equal(outs[0][b, :, :F], outs[1][b, :, :F]), ("blocks per CU", b)
+        assert np.array_equal(outs[0][b, :, :F], outs[2][b, :, :F]), ("alignment", b)
