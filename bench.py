@@ -27,6 +27,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# revision of the fused MFMA kernel; recorded traffic (profiles/*.md `traffic:` lines) applies only to
+# the revision it was measured on
+KERNEL_REVISION = "wave8-ring"
 MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 MI355X_HBM_GBS = 8000.0
 
@@ -40,8 +43,8 @@ def recorded_traffic(workload, precision, tiles):
     FETCH_SIZE/WRITE_SIZE runs summarised under profiles/ (a `traffic:` line per configuration,
     counters already scaled by the copy-kernel calibration), or None when none matches."""
     import glob
-    key = {"workload": str(workload), "precision": precision, "frames_per_tile": str(tiles[0]),
-           "blocks_per_cu": str(tiles[1])}
+    key = {"kernel": KERNEL_REVISION, "workload": str(workload), "precision": precision,
+           "frames_per_tile": str(tiles[0]), "blocks_per_cu": str(tiles[1])}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             if not line.startswith("traffic:"):

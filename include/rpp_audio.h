@@ -242,10 +242,11 @@ RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
                                 Rpp32s *srcLengthTensor, const RpptLogMelParams *params, int perClip,
                                 rppHandle_t rppHandle);
 
-/* Tile shape of the fused launch; 0 fields mean "autotuned / default". */
+/* Launch shape of the fused kernel; 0 fields mean "default". */
 typedef struct {
-    Rpp32s framesPerTile;   /* frames per work item: 32, 64 or 128 */
-    Rpp32s blocksPerCU;     /* persistent blocks per compute unit */
+    Rpp32s framesPerTile;   /* reported only: frames per work item of the kernel the launch uses (FP32 32;
+                               MFMA 64 for nfft <= 512, 32 above) */
+    Rpp32s blocksPerCU;     /* persistent blocks per compute unit (default 1) */
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);
 /* Autotuner: times every supported tile shape on this batch (`iters` launches each, events on the

@@ -76,6 +76,7 @@ struct rppHandle {
     hipStream_t stream = nullptr;
     size_t batchSize = 1;
     int numCU = 256;
+    int maxLdsPerBlock = 64 * 1024;   // bytes of LDS one workgroup may allocate
     int numXCD = 8;
 
     // grow-only device workspace

@@ -44,7 +44,10 @@ RPPA_EXPORT rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u
     if (!h) return rppStatusAllocFailed;
     if (hipGetDevice(&h->device) != hipSuccess) { delete h; return rppStatusInternalError; }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) h->numCU = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
+        h->numCU = prop.multiProcessorCount;
+        h->maxLdsPerBlock = static_cast<int>(prop.sharedMemPerBlock);
+    }
     h->stream = static_cast<hipStream_t>(stream);
     h->batchSize = nBatchSize ? nBatchSize : 1;
     *handle = h;

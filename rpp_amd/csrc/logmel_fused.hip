@@ -71,6 +71,21 @@ struct LogMelArgs {
 
 __host__ __device__ __forceinline__ int seg_pos(int i, int H) { return i + (i / H) * kSkew; }
 
+// The value of v, hidden from the optimiser: thread-index arithmetic recomputed from it inside the
+// tile loop stays there instead of being hoisted into registers that live across the whole kernel.
+__device__ __forceinline__ int opaque(int v)
+{
+    __asm__ volatile("" : "+v"(v));
+    return v;
+}
+
+// seg_pos for 0 <= i < 2^20 without an integer division: with i = qH + r, (i + 0.5)/H lies at least
+// 0.5/H away from an integer, far beyond the FP32 rounding of the product.
+__device__ __forceinline__ int seg_pos_r(int i, float invH)
+{
+    return i + static_cast<int>((static_cast<float>(i) + 0.5f) * invH) * kSkew;
+}
+
 // Where clip b lives: samples, length, frame count, output base and output row stride.
 struct ClipView {
     const float *x;
@@ -380,181 +395,400 @@ __device__ __forceinline__ void mfma3(f32x16 &acc, const u16x8 &ah, const u16x8 
     }
 }
 
-// FT frames per tile; CTH 32-bin tiles per wave (each a cos tile and a sin tile), so the four
-// waves cover 4*CTH*32 = N/2 bins.  Folded DFT (see DftTables): contraction length W/2.
-template <int PREC, int FT, int CTH>
-__global__ void __launch_bounds__(kThreads, ((FT == 64 || CTH >= 4) ? 1 : 2))
-logmel_mfma_kernel(LogMelArgs a)
+// Largest b with tileStart[b] <= tile, by a 64-ary search over the wave (two rounds of loads for a
+// few hundred clips).  tileStart is non-decreasing, so the lanes whose candidate passes form a
+// prefix.  Call with the whole wave active.
+__device__ __forceinline__ int owner_clip_wave(const int *tileStart, int B, int tile, int lane)
 {
-    constexpr int RT = FT / 32;         // 32-frame row tiles
-    static_assert(RT * 2 * CTH <= 8, "accumulator budget");
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) / 64;
+        const int cand = lo + lane * step;
+        const bool ok = cand < hi && tileStart[cand] <= tile;
+        const int cnt = __popcll(__ballot(ok));
+        lo = lo + (cnt - 1) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
+// Prefetch of one tile's raw samples for the fast staging path: up to kPF float4 groups per thread
+// (the segment is [s0, s0 + span) with s0 - 1 >= 0 and s0 + span <= L, x + s0 16-byte aligned), plus
+// x[s0 - 1] for the first pre-emphasis product.  Group g goes to thread g % blockDim.
+constexpr int kPF = 6;
+struct SegPrefetch {
+    float4 v[kPF];
+    float first;
+};
+
+__device__ __forceinline__ bool fast_segment(const LogMelArgs &a, const ClipView &v, int f0, int FT, int span)
+{
+    const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
+    return s0 >= 1 && s0 + span <= v.L && v.F - f0 >= FT &&
+           (reinterpret_cast<uintptr_t>(v.x + s0) & 15) == 0 && (span + 3) / 4 <= kPF * static_cast<int>(blockDim.x);
+}
+
+__device__ __forceinline__ void prefetch_segment(SegPrefetch &p, const LogMelArgs &a, const ClipView &v, int f0, int span,
+                                                 int tid)
+{
+    const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
+    const float *x = v.x + s0;
+    const int groups = (span + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+        const int g = tid + q * blockDim.x;
+        if (g < groups) {
+            // the last group may reach past the span; never read past the clip
+            const int i = 4 * g;
+            if (s0 + i + 4 <= v.L) p.v[q] = *reinterpret_cast<const float4 *>(x + i);
+            else p.v[q] = make_float4(x[i], s0 + i + 1 < v.L ? x[i + 1] : 0.0f, s0 + i + 2 < v.L ? x[i + 2] : 0.0f, 0.0f);
+        }
+    }
+    p.first = x[-1];
+}
+
+// Writes the prefetched samples, pre-emphasised, into the skewed segment; returns the thread's max
+// |y|.  The sample before each group is the previous lane's last one (the previous wave's last lane
+// through `last`, one float per wave and group round).  Contains a block barrier.
+__device__ __forceinline__ float store_segment(float *seg, float *last, const SegPrefetch &p, const LogMelArgs &a,
+                                               int span, int tid)
+{
+    const int groups = (span + 3) / 4;
+    const int lane = tid & 63;
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+        const int g = tid + q * blockDim.x;
+        if (lane == 63 && g < groups) last[g >> 6] = p.v[q].w;
+    }
+    __syncthreads();
+    float mx = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+        const int g = tid + q * blockDim.x;
+        float prev = __shfl_up(p.v[q].w, 1);
+        if (lane == 0) prev = g == 0 ? p.first : last[(g >> 6) - 1];
+        if (g < groups) {
+            const float4 x = p.v[q];
+            float4 y;
+            float t;
+            t = a.coeff * prev; y.x = x.x - t;
+            t = a.coeff * x.x;  y.y = x.y - t;
+            t = a.coeff * x.y;  y.z = x.z - t;
+            t = a.coeff * x.z;  y.w = x.w - t;
+            const int i = 4 * g;
+            if (i + 4 > span) {  // past the span: zero (the fp16 scale must not see them)
+                if (i + 2 > span) y.y = 0.0f;
+                if (i + 3 > span) y.z = 0.0f;
+                y.w = 0.0f;
+            }
+            *reinterpret_cast<float4 *>(&seg[seg_pos(i, a.H)]) = y;
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
+        }
+    }
+    return mx;
+}
+
+// Generic staging (clip edges: reflect/zero padding, border rule, partial tiles); thread's max |y|.
+__device__ __forceinline__ float stage_segment_slow(float *seg, const LogMelArgs &a, const ClipView &v, int f0, int span,
+                                                    int tid)
+{
+    const int off = a.center ? a.W / 2 : 0;
+    const int s0 = f0 * a.H - off;
+    const int nfr = max(0, v.F - f0);
+    const int valid = nfr > 0 ? min(span, (nfr - 1) * a.H + a.W) : 0;
+    float mx = 0.0f;
+    for (int i = tid; i < span; i += blockDim.x) {
+        const float y = i < valid ? pe_sample(v.x, v.L, s0 + i, a.coeff, a.border, a.reflect) : 0.0f;
+        seg[seg_pos(i, a.H)] = y;
+        mx = fmaxf(mx, fabsf(y));
+    }
+    return mx;
+}
+
+// Two-element hi+lo split, packed as the adjacent halves of an MFMA operand (element j in the low
+// 16 bits), with the same rounding as split8.
+template <int PREC>
+__device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi, uint32_t &lo)
+{
+    if (PREC == kPrecFP16x3) {
+        const float u0 = v0 * s, u1 = v1 * s;
+        const _Float16 h0 = static_cast<_Float16>(u0), h1 = static_cast<_Float16>(u1);
+        const _Float16 l0 = static_cast<_Float16>(u0 - static_cast<float>(h0));
+        const _Float16 l1 = static_cast<_Float16>(u1 - static_cast<float>(h1));
+        hi = __builtin_bit_cast(uint16_t, h0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
+        lo = __builtin_bit_cast(uint16_t, l0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, l1)) << 16);
+    } else {
+        const __bf16 h0 = static_cast<__bf16>(v0), h1 = static_cast<__bf16>(v1);
+        const __bf16 l0 = static_cast<__bf16>(v0 - static_cast<float>(h0));
+        const __bf16 l1 = static_cast<__bf16>(v1 - static_cast<float>(h1));
+        hi = __builtin_bit_cast(uint16_t, h0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
+        lo = __builtin_bit_cast(uint16_t, l0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, l1)) << 16);
+    }
+}
+
+// Folded, split DFT operand of one 16-row contraction step s, built into buffer `buf` of the slice
+// ring while the waves run the previous step's products: slot ((buf * 4 + arr) * 2 + g) * FT + f
+// holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7, for arr = 0 e-hi, 1 e-lo, 2 o-hi,
+// 3 o-lo -- the lane order of a 32x32x16 MFMA A fragment.  A work unit forms four rows of one
+// frame (one 8-byte half of a slot); `nyq` gathers the thread's share of bin N/2 (sin term only).
+template <int PREC, int FT>
+__device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const LogMelArgs &a, int s, int buf,
+                                          float segScale, float invH, float &nyq, int tid)
+{
+    const int half = a.W / 2;
+    uint2 *dst = reinterpret_cast<uint2 *>(ring);
+    for (int u = tid; u < 4 * FT; u += blockDim.x) {
+        const int f = u % FT, qd = u / FT;
+        const int j = 16 * s + 4 * qd;                 // rows j .. j+3 (j even)
+        float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
+        if (j < half) {
+            const int fb = f * (a.H + kSkew);           // seg_pos(f*H + t) = f*(H + skew) + seg_pos(t)
+            const float4 up = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half + j, invH)]);
+            const float4 dn = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half - 4 - j, invH)]);  // rows j+3..j
+            e0 = up.x + dn.w; o0 = up.x - dn.w;
+            e1 = up.y + dn.z; o1 = up.y - dn.z;
+            e2 = up.z + dn.y; o2 = up.z - dn.y;
+            e3 = up.w + dn.x; o3 = up.w - dn.x;
+            const float4 w = *reinterpret_cast<const float4 *>(&a.wHalf[j]);
+            nyq = fmaf(o0, w.x, nyq);                  // (-1)^row
+            nyq = fmaf(o1, -w.y, nyq);
+            nyq = fmaf(o2, w.z, nyq);
+            nyq = fmaf(o3, -w.w, nyq);
+        }
+        uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
+        split2<PREC>(e0, e1, segScale, eh0, el0);
+        split2<PREC>(e2, e3, segScale, eh1, el1);
+        split2<PREC>(o0, o1, segScale, oh0, ol0);
+        split2<PREC>(o2, o3, segScale, oh1, ol1);
+        const int slot = (buf * 8 + (qd >> 1)) * FT + f, hh = qd & 1;
+        dst[(slot + 0 * 2 * FT) * 2 + hh] = make_uint2(eh0, eh1);
+        dst[(slot + 1 * 2 * FT) * 2 + hh] = make_uint2(el0, el1);
+        dst[(slot + 2 * 2 * FT) * 2 + hh] = make_uint2(oh0, oh1);
+        dst[(slot + 3 * 2 * FT) * 2 + hh] = make_uint2(ol0, ol1);
+    }
+}
+
+// bf16 hi+lo split of one power value into the two planes the mel product reads (same rounding
+// as split_bf16).
+__device__ __forceinline__ void put_power(uint16_t *pwH, uint16_t *pwL, int idx, float p)
+{
+    const __bf16 h = static_cast<__bf16>(p);
+    const __bf16 l = static_cast<__bf16>(p - static_cast<float>(h));
+    pwH[idx] = __builtin_bit_cast(uint16_t, h);
+    pwL[idx] = __builtin_bit_cast(uint16_t, l);
+}
+
+// power = cos^2 + sin^2 of the lane's accumulators (|X| when SQRT), split into the planes.
+template <bool SQRT, int RT, int CT>
+__device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&accC)[RT][CT],
+                                            const f32x16 (&accS)[RT][CT], float unscale, int bin0, int lane)
+{
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            const int base = (rt * 32 + 4 * (lane >> 5)) * pws2 + bin0 + ct * 32;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
+                float p = re * re + im * im;
+                if (SQRT) p = sqrtf(p);
+                put_power(pwH, pwL, base + (8 * (r >> 2) + (r & 3)) * pws2, p);
+            }
+        }
+    }
+}
+
+// Fused MFMA kernel.  Block = NW waves (NW = blockDim.x / 64); wave w owns CT 32-bin column tiles
+// (bins w*CT*32 .. +CT*32, each a cos tile and a sin tile of the folded basis), so NW*CT*32 = N/2;
+// FT = 32*RT frames per tile.  Per tile: stage the segment (prefetched one tile ahead on the fast
+// path); then per 16-row step of the folded contraction, the products of step s run from one
+// buffer of the operand ring while the block builds step s+1 into the other (one barrier per
+// step), with the basis fragments double-buffered in registers; power in lane, banded mel on
+// MFMA, dB out.
+template <int PREC, int FT, int CT>
+__global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segFloats)
+{
+    constexpr int RT = FT / 32;
+    static_assert(RT * CT == 2, "accumulator budget: four 32x32 tiles (cos and sin) per wave");
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    __shared__ float red[4];
-    float *seg = lds;
-    float *pw = lds;                    // aliases the segment once the DFT is done
-    const int pws = a.KP + kSkew;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ float red[16];
+    __shared__ float nyqPart[512];
+    __shared__ float lastOf[kPF * 8];
+    float *seg = lds;                                    // the power planes alias segment + ring after the DFT
+    u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2][FT] operand slices
+    const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
+    const int nw = blockDim.x >> 6;
     const int T = a.tileStart[a.B];
     const int half = a.W / 2;
     const int span = (FT - 1) * a.H + a.W;
     const int halfN = a.N / 2;
-    const int binBase = wave * (CTH * 32) + (lane & 31);     // cos column == bin; sin column = halfN + bin
-    const int jLane = 8 * (lane >> 5);
-    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
-        const int b = owner_clip(a.tileStart, a.B, tile);
+    const int nSteps = a.Wp / 16;
+    const float invH = 1.0f / static_cast<float>(a.H);
+
+    SegPrefetch pf;
+    int tile = blockIdx.x;
+    bool fastCur = false;
+    if (tile < T) {
+        const int tid = threadIdx.x, lane = tid & 63;
+        const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
+        const ClipView v = clip_view(a, b);
+        const int f0 = (tile - a.tileStart[b]) * FT;
+        fastCur = fast_segment(a, v, f0, FT, span);
+        if (fastCur) prefetch_segment(pf, a, v, f0, span, tid);
+    }
+    for (; tile < T; tile += gridDim.x) {
+        const int tid = opaque(threadIdx.x), wave = tid >> 6, lane = tid & 63;
+        const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
+        const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
         const int f0 = (tile - a.tileStart[b]) * FT;
         const ClipView v = clip_view(a, b);
 
-        float segScale = 1.0f;
-        if (PREC == kPrecFP16x3) {
-            // e = up + down can reach twice the largest sample: keep 2*max*scale below 2^11
-            float mx = stage_segment<true>(seg, a, v.x, v.L, f0, v.F, span, red);
-            if (mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));
-        } else {
-            stage_segment<false>(seg, a, v.x, v.L, f0, v.F, span, nullptr);
-        }
+        // ---- stage the pre-emphasised segment; block max for the fp16 scale
+        float mx = fastCur ? store_segment(seg, lastOf, pf, a, span, tid) : stage_segment_slow(seg, a, v, f0, span, tid);
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if (lane == 0) red[wave] = mx;
         __syncthreads();
+        mx = red[0];
+        for (int w = 1; w < nw; w++) mx = fmaxf(mx, red[w]);
+        float segScale = 1.0f;
+        if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));  // 2*max*scale < 2^11
 
-        // ---- folded DFT on MFMA 32x32x16 (rolling one-step prefetch of the basis fragments)
-        f32x16 accC[RT][CTH], accS[RT][CTH];
+        // ---- next tile's samples are fetched while this one computes
+        {
+            const int nt = tile + gridDim.x;
+            fastCur = false;
+            if (nt < T) {
+                const int nb = owner_clip_wave(a.tileStart, a.B, nt, lane);
+                const ClipView nv = clip_view(a, nb);
+                const int nf0 = (nt - a.tileStart[nb]) * FT;
+                fastCur = fast_segment(a, nv, nf0, FT, span);
+                if (fastCur) prefetch_segment(pf, a, nv, nf0, span, tid);
+            }
+        }
+
+        // ---- folded DFT, chunk by chunk
+        f32x16 accC[RT][CT], accS[RT][CT];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < CTH; ct++)
+            for (int ct = 0; ct < CT; ct++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) { accC[rt][ct][r] = 0.0f; accS[rt][ct][r] = 0.0f; }
-
-        u16x8 bcH[CTH], bcL[CTH], bsH[CTH], bsL[CTH];
-        auto loadB = [&](int k0) {
-            const size_t row = static_cast<size_t>((k0 >> 3) + (lane >> 5)) * a.N;
-#pragma unroll
-            for (int ct = 0; ct < CTH; ct++) {
-                const size_t oc = (row + binBase + ct * 32) * 8, os = oc + static_cast<size_t>(halfN) * 8;
-                bcH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
-                bsH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
-                if (PREC != kPrecBF16) {
-                    bcL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
-                    bsL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
-                }
+        // basis fragments [buffer][col tile][hi, lo]: one column tile per wave double-buffers whole
+        // steps; two refill each tile's fragments right after its own products (the other tile's
+        // products cover the latency), which keeps the register budget
+        constexpr int NB = CT == 1 ? 2 : 1;
+        u16x8 bc[NB][CT][2], bs[NB][CT][2];
+        auto loadB = [&](int buf, int ct, int s) {
+            const size_t oc = (static_cast<size_t>(2 * s + (lane >> 5)) * a.N + bin0 + ct * 32) * 8;
+            const size_t os = oc + static_cast<size_t>(halfN) * 8;
+            bc[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
+            bs[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
+            if (PREC != kPrecBF16) {
+                bc[buf][ct][1] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
+                bs[buf][ct][1] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
             }
         };
-        loadB(0);
-        for (int k0 = 0; k0 < a.Wp; k0 += 16) {
-            const int j0 = k0 + jLane;
-            u16x8 eH[RT], eL[RT], oH[RT], oL[RT];
+        float nyq = 0.0f;
+        prep_step<PREC, FT>(ring, seg, a, 0, 0, segScale, invH, nyq, tid);
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                float e[8], o[8];
-                if (j0 < half) {
-                    const int fb = (rt * 32 + (lane & 31)) * a.H;
-                    const int pu = seg_pos(fb + half + j0, a.H), pd = seg_pos(fb + half - 8 - j0, a.H);
-                    const float4 u0 = *reinterpret_cast<const float4 *>(&seg[pu]);
-                    const float4 u1 = *reinterpret_cast<const float4 *>(&seg[pu + 4]);
-                    const float4 d0 = *reinterpret_cast<const float4 *>(&seg[pd]);
-                    const float4 d1 = *reinterpret_cast<const float4 *>(&seg[pd + 4]);
-                    const float up[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-                    const float dn[8] = {d1.w, d1.z, d1.y, d1.x, d0.w, d0.z, d0.y, d0.x};  // reversed
+        for (int ct = 0; ct < CT; ct++) loadB(0, ct, 0);
+        __syncthreads();
+        for (int s0 = 0; s0 < nSteps; s0 += 2) {
 #pragma unroll
-                    for (int i = 0; i < 8; i++) { e[i] = up[i] + dn[i]; o[i] = up[i] - dn[i]; }
-                } else {
+            for (int u = 0; u < 2; u++) {
+                const int s = s0 + u;                   // s & 1 == u: operand buffer of step s
+                const int buf = NB == 2 ? u : 0;         // basis buffer holding step s
+                if (s < nSteps) {
+                    if (NB == 2 && s + 1 < nSteps) loadB(buf ^ 1, 0, s + 1);
+                    u16x8 ah[RT][2], oh[RT][2];
 #pragma unroll
-                    for (int i = 0; i < 8; i++) { e[i] = 0.0f; o[i] = 0.0f; }
-                }
-                split8<PREC>(e, segScale, eH[rt], eL[rt]);
-                split8<PREC>(o, segScale, oH[rt], oL[rt]);
-            }
-            const bool more = k0 + 16 < a.Wp;
-            const size_t nrow = static_cast<size_t>(((k0 + 16) >> 3) + (lane >> 5)) * a.N;
-#pragma unroll
-            for (int ct = 0; ct < CTH; ct++) {
-#pragma unroll
-                for (int rt = 0; rt < RT; rt++) {
-                    mfma3<PREC>(accC[rt][ct], eH[rt], eL[rt], bcH[ct], bcL[ct]);
-                    mfma3<PREC>(accS[rt][ct], oH[rt], oL[rt], bsH[ct], bsL[ct]);
-                }
-                if (more) {  // refill this tile's fragments for the next step while the others run
-                    const size_t oc = (nrow + binBase + ct * 32) * 8, os = oc + static_cast<size_t>(halfN) * 8;
-                    bcH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
-                    bsH[ct] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
-                    if (PREC != kPrecBF16) {
-                        bcL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
-                        bsL[ct] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
+                    for (int rt = 0; rt < RT; rt++) {
+                        const int slot = (u * 8 + (lane >> 5)) * FT + rt * 32 + (lane & 31);
+                        ah[rt][0] = ring[slot + 0 * 2 * FT];
+                        ah[rt][1] = ring[slot + 1 * 2 * FT];
+                        oh[rt][0] = ring[slot + 2 * 2 * FT];
+                        oh[rt][1] = ring[slot + 3 * 2 * FT];
                     }
+                    // keep the LDS reads together ahead of the products (left alone, the scheduler
+                    // serialises each read with its first product)
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) {
+#pragma unroll
+                        for (int rt = 0; rt < RT; rt++) {
+                            mfma3<PREC>(accC[rt][ct], ah[rt][0], ah[rt][1], bc[buf][ct][0], bc[buf][ct][1]);
+                            mfma3<PREC>(accS[rt][ct], oh[rt][0], oh[rt][1], bs[buf][ct][0], bs[buf][ct][1]);
+                        }
+                        if (NB == 1 && s + 1 < nSteps) loadB(0, ct, s + 1);
+                    }
+                    if (s + 1 < nSteps) prep_step<PREC, FT>(ring, seg, a, s + 1, u ^ 1, segScale, invH, nyq, tid);
+                    __syncthreads();                    // step s+1 built; buffer u free again
                 }
             }
         }
+        nyqPart[tid] = nyq;
+        __syncthreads();  // segment and operand are dead: pw may overwrite them
 
-        // ---- bin N/2: its cos term vanishes with the half-sample origin; sin term on the VALU
-        float nyq[FT / 4];
-#pragma unroll
-        for (int i = 0; i < FT / 4; i++) {
-            const int f = wave + 4 * i;
-            float sum = 0.0f;
-            for (int j = lane; j < half; j += 64) {
-                const int fb = f * a.H;
-                const float d = seg[seg_pos(fb + half + j, a.H)] - seg[seg_pos(fb + half - 1 - j, a.H)];
-                const float w = (j & 1) ? -a.wHalf[j] : a.wHalf[j];
-                sum = fmaf(d, w, sum);
-            }
-            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-            nyq[i] = sum;
-        }
-        __syncthreads();  // the segment is dead: pw may overwrite it
-
-        // ---- power = cos^2 + sin^2, in lane (cos and sin tiles hold the same bins)
+        // ---- power = cos^2 + sin^2 in lane, bin N/2 from the Nyquist partials, stored as bf16 hi and
+        //      lo planes [FT][pws2] for the mel product (each value split once)
+        uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
+        uint16_t *pwL = pwH + FT * pws2;
         const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
-#pragma unroll
-        for (int rt = 0; rt < RT; rt++) {
-#pragma unroll
-            for (int ct = 0; ct < CTH; ct++) {
-                const int bin = binBase + ct * 32;
-#pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
-                    float p = re * re + im * im;
-                    if (a.power == 1) p = sqrtf(p);
-                    const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    pw[row * pws + bin] = p;
-                }
-            }
+        if (a.power == 1) store_power<true, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, bin0, lane);
+        else store_power<false, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, bin0, lane);
+        if (tid < FT) {
+            float n = 0.0f;
+            for (int q = tid; q < static_cast<int>(blockDim.x); q += FT) n += nyqPart[q];
+            float p = n * n;
+            if (a.power == 1) p = sqrtf(p);
+            put_power(pwH, pwL, tid * pws2 + halfN, p);
         }
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < FT / 4; i++) {
-                float p = nyq[i] * nyq[i];
-                if (a.power == 1) p = sqrtf(p);
-                pw[(wave + 4 * i) * pws + halfN] = p;
-            }
-        }
-        // zero the padding bins [K, KP) that the mel contraction reads
-        for (int e = tid; e < FT * (a.KP - a.K); e += kThreads) {
+        for (int e = tid; e < FT * (a.KP - a.K); e += blockDim.x) {  // padding bins the mel product reads
             const int f = e / (a.KP - a.K), k = a.K + e % (a.KP - a.K);
-            pw[f * pws + k] = 0.0f;
+            pwH[f * pws2 + k] = 0;
+            pwL[f * pws2 + k] = 0;
         }
         __syncthreads();
 
-        // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16, split hi+lo
+        // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
+        //      16-filter tile's band of bins only
         const int nMT = a.Mpad / 16, nFT = FT / 16, nTiles = nMT * nFT;
-        for (int tt = wave; tt < nTiles; tt += 4) {
+        for (int tt = wave; tt < nTiles; tt += nw) {
             const int mt = tt % nMT, ft = tt / nMT;
+            int kb = a.KP, ke = 0;
+            {
+                const int m = mt * 16 + (lane & 15);
+                if (lane < 16 && m < a.M) { kb = a.filterBins[2 * m]; ke = a.filterBins[2 * m + 1]; }
+                for (int o = 8; o > 0; o >>= 1) {
+                    kb = min(kb, __shfl_xor(kb, o));
+                    ke = max(ke, __shfl_xor(ke, o));
+                }
+                kb = __shfl(kb, 0);
+                ke = __shfl(ke, 0);
+                kb = kb / 32 * 32;
+                ke = min(a.KP, (ke + 31) / 32 * 32);
+            }
             f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
             const int fr = ft * 16 + (lane & 15);
-            size_t o = (static_cast<size_t>(lane >> 4) * a.Mpad + mt * 16 + (lane & 15)) * 8;
             const size_t ostep = static_cast<size_t>(4) * a.Mpad * 8;  // 32 bins further
-            u16x8 wh = *reinterpret_cast<const u16x8 *>(a.mHi + o), wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
-            for (int k0 = 0; k0 < a.KP; k0 += 32) {
+            size_t o = (static_cast<size_t>(kb / 8 + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
+            u16x8 wh = {}, wl = {};
+            if (kb < ke) {
+                wh = *reinterpret_cast<const u16x8 *>(a.mHi + o);
+                wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
+            }
+            for (int k0 = kb; k0 < ke; k0 += 32) {
                 const u16x8 ch = wh, cl = wl;
-                if (k0 + 32 < a.KP) {
+                if (k0 + 32 < ke) {
                     o += ostep;
                     wh = *reinterpret_cast<const u16x8 *>(a.mHi + o);
                     wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
                 }
-                const int q = fr * pws + k0 + 8 * (lane >> 4);
-                const float4 v0 = *reinterpret_cast<const float4 *>(&pw[q]);
-                const float4 v1 = *reinterpret_cast<const float4 *>(&pw[q + 4]);
-                const float pv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-                bf16x8 ph, pl;
-                split_bf16(pv, ph, pl);
+                const int q = fr * pws2 + k0 + 8 * (lane >> 4);
+                const bf16x8 ph = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q]));
+                const bf16x8 pl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q]));
                 const bf16x8 w0 = __builtin_bit_cast(bf16x8, ch), w1 = __builtin_bit_cast(bf16x8, cl);
                 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
                 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
@@ -653,12 +887,11 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    // tile shape: frames per work item and persistent blocks per CU
-    int FT;
-    if (!mfma) FT = 32;
-    else if (nfft > 512) FT = 32;
-    else FT = (h->tiles.framesPerTile == 64) ? 64 : 32;
-    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 2;
+    // tile shape: FP32 kernel 32 frames per tile; MFMA kernel CT 32-bin column tiles per wave
+    // (2 for nfft >= 768, else 1), N/2/(32*CT) waves, 32*(2/CT) frames per tile
+    const int CT = nfft >= 768 ? 2 : 1;
+    const int FT = mfma ? 64 / CT : kFT32;
+    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 1;
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
     hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.hStride,
@@ -672,26 +905,31 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int passes = (a.cols + kCC - 1) / kCC;
         const int alias = passes == 1 ? 1 : 0;
         const size_t bytes = sizeof(float) * (segFloats + (alias ? std::max(bFloats, pwFloats) : bFloats + pwFloats));
-        if (bytes > 160 * 1024) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
+        if (bytes > static_cast<size_t>(h->maxLdsPerBlock) - kFT32 * kKC * sizeof(float))
+            return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
         hipLaunchKernelGGL(logmel_fp32_kernel, dim3(grid), dim3(kThreads), bytes, h->stream, a, segFloats, bFloats,
                            pwFloats, alias);
     } else {
+        const int nw = nfft / 2 / (32 * CT);
         const int span = (FT - 1) * a.H + a.W;
-        const int segFloats = seg_pos(span - 1, a.H) + 1 + 8;
-        const int pwFloats = FT * pws;
-        const size_t bytes = sizeof(float) * std::max(segFloats, pwFloats);
-        if (bytes > 160 * 1024) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
-        const int cth = nfft / 256;  // 32-bin tiles per wave: N/2 bins over 4 waves
-#define RPPA_LAUNCH(PR)                                                                                     \
-        do {                                                                                                \
-            if (FT == 64 && cth == 2)                                                                        \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 2>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
-            else if (cth == 2)                                                                               \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 2>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
-            else if (cth == 4)                                                                               \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 4>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
-            else                                                                                             \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 1>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+        const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
+        const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
+        const size_t staticBytes = sizeof(float) * (16 + 512 + kPF * 8);
+        const size_t bytes = std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes);
+        if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
+#define RPPA_LAUNCH(PR)                                                                                         \
+        do {                                                                                                    \
+            if (CT == 1) {                                                                                      \
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, 64, 1>),       \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)); \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 1>), dim3(grid), dim3(64 * nw), bytes, h->stream, \
+                                   a, segFloats);                                                           \
+            } else {                                                                                            \
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, 32, 2>),       \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)); \
+                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 2>), dim3(grid), dim3(64 * nw), bytes, h->stream, \
+                                   a, segFloats);                                                           \
+            }                                                                                                   \
         } while (0)
         if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
         else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);
@@ -715,7 +953,7 @@ RppStatus check_logmel_params(const RpptLogMelParams *p, RpptLogMelPrecision pre
 RPPA_EXPORT RppStatus rppt_logmel_set_tiles(rppHandle_t h, const RpptLogMelTiles *t)
 {
     if (!h || !t) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (t->framesPerTile != 0 && t->framesPerTile != 32 && t->framesPerTile != 64) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (t->framesPerTile < 0) return RPP_ERROR_INVALID_ARGUMENTS;
     if (t->blocksPerCU < 0 || t->blocksPerCU > 8) return RPP_ERROR_INVALID_ARGUMENTS;
     h->tiles = *t;
     return RPP_SUCCESS;
@@ -758,11 +996,8 @@ RPPA_EXPORT RppStatus rppt_logmel_autotune(RppPtr_t srcPtr, RpptDescPtr srcDescP
     if (!h || !p || iters <= 0) return RPP_ERROR_INVALID_ARGUMENTS;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     std::vector<RpptLogMelTiles> cands;
-    const int fts[2] = {32, 64};
-    for (int ft : fts) {
-        if (ft == 64 && (precision == RPPT_LOGMEL_FP32 || nfft > 512)) continue;
-        for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
-    }
+    const int ft = precision == RPPT_LOGMEL_FP32 ? kFT32 : (nfft >= 768 ? 32 : 64);
+    for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
     hipEvent_t e0, e1;
     RPPA_HIP(hipEventCreate(&e0));
     RPPA_HIP(hipEventCreate(&e1));
