@@ -412,11 +412,11 @@ __device__ __forceinline__ int owner_clip_wave(const int *tileStart, int B, int 
     return lo;
 }
 
-// Prefetch of one tile's raw samples for the fast staging path: up to kPF float4 groups per thread
-// (the segment is [s0, s0 + span) with s0 - 1 >= 0 and s0 + span <= L, x + s0 16-byte aligned), plus
-// x[s0 - 1] for the first pre-emphasis product.  Group g goes to thread g % blockDim.
-constexpr int kPF = 6;
-struct SegPrefetch {
+// Fast staging (tiles whose segment [s0, s0 + span) lies inside the clip with s0 >= 1 and starts
+// 16-byte aligned): every thread issues all its float4 loads first, up to kPF groups (group g goes
+// to thread g % blockDim), plus x[s0 - 1] for the first pre-emphasis product.
+constexpr int kPF = 12;
+struct SegLoad {
     float4 v[kPF];
     float first;
 };
@@ -428,8 +428,8 @@ __device__ __forceinline__ bool fast_segment(const LogMelArgs &a, const ClipView
            (reinterpret_cast<uintptr_t>(v.x + s0) & 15) == 0 && (span + 3) / 4 <= kPF * static_cast<int>(blockDim.x);
 }
 
-__device__ __forceinline__ void prefetch_segment(SegPrefetch &p, const LogMelArgs &a, const ClipView &v, int f0, int span,
-                                                 int tid)
+__device__ __forceinline__ void load_segment(SegLoad &p, const LogMelArgs &a, const ClipView &v, int f0, int span,
+                                             int tid)
 {
     const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
     const float *x = v.x + s0;
@@ -447,11 +447,11 @@ __device__ __forceinline__ void prefetch_segment(SegPrefetch &p, const LogMelArg
     p.first = x[-1];
 }
 
-// Writes the prefetched samples, pre-emphasised, into the skewed segment; returns the thread's max
-// |y|.  The sample before each group is the previous lane's last one (the previous wave's last lane
+// Writes the loaded samples, pre-emphasised, into the skewed segment; returns the thread's max |y|.
+// The sample before each group is the previous lane's last one (the previous wave's last lane
 // through `last`, one float per wave and group round).  Contains a block barrier.
-__device__ __forceinline__ float store_segment(float *seg, float *last, const SegPrefetch &p, const LogMelArgs &a,
-                                               int span, int tid)
+__device__ __forceinline__ float store_segment(float *seg, float *last, const SegLoad &p, const LogMelArgs &a,
+                                               int span, float invH, int tid)
 {
     const int groups = (span + 3) / 4;
     const int lane = tid & 63;
@@ -481,26 +481,60 @@ __device__ __forceinline__ float store_segment(float *seg, float *last, const Se
                 if (i + 3 > span) y.z = 0.0f;
                 y.w = 0.0f;
             }
-            *reinterpret_cast<float4 *>(&seg[seg_pos(i, a.H)]) = y;
+            *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
             mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
         }
     }
     return mx;
 }
 
+// pe_sample without branches around the loads (index and border rule resolved by selects), so a
+// thread can have several samples' loads in flight.  Same arithmetic as pe_sample.
+__device__ __forceinline__ float pe_sample_sel(const float *x, int L, int j, float c, int border, int reflect)
+{
+    const bool outside = j < 0 || j >= L;
+    int jj = outside && reflect ? idx_reflect(j, L) : j;
+    jj = min(max(jj, 0), L - 1);
+    int pi = jj - 1;
+    bool pz = false;
+    if (pi < 0) {
+        pi = (border == CLAMP) ? 0 : 1;
+        pz = !(border == CLAMP || (border == REFLECT && L > 1));
+    }
+    pi = min(pi, L - 1);
+    const float xv = x[jj];
+    const float pv = pz ? 0.0f : x[pi];
+    const float p = c * pv;
+    const float y = xv - p;
+    return (outside && !reflect) ? 0.0f : y;
+}
+
 // Generic staging (clip edges: reflect/zero padding, border rule, partial tiles); thread's max |y|.
+// Four samples per thread per round, their loads issued together.
 __device__ __forceinline__ float stage_segment_slow(float *seg, const LogMelArgs &a, const ClipView &v, int f0, int span,
-                                                    int tid)
+                                                    float invH, int tid)
 {
     const int off = a.center ? a.W / 2 : 0;
     const int s0 = f0 * a.H - off;
     const int nfr = max(0, v.F - f0);
-    const int valid = nfr > 0 ? min(span, (nfr - 1) * a.H + a.W) : 0;
+    const int valid = (nfr > 0 && v.L > 0) ? min(span, (nfr - 1) * a.H + a.W) : 0;
+    const int nt = blockDim.x;
     float mx = 0.0f;
-    for (int i = tid; i < span; i += blockDim.x) {
-        const float y = i < valid ? pe_sample(v.x, v.L, s0 + i, a.coeff, a.border, a.reflect) : 0.0f;
-        seg[seg_pos(i, a.H)] = y;
-        mx = fmaxf(mx, fabsf(y));
+    for (int i0 = tid; i0 < span; i0 += 4 * nt) {
+        float y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + k * nt;
+            y[k] = i < valid ? pe_sample_sel(v.x, v.L, s0 + i, a.coeff, a.border, a.reflect) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + k * nt;
+            if (i < span) {
+                seg[seg_pos_r(i, invH)] = y[k];
+                mx = fmaxf(mx, fabsf(y[k]));
+            }
+        }
     }
     return mx;
 }
@@ -599,45 +633,61 @@ __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pw
     }
 }
 
-// Fused MFMA kernel.  Block = NW waves (NW = blockDim.x / 64); wave w owns CT 32-bin column tiles
-// (bins w*CT*32 .. +CT*32, each a cos tile and a sin tile of the folded basis), so NW*CT*32 = N/2;
-// FT = 32*RT frames per tile.  Per tile: stage the segment (prefetched one tile ahead on the fast
-// path); then per 16-row step of the folded contraction, the products of step s run from one
-// buffer of the operand ring while the block builds step s+1 into the other (one barrier per
-// step), with the basis fragments double-buffered in registers; power in lane, banded mel on
-// MFMA, dB out.
+// Mel work list of one wave: entries (16-filter x 16-frame tile, 32-bin step of its band), packed
+// as tile << 8 | step; every tile has at least one step.  Capacity per wave:
+constexpr int kMelSeq = 320;
+
+// Fused MFMA kernel.  Block = 4 waves; wave w owns CT 32-bin column tiles (bins w*CT*32 .. +CT*32,
+// each a cos and a sin tile of the folded basis), so 4*CT*32 = N/2, for all FT = 32*RT frames of a
+// tile.  Two blocks share a CU, so one block's staging and power/mel phases overlap the other's
+// products.  Per tile: stage the pre-emphasised segment; per 16-row step of the folded
+// contraction, the products of step s run from one buffer of the operand ring while the block
+// builds step s+1 into the other (one barrier per step); power in lane into bf16 hi/lo planes;
+// banded mel on MFMA with the weights loaded two steps ahead; dB out.
 template <int PREC, int FT, int CT>
-__global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segFloats)
+__global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats)
 {
     constexpr int RT = FT / 32;
-    static_assert(RT * CT == 2, "accumulator budget: four 32x32 tiles (cos and sin) per wave");
+    constexpr int NW = 4;
+    static_assert(RT * CT <= 4, "accumulator budget: eight 32x32 tiles per wave");
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    __shared__ float red[16];
-    __shared__ float nyqPart[512];
-    __shared__ float lastOf[kPF * 8];
+    __shared__ float red[NW];
+    __shared__ float nyqPart[NW * 64];
+    __shared__ float lastOf[kPF * NW];
+    __shared__ uint16_t melSeq[NW][kMelSeq];
+    __shared__ int melSeqLen[NW];
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
     u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2][FT] operand slices
     const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
-    const int nw = blockDim.x >> 6;
     const int T = a.tileStart[a.B];
-    const int half = a.W / 2;
     const int span = (FT - 1) * a.H + a.W;
     const int halfN = a.N / 2;
     const int nSteps = a.Wp / 16;
     const float invH = 1.0f / static_cast<float>(a.H);
+    const int nMT = a.Mpad / 16, nMelTiles = nMT * (FT / 16);
 
-    SegPrefetch pf;
-    int tile = blockIdx.x;
-    bool fastCur = false;
-    if (tile < T) {
-        const int tid = threadIdx.x, lane = tid & 63;
-        const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
-        const ClipView v = clip_view(a, b);
-        const int f0 = (tile - a.tileStart[b]) * FT;
-        fastCur = fast_segment(a, v, f0, FT, span);
-        if (fastCur) prefetch_segment(pf, a, v, f0, span, tid);
+    // each wave's mel work list (the same for every tile): bands of the filter tiles it owns
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        int n = 0;
+        for (int tt = w; tt < nMelTiles; tt += NW) {
+            const int mt = tt % nMT;
+            int kb = a.KP, ke = 0;
+            for (int i = 0; i < 16; i++) {
+                const int m = mt * 16 + i;
+                if (m < a.M) {
+                    kb = min(kb, a.filterBins[2 * m]);
+                    ke = max(ke, a.filterBins[2 * m + 1]);
+                }
+            }
+            kb = kb >= ke ? 0 : kb / 32 * 32;        // an empty band still takes one (all-zero) step
+            ke = max(min(a.KP, (ke + 31) / 32 * 32), kb + 32);
+            for (int k = kb; k < ke && n < kMelSeq; k += 32) melSeq[w][n++] = static_cast<uint16_t>((tt << 8) | (k / 32));
+        }
+        melSeqLen[w] = n;
     }
-    for (; tile < T; tile += gridDim.x) {
+
+    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
         const int tid = opaque(threadIdx.x), wave = tid >> 6, lane = tid & 63;
         const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
         const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
@@ -645,29 +695,22 @@ __global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segF
         const ClipView v = clip_view(a, b);
 
         // ---- stage the pre-emphasised segment; block max for the fp16 scale
-        float mx = fastCur ? store_segment(seg, lastOf, pf, a, span, tid) : stage_segment_slow(seg, a, v, f0, span, tid);
+        float mx;
+        if (fast_segment(a, v, f0, FT, span)) {
+            SegLoad ld;
+            load_segment(ld, a, v, f0, span, tid);
+            mx = store_segment(seg, lastOf, ld, a, span, invH, tid);
+        } else {
+            mx = stage_segment_slow(seg, a, v, f0, span, invH, tid);
+        }
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
         if (lane == 0) red[wave] = mx;
         __syncthreads();
-        mx = red[0];
-        for (int w = 1; w < nw; w++) mx = fmaxf(mx, red[w]);
+        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         float segScale = 1.0f;
         if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));  // 2*max*scale < 2^11
 
-        // ---- next tile's samples are fetched while this one computes
-        {
-            const int nt = tile + gridDim.x;
-            fastCur = false;
-            if (nt < T) {
-                const int nb = owner_clip_wave(a.tileStart, a.B, nt, lane);
-                const ClipView nv = clip_view(a, nb);
-                const int nf0 = (nt - a.tileStart[nb]) * FT;
-                fastCur = fast_segment(a, nv, nf0, FT, span);
-                if (fastCur) prefetch_segment(pf, a, nv, nf0, span, tid);
-            }
-        }
-
-        // ---- folded DFT, chunk by chunk
+        // ---- folded DFT
         f32x16 accC[RT][CT], accS[RT][CT];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
@@ -676,7 +719,7 @@ __global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segF
 #pragma unroll
                 for (int r = 0; r < 16; r++) { accC[rt][ct][r] = 0.0f; accS[rt][ct][r] = 0.0f; }
         // basis fragments [buffer][col tile][hi, lo]: one column tile per wave double-buffers whole
-        // steps; two refill each tile's fragments right after its own products (the other tile's
+        // steps; more refill each tile's fragments right after its own products (the other tiles'
         // products cover the latency), which keeps the register budget
         constexpr int NB = CT == 1 ? 2 : 1;
         u16x8 bc[NB][CT][2], bs[NB][CT][2];
@@ -729,7 +772,7 @@ __global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segF
             }
         }
         nyqPart[tid] = nyq;
-        __syncthreads();  // segment and operand are dead: pw may overwrite them
+        __syncthreads();  // segment and operand ring are dead: the power planes may overwrite them
 
         // ---- power = cos^2 + sin^2 in lane, bin N/2 from the Nyquist partials, stored as bf16 hi and
         //      lo planes [FT][pws2] for the mel product (each value split once)
@@ -740,12 +783,12 @@ __global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segF
         else store_power<false, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, bin0, lane);
         if (tid < FT) {
             float n = 0.0f;
-            for (int q = tid; q < static_cast<int>(blockDim.x); q += FT) n += nyqPart[q];
+            for (int q = tid; q < NW * 64; q += FT) n += nyqPart[q];
             float p = n * n;
             if (a.power == 1) p = sqrtf(p);
             put_power(pwH, pwL, tid * pws2 + halfN, p);
         }
-        for (int e = tid; e < FT * (a.KP - a.K); e += blockDim.x) {  // padding bins the mel product reads
+        for (int e = tid; e < FT * (a.KP - a.K); e += NW * 64) {  // padding bins the mel product reads
             const int f = e / (a.KP - a.K), k = a.K + e % (a.KP - a.K);
             pwH[f * pws2 + k] = 0;
             pwL[f * pws2 + k] = 0;
@@ -753,58 +796,53 @@ __global__ void __launch_bounds__(512) logmel_mfma_kernel(LogMelArgs a, int segF
         __syncthreads();
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
-        //      16-filter tile's band of bins only
-        const int nMT = a.Mpad / 16, nFT = FT / 16, nTiles = nMT * nFT;
-        for (int tt = wave; tt < nTiles; tt += nw) {
-            const int mt = tt % nMT, ft = tt / nMT;
-            int kb = a.KP, ke = 0;
-            {
-                const int m = mt * 16 + (lane & 15);
-                if (lane < 16 && m < a.M) { kb = a.filterBins[2 * m]; ke = a.filterBins[2 * m + 1]; }
-                for (int o = 8; o > 0; o >>= 1) {
-                    kb = min(kb, __shfl_xor(kb, o));
-                    ke = max(ke, __shfl_xor(ke, o));
-                }
-                kb = __shfl(kb, 0);
-                ke = __shfl(ke, 0);
-                kb = kb / 32 * 32;
-                ke = min(a.KP, (ke + 31) / 32 * 32);
-            }
+        //      16-filter tile's band of bins; the wave walks its work list with the weight
+        //      fragments loaded two entries ahead
+        {
+            const int n = melSeqLen[wave];
+            const uint16_t *seq = melSeq[wave];
+            u16x8 wh[2], wl[2];
+            auto loadW = [&](int slot, int e) {
+                const int ent = seq[e], tt = ent >> 8, k0 = (ent & 255) * 32;
+                const size_t o = (static_cast<size_t>(k0 / 8 + (lane >> 4)) * a.Mpad + (tt % nMT) * 16 + (lane & 15)) * 8;
+                wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
+                wl[slot] = *reinterpret_cast<const u16x8 *>(a.mLo + o);
+            };
+            if (n > 0) loadW(0, 0);
+            if (n > 1) loadW(1, 1);
             f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            const int fr = ft * 16 + (lane & 15);
-            const size_t ostep = static_cast<size_t>(4) * a.Mpad * 8;  // 32 bins further
-            size_t o = (static_cast<size_t>(kb / 8 + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
-            u16x8 wh = {}, wl = {};
-            if (kb < ke) {
-                wh = *reinterpret_cast<const u16x8 *>(a.mHi + o);
-                wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
-            }
-            for (int k0 = kb; k0 < ke; k0 += 32) {
-                const u16x8 ch = wh, cl = wl;
-                if (k0 + 32 < ke) {
-                    o += ostep;
-                    wh = *reinterpret_cast<const u16x8 *>(a.mHi + o);
-                    wl = *reinterpret_cast<const u16x8 *>(a.mLo + o);
-                }
-                const int q = fr * pws2 + k0 + 8 * (lane >> 4);
-                const bf16x8 ph = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q]));
-                const bf16x8 pl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q]));
-                const bf16x8 w0 = __builtin_bit_cast(bf16x8, ch), w1 = __builtin_bit_cast(bf16x8, cl);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph, c, 0, 0, 0);
-            }
-            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
-            if (f0 + fr < v.F) {
-                float *out = v.out + f0 + fr;
+            for (int e0 = 0; e0 < n; e0 += 2) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int m = mt * 16 + 4 * (lane >> 4) + r;
-                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c[r], a);
+                for (int u = 0; u < 2; u++) {
+                    const int e = e0 + u;
+                    if (e < n) {
+                        const int ent = seq[e], tt = ent >> 8, k0 = (ent & 255) * 32;
+                        const bf16x8 w0 = __builtin_bit_cast(bf16x8, wh[u]), w1 = __builtin_bit_cast(bf16x8, wl[u]);
+                        if (e + 2 < n) loadW(u, e + 2);
+                        const int mt = tt % nMT, fr = (tt / nMT) * 16 + (lane & 15);
+                        const int q = fr * pws2 + k0 + 8 * (lane >> 4);
+                        const bf16x8 ph = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q]));
+                        const bf16x8 pl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q]));
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph, c, 0, 0, 0);
+                        if (e + 1 == n || (seq[e + 1] >> 8) != tt) {
+                            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
+                            if (f0 + fr < v.F) {
+                                float *out = v.out + f0 + fr;
+#pragma unroll
+                                for (int r = 0; r < 4; r++) {
+                                    const int m = mt * 16 + 4 * (lane >> 4) + r;
+                                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c[r], a);
+                                }
+                            }
+                            c = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                        }
+                    }
                 }
             }
         }
-        __syncthreads();  // pw is reused by the next tile's segment
+        __syncthreads();  // the power planes are reused by the next tile's segment
     }
 }
 
@@ -819,10 +857,11 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
         return RPP_ERROR_INVALID_ARGUMENTS;
     if (!(p->referenceMagnitude > 0.0f)) return RPP_ERROR_NOT_IMPLEMENTED;  // per-clip max needs a second pass
     if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
-    // the folded MFMA path: even nfft <= 1024 with N/2 a multiple of 128, window length a multiple of
-    // 16 (its half a multiple of 8), hop a multiple of 8
+    // the folded MFMA path: nfft 256, 512, 768 or 1024, window length a multiple of 16 (its half a
+    // multiple of 8), hop a multiple of 8, at most 256 mel filters
     if (precision != RPPT_LOGMEL_FP32 &&
-        ((p->windowStep % 8) != 0 || (p->windowLength % 16) != 0 || nfft > 1024 || (nfft % 256) != 0))
+        ((p->windowStep % 8) != 0 || (p->windowLength % 16) != 0 || nfft > 1024 || (nfft % 256) != 0 ||
+         p->numFilter > 256))
         return RPP_ERROR_NOT_IMPLEMENTED;
     return RPP_SUCCESS;
 }
@@ -887,11 +926,11 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    // tile shape: FP32 kernel 32 frames per tile; MFMA kernel CT 32-bin column tiles per wave
-    // (2 for nfft >= 768, else 1), N/2/(32*CT) waves, 32*(2/CT) frames per tile
-    const int CT = nfft >= 768 ? 2 : 1;
-    const int FT = mfma ? 64 / CT : kFT32;
-    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : 1;
+    // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
+    // tiles each, 64 frames per tile up to nfft 512 and 32 above
+    const int CT = nfft / 256;
+    const int FT = mfma ? (CT <= 2 ? 64 : 32) : kFT32;
+    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
     hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.hStride,
@@ -910,31 +949,30 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         hipLaunchKernelGGL(logmel_fp32_kernel, dim3(grid), dim3(kThreads), bytes, h->stream, a, segFloats, bFloats,
                            pwFloats, alias);
     } else {
-        const int nw = nfft / 2 / (32 * CT);
         const int span = (FT - 1) * a.H + a.W;
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
-        const size_t staticBytes = sizeof(float) * (16 + 512 + kPF * 8);
+        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + kPF * 4 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
         const size_t bytes = std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes);
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
+#define RPPA_LAUNCH_ONE(PR, F, C)                                                                               \
+        do {                                                                                                    \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, F, C>),            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));     \
+            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats); \
+        } while (0)
 #define RPPA_LAUNCH(PR)                                                                                         \
         do {                                                                                                    \
-            if (CT == 1) {                                                                                      \
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, 64, 1>),       \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)); \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 64, 1>), dim3(grid), dim3(64 * nw), bytes, h->stream, \
-                                   a, segFloats);                                                           \
-            } else {                                                                                            \
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, 32, 2>),       \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)); \
-                hipLaunchKernelGGL((logmel_mfma_kernel<PR, 32, 2>), dim3(grid), dim3(64 * nw), bytes, h->stream, \
-                                   a, segFloats);                                                           \
-            }                                                                                                   \
+            if (CT == 1) RPPA_LAUNCH_ONE(PR, 64, 1);                                                            \
+            else if (CT == 2) RPPA_LAUNCH_ONE(PR, 64, 2);                                                       \
+            else if (CT == 3) RPPA_LAUNCH_ONE(PR, 32, 3);                                                       \
+            else RPPA_LAUNCH_ONE(PR, 32, 4);                                                                    \
         } while (0)
         if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
         else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);
         else RPPA_LAUNCH(kPrecBF16);
 #undef RPPA_LAUNCH
+#undef RPPA_LAUNCH_ONE
     }
     RPPA_HIP(hipGetLastError());
     return RPP_SUCCESS;
