@@ -488,25 +488,29 @@ __device__ __forceinline__ float store_segment(float *seg, float *last, const Se
     return mx;
 }
 
-// pe_sample without branches around the loads (index and border rule resolved by selects), so a
-// thread can have several samples' loads in flight.  Same arithmetic as pe_sample.
+// pe_sample with the index and border rule resolved before the loads (the reflect map only for
+// samples outside the clip), so a thread can have several samples' loads in flight.  Same
+// arithmetic as pe_sample.
 __device__ __forceinline__ float pe_sample_sel(const float *x, int L, int j, float c, int border, int reflect)
 {
-    const bool outside = j < 0 || j >= L;
-    int jj = outside && reflect ? idx_reflect(j, L) : j;
-    jj = min(max(jj, 0), L - 1);
+    int jj = j;
+    bool zero = false;
+    if (j < 0 || j >= L) {
+        if (reflect) jj = idx_reflect(j, L);
+        else zero = true;
+        jj = min(max(jj, 0), L - 1);
+    }
     int pi = jj - 1;
     bool pz = false;
     if (pi < 0) {
-        pi = (border == CLAMP) ? 0 : 1;
+        pi = (border == CLAMP || L < 2) ? 0 : 1;
         pz = !(border == CLAMP || (border == REFLECT && L > 1));
     }
-    pi = min(pi, L - 1);
     const float xv = x[jj];
     const float pv = pz ? 0.0f : x[pi];
     const float p = c * pv;
     const float y = xv - p;
-    return (outside && !reflect) ? 0.0f : y;
+    return zero ? 0.0f : y;
 }
 
 // Generic staging (clip edges: reflect/zero padding, border rule, partial tiles); thread's max |y|.
@@ -566,8 +570,8 @@ __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi
 // 3 o-lo -- the lane order of a 32x32x16 MFMA A fragment.  A work unit forms four rows of one
 // frame (one 8-byte half of a slot); `nyq` gathers the thread's share of bin N/2 (sin term only).
 template <int PREC, int FT>
-__device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const LogMelArgs &a, int s, int buf,
-                                          float segScale, float invH, float &nyq, int tid)
+__device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const float *wHalf, const LogMelArgs &a,
+                                          int s, int buf, float segScale, float invH, float &nyq, int tid)
 {
     const int half = a.W / 2;
     uint2 *dst = reinterpret_cast<uint2 *>(ring);
@@ -583,7 +587,7 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const L
             e1 = up.y + dn.z; o1 = up.y - dn.z;
             e2 = up.z + dn.y; o2 = up.z - dn.y;
             e3 = up.w + dn.x; o3 = up.w - dn.x;
-            const float4 w = *reinterpret_cast<const float4 *>(&a.wHalf[j]);
+            const float4 w = *reinterpret_cast<const float4 *>(&wHalf[j]);
             nyq = fmaf(o0, w.x, nyq);                  // (-1)^row
             nyq = fmaf(o1, -w.y, nyq);
             nyq = fmaf(o2, w.z, nyq);
@@ -645,7 +649,7 @@ constexpr int kMelSeq = 320;
 // builds step s+1 into the other (one barrier per step); power in lane into bf16 hi/lo planes;
 // banded mel on MFMA with the weights loaded two steps ahead; dB out.
 template <int PREC, int FT, int CT>
-__global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats)
+__global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
 {
     constexpr int RT = FT / 32;
     constexpr int NW = 4;
@@ -665,6 +669,8 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     const int nSteps = a.Wp / 16;
     const float invH = 1.0f / static_cast<float>(a.H);
     const int nMT = a.Mpad / 16, nMelTiles = nMT * (FT / 16);
+    float *wH = lds + wOff;                              // window half (the Nyquist sin term), past the aliased region
+    for (int j = threadIdx.x; j < a.W / 2; j += blockDim.x) wH[j] = a.wHalf[j];
 
     // each wave's mel work list (the same for every tile): bands of the filter tiles it owns
     if ((threadIdx.x & 63) == 0) {
@@ -734,7 +740,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             }
         };
         float nyq = 0.0f;
-        prep_step<PREC, FT>(ring, seg, a, 0, 0, segScale, invH, nyq, tid);
+        prep_step<PREC, FT>(ring, seg, wH, a, 0, 0, segScale, invH, nyq, tid);
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) loadB(0, ct, 0);
         __syncthreads();
@@ -766,7 +772,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                         }
                         if (NB == 1 && s + 1 < nSteps) loadB(0, ct, s + 1);
                     }
-                    if (s + 1 < nSteps) prep_step<PREC, FT>(ring, seg, a, s + 1, u ^ 1, segScale, invH, nyq, tid);
+                    if (s + 1 < nSteps) prep_step<PREC, FT>(ring, seg, wH, a, s + 1, u ^ 1, segScale, invH, nyq, tid);
                     __syncthreads();                    // step s+1 built; buffer u free again
                 }
             }
@@ -810,7 +816,8 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             };
             if (n > 0) loadW(0, 0);
             if (n > 1) loadW(1, 1);
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            // two partial sums (even and odd entries) keep consecutive products independent
+            f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int e0 = 0; e0 < n; e0 += 2) {
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
@@ -823,6 +830,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                         const int q = fr * pws2 + k0 + 8 * (lane >> 4);
                         const bf16x8 ph = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q]));
                         const bf16x8 pl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q]));
+                        f32x4 &c = u ? c1 : c0;
                         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
                         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
                         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph, c, 0, 0, 0);
@@ -833,10 +841,11 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
 #pragma unroll
                                 for (int r = 0; r < 4; r++) {
                                     const int m = mt * 16 + 4 * (lane >> 4) + r;
-                                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c[r], a);
+                                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c0[r] + c1[r], a);
                                 }
                             }
-                            c = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                            c0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                            c1 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                         }
                     }
                 }
@@ -953,13 +962,15 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
         const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + kPF * 4 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
-        const size_t bytes = std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes);
+        const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes) + 15) / 16 * 16;
+        const int wOff = static_cast<int>(alias / sizeof(float));
+        const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
 #define RPPA_LAUNCH_ONE(PR, F, C)                                                                               \
         do {                                                                                                    \
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, F, C>),            \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));     \
-            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats); \
+            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats, wOff); \
         } while (0)
 #define RPPA_LAUNCH(PR)                                                                                         \
         do {                                                                                                    \
