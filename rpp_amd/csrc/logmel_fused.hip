@@ -413,76 +413,56 @@ __device__ __forceinline__ int owner_clip_wave(const int *tileStart, int B, int 
 }
 
 // Fast staging (tiles whose segment [s0, s0 + span) lies inside the clip with s0 >= 1 and starts
-// 16-byte aligned): every thread issues all its float4 loads first, up to kPF groups (group g goes
-// to thread g % blockDim), plus x[s0 - 1] for the first pre-emphasis product.
-constexpr int kPF = 12;
-struct SegLoad {
-    float4 v[kPF];
-    float first;
-};
+// 16-byte aligned): groups of four samples (group g to thread g % blockDim), kChunk groups per
+// thread in flight at a time, each with the sample before it for the pre-emphasis product.
+constexpr int kChunk = 4;
 
 __device__ __forceinline__ bool fast_segment(const LogMelArgs &a, const ClipView &v, int f0, int FT, int span)
 {
     const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
-    return s0 >= 1 && s0 + span <= v.L && v.F - f0 >= FT &&
-           (reinterpret_cast<uintptr_t>(v.x + s0) & 15) == 0 && (span + 3) / 4 <= kPF * static_cast<int>(blockDim.x);
+    return s0 >= 1 && s0 + span <= v.L && v.F - f0 >= FT && (reinterpret_cast<uintptr_t>(v.x + s0) & 15) == 0;
 }
 
-__device__ __forceinline__ void load_segment(SegLoad &p, const LogMelArgs &a, const ClipView &v, int f0, int span,
-                                             int tid)
+// Stages the segment of a fast tile; returns the thread's max |y|.
+__device__ __forceinline__ float stage_segment_fast(float *seg, const LogMelArgs &a, const ClipView &v, int f0,
+                                                    int span, float invH, int tid)
 {
     const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
     const float *x = v.x + s0;
-    const int groups = (span + 3) / 4;
-#pragma unroll
-    for (int q = 0; q < kPF; q++) {
-        const int g = tid + q * blockDim.x;
-        if (g < groups) {
-            // the last group may reach past the span; never read past the clip
-            const int i = 4 * g;
-            if (s0 + i + 4 <= v.L) p.v[q] = *reinterpret_cast<const float4 *>(x + i);
-            else p.v[q] = make_float4(x[i], s0 + i + 1 < v.L ? x[i + 1] : 0.0f, s0 + i + 2 < v.L ? x[i + 2] : 0.0f, 0.0f);
-        }
-    }
-    p.first = x[-1];
-}
-
-// Writes the loaded samples, pre-emphasised, into the skewed segment; returns the thread's max |y|.
-// The sample before each group is the previous lane's last one (the previous wave's last lane
-// through `last`, one float per wave and group round).  Contains a block barrier.
-__device__ __forceinline__ float store_segment(float *seg, float *last, const SegLoad &p, const LogMelArgs &a,
-                                               int span, float invH, int tid)
-{
-    const int groups = (span + 3) / 4;
-    const int lane = tid & 63;
-#pragma unroll
-    for (int q = 0; q < kPF; q++) {
-        const int g = tid + q * blockDim.x;
-        if (lane == 63 && g < groups) last[g >> 6] = p.v[q].w;
-    }
-    __syncthreads();
+    const int groups = (span + 3) / 4, nt = blockDim.x;
     float mx = 0.0f;
+    for (int g0 = tid; g0 < groups; g0 += kChunk * nt) {
+        float4 cur[kChunk];
+        float prev[kChunk];
 #pragma unroll
-    for (int q = 0; q < kPF; q++) {
-        const int g = tid + q * blockDim.x;
-        float prev = __shfl_up(p.v[q].w, 1);
-        if (lane == 0) prev = g == 0 ? p.first : last[(g >> 6) - 1];
-        if (g < groups) {
-            const float4 x = p.v[q];
-            float4 y;
-            float t;
-            t = a.coeff * prev; y.x = x.x - t;
-            t = a.coeff * x.x;  y.y = x.y - t;
-            t = a.coeff * x.y;  y.z = x.z - t;
-            t = a.coeff * x.z;  y.w = x.w - t;
-            const int i = 4 * g;
-            if (i + 4 > span) {  // past the span: zero (the fp16 scale must not see them)
-                if (i + 2 > span) y.y = 0.0f;
-                if (i + 3 > span) y.z = 0.0f;
-                y.w = 0.0f;
+        for (int k = 0; k < kChunk; k++) {
+            const int i = 4 * (g0 + k * nt);
+            if (i < span) {
+                // the last group may reach past the span; never read past the clip
+                if (s0 + i + 4 <= v.L) cur[k] = *reinterpret_cast<const float4 *>(x + i);
+                else cur[k] = make_float4(x[i], s0 + i + 1 < v.L ? x[i + 1] : 0.0f, s0 + i + 2 < v.L ? x[i + 2] : 0.0f, 0.0f);
+                prev[k] = x[i - 1];
             }
-            *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
-            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) {
+            const int i = 4 * (g0 + k * nt);
+            if (i < span) {
+                const float4 c = cur[k];
+                float4 y;
+                float t;
+                t = a.coeff * prev[k]; y.x = c.x - t;
+                t = a.coeff * c.x;     y.y = c.y - t;
+                t = a.coeff * c.y;     y.z = c.z - t;
+                t = a.coeff * c.z;     y.w = c.w - t;
+                if (i + 4 > span) {  // past the span: zero (the fp16 scale must not see them)
+                    if (i + 2 > span) y.y = 0.0f;
+                    if (i + 3 > span) y.z = 0.0f;
+                    y.w = 0.0f;
+                }
+                *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
+                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
+            }
         }
     }
     return mx;
@@ -657,7 +637,6 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[NW];
     __shared__ float nyqPart[NW * 64];
-    __shared__ float lastOf[kPF * NW];
     __shared__ uint16_t melSeq[NW][kMelSeq];
     __shared__ int melSeqLen[NW];
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
@@ -703,9 +682,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         // ---- stage the pre-emphasised segment; block max for the fp16 scale
         float mx;
         if (fast_segment(a, v, f0, FT, span)) {
-            SegLoad ld;
-            load_segment(ld, a, v, f0, span, tid);
-            mx = store_segment(seg, lastOf, ld, a, span, invH, tid);
+            mx = stage_segment_fast(seg, a, v, f0, span, invH, tid);
         } else {
             mx = stage_segment_slow(seg, a, v, f0, span, invH, tid);
         }
@@ -961,7 +938,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int span = (FT - 1) * a.H + a.W;
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
-        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + kPF * 4 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
+        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
         const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
         const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);

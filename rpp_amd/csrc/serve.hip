@@ -294,10 +294,13 @@ RPPA_EXPORT RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLog
             return RPP_ERROR_NOT_ENOUGH_MEMORY;
         }
     }
-    // build every table and size every workspace before traffic arrives: one launch per slot with
-    // capB short zero clips, the largest clip count a batch can have (a workspace that grew later
-    // would free its old buffer, which waits for the whole device)
-    const int warmLen = static_cast<int>(std::max<size_t>(4, std::min<size_t>(4096, inCap / capB / 4 * 4)));
+    // build every table, size every workspace and take each slot's copy paths once before traffic
+    // arrives: capB zero clips spread over the first 64 MB of the pinned input buffer, copied in,
+    // run and copied back (a workspace that grew later would free its old buffer, which waits for
+    // the whole device; the first copies through a fresh pinned range are slow)
+    const size_t warmIn = std::min<size_t>(inCap, static_cast<size_t>(16) << 20);
+    int warmLen = static_cast<int>(std::max<size_t>(4, warmIn / capB / 4 * 4));
+    while (warmLen > 4 && static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB > outCap) warmLen = (warmLen / 2 + 3) / 4 * 4;
     const bool canWarm = static_cast<size_t>(warmLen) * capB <= inCap &&
                          static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB <= outCap;
     for (Slot &sl : s->slots) {
@@ -309,11 +312,15 @@ RPPA_EXPORT RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLog
             dstOff[b] = static_cast<long long>(b) * frames_of(s, warmLen) * s->M;
             len[b] = warmLen;
         }
-        (void)hipMemsetAsync(sl.dIn, 0, sizeof(float) * warmLen * capB, sl.stream);
+        const size_t nIn = static_cast<size_t>(warmLen) * capB;
+        const size_t nOut = static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB;
+        std::memset(sl.hIn, 0, sizeof(float) * nIn);
+        (void)hipMemcpyAsync(sl.dIn, sl.hIn, sizeof(float) * nIn, hipMemcpyHostToDevice, sl.stream);
         (void)hipMemcpyAsync(sl.dMeta, sl.hMeta, sizeof(long long) * 2 * capB + sizeof(int) * capB,
                              hipMemcpyHostToDevice, sl.stream);
         RppStatus st = rppt_logmel_fused_packed_gpu(sl.dIn, sl.dMeta, reinterpret_cast<int *>(sl.dMeta + 2 * capB),
                                                     capB, sl.dOut, sl.dMeta + capB, params, precision, sl.h);
+        (void)hipMemcpyAsync(sl.hOut, sl.dOut, sizeof(float) * nOut, hipMemcpyDeviceToHost, sl.stream);
         (void)hipStreamSynchronize(sl.stream);
         if (st != RPP_SUCCESS) {
             destroy_server(s);
