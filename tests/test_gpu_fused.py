@@ -183,3 +183,30 @@ def test_fused_grid_and_alignment_invariance(gpu, orc, precision):
     for b, F in enumerate(frames):
         assert np.array_equal(outs[0][b, :, :F], outs[1][b, :, :F]), ("blocks per CU", b)
         assert np.array_equal(outs[0][b, :, :F], outs[2][b, :, :F]), ("alignment", b)
+
+
+@pytest.mark.parametrize("nfft,W,H,nmel,sr", [(256, 256, 128, 40, 8000.0), (768, 768, 192, 96, 24000.0),
+                                              (1024, 800, 320, 128, 32000.0), (512, 512, 256, 64, 16000.0)])
+def test_fused_mfma_shapes(gpu, handle, orc, nfft, W, H, nmel, sr):
+    """Every MFMA kernel instance (1-4 column tiles per wave, 64- and 32-frame tiles) against the
+    FP64 oracle at 0.05 dB, on ragged clips including edge lengths, with HTK and unnormalised
+    filters on one of them."""
+    rng = np.random.default_rng(nfft + W)
+    L = np.array([1, W // 2, W, W + H, 7 * H + 3, 40 * H, 123 * H + 17, 300 * H + 5], np.int32)
+    x = np.zeros((len(L), int(L.max()) + 8), np.float32)
+    for b, n in enumerate(L):
+        t = np.arange(n) / sr
+        x[b, :n] = (0.2 * rng.normal(size=n) + 0.5 * np.sin(2 * np.pi * (300 + 40 * b) * t)).astype(np.float32)
+    htk = nfft == 768
+    feat = dict(nfft=nfft, W=W, H=H, nmel=nmel, sample_rate=sr, fmin=0.0, fmax=sr / 2)
+    frames = L // H + 1
+    out = torch.full((len(L), nmel, int(frames.max())), float("nan"), device="cuda")
+    p = gpu.logmel_params(nfft=nfft, window_length=W, window_step=H, sample_rate=sr, min_freq=0.0, max_freq=sr / 2,
+                          num_filter=nmel, formula="htk" if htk else "slaney", normalize=not htk)
+    gpu.logmel_fused(handle, dev(x), out, dev(L), p, "fp16x3")
+    torch.cuda.synchronize()
+    ref, _, fr = orc.logmel_f64(x, L, nfft=nfft, W=W, H=H, nmel=nmel, sample_rate=sr, fmax=sr / 2,
+                                formula="htk" if htk else "slaney", normalize=not htk, nthreads=8)
+    err = max_db_err(out.cpu().numpy(), ref, fr)
+    print(f"fp16x3 nfft {nfft} W {W} H {H} M {nmel}: max |dB err| vs FP64 = {err:.3e}")
+    assert err <= DB_TOL_MIXED
