@@ -412,58 +412,132 @@ __device__ __forceinline__ int owner_clip_wave(const int *tileStart, int B, int 
     return lo;
 }
 
-// Fast staging (tiles whose segment [s0, s0 + span) lies inside the clip with s0 >= 1 and starts
-// 16-byte aligned): groups of four samples (group g to thread g % blockDim), kChunk groups per
-// thread in flight at a time, each with the sample before it for the pre-emphasis product.
-constexpr int kChunk = 4;
+// One tile of the work list: its clip, first frame, and where its segment [s0, s0 + span) sits.
+struct TileInfo {
+    ClipView v;
+    int f0, s0, valid;   // valid: samples of the span that belong to a frame below F (the rest are zero)
+    int iLo, iHi;        // first and last interior group start (iHi < iLo: none)
+    bool aligned;        // clip samples at s0 start 16-byte aligned
+};
 
-__device__ __forceinline__ bool fast_segment(const LogMelArgs &a, const ClipView &v, int f0, int FT, int span)
+__device__ __forceinline__ TileInfo tile_info(const LogMelArgs &a, int tile, int FT, int span, int lane)
 {
-    const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
-    return s0 >= 1 && s0 + span <= v.L && v.F - f0 >= FT && (reinterpret_cast<uintptr_t>(v.x + s0) & 15) == 0;
+    TileInfo t;
+    const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
+    t.v = clip_view(a, b);
+    t.f0 = (tile - a.tileStart[b]) * FT;
+    t.s0 = t.f0 * a.H - (a.center ? a.W / 2 : 0);
+    const int nfr = max(0, t.v.F - t.f0);
+    t.valid = (nfr > 0 && t.v.L > 0) ? min(span, (nfr - 1) * a.H + a.W) : 0;
+    t.aligned = ((reinterpret_cast<uintptr_t>(t.v.x) + 4 * static_cast<intptr_t>(t.s0)) & 15) == 0;
+    // interior group starts i (multiples of 4): s0 + i - 1 >= 0, s0 + i + 3 < L, i + 3 < valid
+    t.iLo = max(0, (1 - t.s0 + 3) & ~3);
+    const int hi = min(t.v.L - 4 - t.s0, t.valid - 4);
+    t.iHi = hi < 0 ? -4 : (hi & ~3);
+    return t;
 }
 
-// Stages the segment of a fast tile; returns the thread's max |y|.
-__device__ __forceinline__ float stage_segment_fast(float *seg, const LogMelArgs &a, const ClipView &v, int f0,
-                                                    int span, float invH, int tid)
+// Staging works on groups of four samples (group g, samples 4g .. 4g+3 of the span, to thread
+// g % blockDim).  An interior group -- its samples and the one before them inside the clip, all in
+// the valid part -- is pre-emphasised from plain loads; the others (clip edges: reflect/zero
+// padding, the border rule, samples past the last frame) sample by sample.  The interior groups
+// of a thread's first kPF rounds are loaded ahead, into registers, while the previous tile's mel
+// product runs.
+constexpr int kPF = 11;
+struct SegPrefetch {
+    float4 v[kPF];
+    float prev[kPF];
+};
+
+__device__ __forceinline__ bool interior_group(const TileInfo &t, int i) { return i >= t.iLo && i <= t.iHi; }
+
+__device__ __forceinline__ void load_group(const TileInfo &t, int i, float4 &x4, float &prev)
 {
-    const int s0 = f0 * a.H - (a.center ? a.W / 2 : 0);
-    const float *x = v.x + s0;
+    const float *x = t.v.x + t.s0 + i;
+    if (t.aligned) x4 = *reinterpret_cast<const float4 *>(x);
+    else x4 = make_float4(x[0], x[1], x[2], x[3]);
+    prev = x[-1];
+}
+
+// Fresh (zero) contents: the registers of the previous prefetch must not stay live across a tile
+// whose successor loads nothing.
+__device__ __forceinline__ void seg_prefetch_none(SegPrefetch &p)
+{
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+        p.v[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        p.prev[q] = 0.0f;
+    }
+}
+
+// Straight-line loads (a group outside the interior reads the nearest interior group instead, and
+// its values are never used), so the compiler keeps them in registers while the mel product runs.
+__device__ __forceinline__ void seg_prefetch(SegPrefetch &p, const TileInfo &t, int span, int tid)
+{
+    if (t.iHi < t.iLo) {
+        seg_prefetch_none(p);
+        return;
+    }
+    const int nt = blockDim.x;
+#pragma unroll
+    for (int q = 0; q < kPF; q++) {
+        const int i = min(max(4 * (tid + q * nt), t.iLo), t.iHi);
+        load_group(t, i, p.v[q], p.prev[q]);
+    }
+}
+
+__device__ __forceinline__ float4 pe_group(const float4 x, float prev, float c)
+{
+    float4 y;
+    float t;
+    t = c * prev; y.x = x.x - t;
+    t = c * x.x;  y.y = x.y - t;
+    t = c * x.y;  y.z = x.z - t;
+    t = c * x.z;  y.w = x.w - t;
+    return y;
+}
+
+__device__ __forceinline__ float pe_sample_sel(const float *x, int L, int j, float c, int border, int reflect);
+
+// Edge group: sample by sample, zero past the valid part.
+__device__ __forceinline__ float4 edge_group(const LogMelArgs &a, const TileInfo &t, int i)
+{
+    float y[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        y[k] = (i + k < t.valid) ? pe_sample_sel(t.v.x, t.v.L, t.s0 + i + k, a.coeff, a.border, a.reflect) : 0.0f;
+    return make_float4(y[0], y[1], y[2], y[3]);
+}
+
+// Writes the tile's pre-emphasised segment into LDS (skewed); returns the thread's max |y|.
+__device__ __forceinline__ float seg_store(float *seg, const SegPrefetch &p, const LogMelArgs &a, const TileInfo &t,
+                                           int span, float invH, int tid)
+{
     const int groups = (span + 3) / 4, nt = blockDim.x;
     float mx = 0.0f;
-    for (int g0 = tid; g0 < groups; g0 += kChunk * nt) {
-        float4 cur[kChunk];
-        float prev[kChunk];
+    auto put = [&](int i, float4 y) {
+        *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
+    };
 #pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            const int i = 4 * (g0 + k * nt);
-            if (i < span) {
-                // the last group may reach past the span; never read past the clip
-                if (s0 + i + 4 <= v.L) cur[k] = *reinterpret_cast<const float4 *>(x + i);
-                else cur[k] = make_float4(x[i], s0 + i + 1 < v.L ? x[i + 1] : 0.0f, s0 + i + 2 < v.L ? x[i + 2] : 0.0f, 0.0f);
-                prev[k] = x[i - 1];
-            }
+    for (int q = 0; q < kPF; q++) {  // interior groups of the prefetched rounds
+        const int g = tid + q * nt, i = 4 * g;
+        if (g < groups && interior_group(t, i)) put(i, pe_group(p.v[q], p.prev[q], a.coeff));
+    }
+    for (int g = tid, q = 0; g < groups; g += nt, q++) {  // edge groups, and rounds past the prefetch
+        const int i = 4 * g;
+        const bool in = interior_group(t, i);
+        if (in && q < kPF) continue;
+        float4 y;
+        if (in) {
+            float4 x4;
+            float prev;
+            load_group(t, i, x4, prev);
+            y = pe_group(x4, prev, a.coeff);
+        } else {
+            y = edge_group(a, t, i);
         }
-#pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            const int i = 4 * (g0 + k * nt);
-            if (i < span) {
-                const float4 c = cur[k];
-                float4 y;
-                float t;
-                t = a.coeff * prev[k]; y.x = c.x - t;
-                t = a.coeff * c.x;     y.y = c.y - t;
-                t = a.coeff * c.y;     y.z = c.z - t;
-                t = a.coeff * c.z;     y.w = c.w - t;
-                if (i + 4 > span) {  // past the span: zero (the fp16 scale must not see them)
-                    if (i + 2 > span) y.y = 0.0f;
-                    if (i + 3 > span) y.z = 0.0f;
-                    y.w = 0.0f;
-                }
-                *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
-                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fmaxf(fabsf(y.z), fabsf(y.w))));
-            }
-        }
+        put(i, y);
     }
     return mx;
 }
@@ -491,36 +565,6 @@ __device__ __forceinline__ float pe_sample_sel(const float *x, int L, int j, flo
     const float p = c * pv;
     const float y = xv - p;
     return zero ? 0.0f : y;
-}
-
-// Generic staging (clip edges: reflect/zero padding, border rule, partial tiles); thread's max |y|.
-// Four samples per thread per round, their loads issued together.
-__device__ __forceinline__ float stage_segment_slow(float *seg, const LogMelArgs &a, const ClipView &v, int f0, int span,
-                                                    float invH, int tid)
-{
-    const int off = a.center ? a.W / 2 : 0;
-    const int s0 = f0 * a.H - off;
-    const int nfr = max(0, v.F - f0);
-    const int valid = (nfr > 0 && v.L > 0) ? min(span, (nfr - 1) * a.H + a.W) : 0;
-    const int nt = blockDim.x;
-    float mx = 0.0f;
-    for (int i0 = tid; i0 < span; i0 += 4 * nt) {
-        float y[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = i0 + k * nt;
-            y[k] = i < valid ? pe_sample_sel(v.x, v.L, s0 + i, a.coeff, a.border, a.reflect) : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = i0 + k * nt;
-            if (i < span) {
-                seg[seg_pos_r(i, invH)] = y[k];
-                mx = fmaxf(mx, fabsf(y[k]));
-            }
-        }
-    }
-    return mx;
 }
 
 // Two-element hi+lo split, packed as the adjacent halves of an MFMA operand (element j in the low
@@ -672,20 +716,23 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         melSeqLen[w] = n;
     }
 
-    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
+    // the first tile's segment is fetched here; every later one while its predecessor's mel runs
+    SegPrefetch pf;
+    TileInfo cur;
+    int tile = blockIdx.x;
+    if (tile < T) {
+        const int tid = threadIdx.x;
+        cur = tile_info(a, tile, FT, span, tid & 63);
+        seg_prefetch(pf, cur, span, tid);
+    }
+    for (; tile < T; tile += gridDim.x) {
         const int tid = opaque(threadIdx.x), wave = tid >> 6, lane = tid & 63;
         const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
-        const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
-        const int f0 = (tile - a.tileStart[b]) * FT;
-        const ClipView v = clip_view(a, b);
+        const ClipView v = cur.v;
+        const int f0 = cur.f0;
 
         // ---- stage the pre-emphasised segment; block max for the fp16 scale
-        float mx;
-        if (fast_segment(a, v, f0, FT, span)) {
-            mx = stage_segment_fast(seg, a, v, f0, span, invH, tid);
-        } else {
-            mx = stage_segment_slow(seg, a, v, f0, span, invH, tid);
-        }
+        float mx = seg_store(seg, pf, a, cur, span, invH, tid);
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
         if (lane == 0) red[wave] = mx;
         __syncthreads();
@@ -777,6 +824,17 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             pwL[f * pws2 + k] = 0;
         }
         __syncthreads();
+
+        // ---- the next tile's samples load while the mel product runs (the accumulators are dead)
+        {
+            const int nt = tile + gridDim.x;
+            if (nt < T) {
+                cur = tile_info(a, nt, FT, span, lane);
+                seg_prefetch(pf, cur, span, tid);
+            } else {
+                seg_prefetch_none(pf);
+            }
+        }
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
         //      16-filter tile's band of bins; the wave walks its work list with the weight
