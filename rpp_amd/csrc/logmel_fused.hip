@@ -588,18 +588,21 @@ __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi
     }
 }
 
-// Folded, split DFT operand of one 16-row contraction step s, built into buffer `buf` of the slice
-// ring while the waves run the previous step's products: slot ((buf * 4 + arr) * 2 + g) * FT + f
-// holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7, for arr = 0 e-hi, 1 e-lo, 2 o-hi,
-// 3 o-lo -- the lane order of a 32x32x16 MFMA A fragment.  A work unit forms four rows of one
-// frame (one 8-byte half of a slot); `nyq` gathers the thread's share of bin N/2 (sin term only).
+// Folded, split DFT operand of SPB 16-row contraction steps starting at step s, built into buffer
+// `buf` of the slice ring while the waves run the previous steps' products: slot
+// ((buf * 4 + arr) * 2SPB + g) * FT + f holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7,
+// for arr = 0 e-hi, 1 e-lo, 2 o-hi, 3 o-lo -- the lane order of a 32x32x16 MFMA A fragment.  A work
+// unit forms four rows of one frame (one 8-byte half of a slot); `nyq` gathers the thread's share
+// of bin N/2 (sin term only).
+constexpr int kSPB = 2;   // contraction steps per barrier
 template <int PREC, int FT>
 __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const float *wHalf, const LogMelArgs &a,
                                           int s, int buf, float segScale, float invH, float &nyq, int tid)
 {
+    constexpr int GB = 2 * kSPB;                         // 8-row groups per buffer
     const int half = a.W / 2;
     uint2 *dst = reinterpret_cast<uint2 *>(ring);
-    for (int u = tid; u < 4 * FT; u += blockDim.x) {
+    for (int u = tid; u < 2 * GB * FT; u += blockDim.x) {
         const int f = u % FT, qd = u / FT;
         const int j = 16 * s + 4 * qd;                 // rows j .. j+3 (j even)
         float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
@@ -622,11 +625,11 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
         split2<PREC>(e2, e3, segScale, eh1, el1);
         split2<PREC>(o0, o1, segScale, oh0, ol0);
         split2<PREC>(o2, o3, segScale, oh1, ol1);
-        const int slot = (buf * 8 + (qd >> 1)) * FT + f, hh = qd & 1;
-        dst[(slot + 0 * 2 * FT) * 2 + hh] = make_uint2(eh0, eh1);
-        dst[(slot + 1 * 2 * FT) * 2 + hh] = make_uint2(el0, el1);
-        dst[(slot + 2 * 2 * FT) * 2 + hh] = make_uint2(oh0, oh1);
-        dst[(slot + 3 * 2 * FT) * 2 + hh] = make_uint2(ol0, ol1);
+        const int slot = (buf * 4 * GB + (qd >> 1)) * FT + f, hh = qd & 1;
+        dst[(slot + 0 * GB * FT) * 2 + hh] = make_uint2(eh0, eh1);
+        dst[(slot + 1 * GB * FT) * 2 + hh] = make_uint2(el0, el1);
+        dst[(slot + 2 * GB * FT) * 2 + hh] = make_uint2(oh0, oh1);
+        dst[(slot + 3 * GB * FT) * 2 + hh] = make_uint2(ol0, ol1);
     }
 }
 
@@ -641,21 +644,37 @@ __device__ __forceinline__ void put_power(uint16_t *pwH, uint16_t *pwL, int idx,
 }
 
 // power = cos^2 + sin^2 of the lane's accumulators (|X| when SQRT), split into the planes.
+// The DFT accumulators hold X^T: row = bin, column = frame, so a lane owns frame rt*32 + lane%32
+// and, per group of four registers, four consecutive bins.  power = cos^2 + sin^2 (|X| when SQRT)
+// of those four bins, split into bf16 hi and lo, goes out as one 8-byte store per plane.
 template <bool SQRT, int RT, int CT>
 __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&accC)[RT][CT],
-                                            const f32x16 (&accS)[RT][CT], float unscale, int bin0, int lane)
+                                            const f32x16 (&accS)[RT][CT], float unscale, int binTile0, int lane)
 {
 #pragma unroll
     for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
-            const int base = (rt * 32 + 4 * (lane >> 5)) * pws2 + bin0 + ct * 32;
+            const int base = (rt * 32 + (lane & 31)) * pws2 + binTile0 + ct * 32 + 4 * (lane >> 5);
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
-                float p = re * re + im * im;
-                if (SQRT) p = sqrtf(p);
-                put_power(pwH, pwL, base + (8 * (r >> 2) + (r & 3)) * pws2, p);
+            for (int q = 0; q < 4; q++) {
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int r = 4 * q + k;
+                    const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
+                    float p = re * re + im * im;
+                    if (SQRT) p = sqrtf(p);
+                    const __bf16 hb = static_cast<__bf16>(p);
+                    const __bf16 lb = static_cast<__bf16>(p - static_cast<float>(hb));
+                    h[k] = __builtin_bit_cast(uint16_t, hb);
+                    l[k] = __builtin_bit_cast(uint16_t, lb);
+                }
+                const int idx = base + 8 * q;   // bins binTile0 + ct*32 + 8q + 4(lane/32) + 0..3
+                *reinterpret_cast<uint2 *>(&pwH[idx]) =
+                    make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
+                *reinterpret_cast<uint2 *>(&pwL[idx]) =
+                    make_uint2(l[0] | (static_cast<uint32_t>(l[1]) << 16), l[2] | (static_cast<uint32_t>(l[3]) << 16));
             }
         }
     }
@@ -684,7 +703,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     __shared__ uint16_t melSeq[NW][kMelSeq];
     __shared__ int melSeqLen[NW];
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
-    u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2][FT] operand slices
+    u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2 kSPB][FT] operand slices
     const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
     const int T = a.tileStart[a.B];
     const int span = (FT - 1) * a.H + a.W;
@@ -764,25 +783,28 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             }
         };
         float nyq = 0.0f;
+        constexpr int GB = 2 * kSPB;
         prep_step<PREC, FT>(ring, seg, wH, a, 0, 0, segScale, invH, nyq, tid);
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) loadB(0, ct, 0);
         __syncthreads();
-        for (int s0 = 0; s0 < nSteps; s0 += 2) {
+        const int nBlk = (nSteps + kSPB - 1) / kSPB;
+        for (int sb = 0; sb < nBlk; sb++) {
+            const int rb = sb & 1;                      // operand ring buffer of these steps
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const int s = s0 + u;                   // s & 1 == u: operand buffer of step s
-                const int buf = NB == 2 ? u : 0;         // basis buffer holding step s
+            for (int h = 0; h < kSPB; h++) {
+                const int s = sb * kSPB + h;
+                const int buf = NB == 2 ? (h & 1) : 0;   // basis buffer holding step s (kSPB is even)
                 if (s < nSteps) {
                     if (NB == 2 && s + 1 < nSteps) loadB(buf ^ 1, 0, s + 1);
                     u16x8 ah[RT][2], oh[RT][2];
 #pragma unroll
                     for (int rt = 0; rt < RT; rt++) {
-                        const int slot = (u * 8 + (lane >> 5)) * FT + rt * 32 + (lane & 31);
-                        ah[rt][0] = ring[slot + 0 * 2 * FT];
-                        ah[rt][1] = ring[slot + 1 * 2 * FT];
-                        oh[rt][0] = ring[slot + 2 * 2 * FT];
-                        oh[rt][1] = ring[slot + 3 * 2 * FT];
+                        const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + rt * 32 + (lane & 31);
+                        ah[rt][0] = ring[slot + 0 * GB * FT];
+                        ah[rt][1] = ring[slot + 1 * GB * FT];
+                        oh[rt][0] = ring[slot + 2 * GB * FT];
+                        oh[rt][1] = ring[slot + 3 * GB * FT];
                     }
                     // keep the LDS reads together ahead of the products (left alone, the scheduler
                     // serialises each read with its first product)
@@ -791,15 +813,16 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                     for (int ct = 0; ct < CT; ct++) {
 #pragma unroll
                         for (int rt = 0; rt < RT; rt++) {
-                            mfma3<PREC>(accC[rt][ct], ah[rt][0], ah[rt][1], bc[buf][ct][0], bc[buf][ct][1]);
-                            mfma3<PREC>(accS[rt][ct], oh[rt][0], oh[rt][1], bs[buf][ct][0], bs[buf][ct][1]);
+                            // X^T = basis (A: rows = bins) x frames (B: columns = frames)
+                            mfma3<PREC>(accC[rt][ct], bc[buf][ct][0], bc[buf][ct][1], ah[rt][0], ah[rt][1]);
+                            mfma3<PREC>(accS[rt][ct], bs[buf][ct][0], bs[buf][ct][1], oh[rt][0], oh[rt][1]);
                         }
                         if (NB == 1 && s + 1 < nSteps) loadB(0, ct, s + 1);
                     }
-                    if (s + 1 < nSteps) prep_step<PREC, FT>(ring, seg, wH, a, s + 1, u ^ 1, segScale, invH, nyq, tid);
-                    __syncthreads();                    // step s+1 built; buffer u free again
                 }
             }
+            if (sb + 1 < nBlk) prep_step<PREC, FT>(ring, seg, wH, a, (sb + 1) * kSPB, rb ^ 1, segScale, invH, nyq, tid);
+            __syncthreads();                            // next steps built; buffer rb free again
         }
         nyqPart[tid] = nyq;
         __syncthreads();  // segment and operand ring are dead: the power planes may overwrite them
@@ -809,8 +832,9 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
         uint16_t *pwL = pwH + FT * pws2;
         const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
-        if (a.power == 1) store_power<true, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, bin0, lane);
-        else store_power<false, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, bin0, lane);
+        const int binTile0 = wave * (CT * 32);
+        if (a.power == 1) store_power<true, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, binTile0, lane);
+        else store_power<false, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, binTile0, lane);
         if (tid < FT) {
             float n = 0.0f;
             for (int q = tid; q < NW * 64; q += FT) n += nyqPart[q];
@@ -997,7 +1021,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
         const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
-        const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * FT, pwBytes) + 15) / 16 * 16;
+        const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * kSPB * FT, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
         const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
