@@ -216,13 +216,15 @@ typedef struct {
     Rpp32s numFilter;
     RpptMelScaleFormula melFormula;
     Rpp32s normalize;
-    Rpp32f cutOffDB, multiplier, referenceMagnitude;  /* referenceMagnitude must be > 0 */
+    Rpp32f cutOffDB, multiplier, referenceMagnitude;  /* referenceMagnitude >= 0; 0 = each clip's maximum */
 } RpptLogMelParams;
 
 /* Whole chain pre-emphasis -> spectrogram -> mel -> to_decibels over a ragged batch in one
- * persistent launch.  src [n][nStride] f32 (device), srcLengthTensor device or pinned,
- * dst NFT [n][numFilter][hStride] f32 with hStride >= frames of every clip.  Equivalent to
- * the four rppt_*_gpu calls with the matching arguments. */
+ * persistent launch.  src [n][nStride] f32 (device), srcLengthTensor device or pinned.
+ * dst NFT [n][numFilter >= h rows][hStride] f32 (a clip's frames beyond hStride are not written)
+ * or NTF [n][h][hStride >= numFilter] (frames beyond h are not written).  Equivalent to the four
+ * rppt_*_gpu calls with the matching arguments.  referenceMagnitude 0 (each clip's maximum, as
+ * rppt_to_decibels_gpu) adds a second in-place pass over the output after the fused launch. */
 RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
                                 Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
                                 RpptLogMelPrecision precision, rppHandle_t rppHandle);

@@ -295,11 +295,12 @@ def logmel_params(*, coeff=0.97, border="clamp", nfft=512, window_length=400, wi
 
 
 def logmel_fused(h: Handle, src, dst, lengths, params: RpptLogMelParams, precision="bf16x3",
-                 check_status=True) -> int:
-    """src [B, stride]; dst [B, numFilter, hStride] NFT; lengths int32 cuda [B]."""
+                 check_status=True, layout=NFT) -> int:
+    """src [B, stride]; lengths int32 cuda [B]; dst [B, numFilter, hStride] (NFT) or
+    [B, maxFrames, hStride >= numFilter] (NTF)."""
     B, S = src.shape
     sd = desc_1d(B, S)
-    dd = desc_2d(B, dst.shape[1], dst.shape[2])
+    dd = desc_2d(B, dst.shape[1], dst.shape[2], layout=layout)
     st = lib().rppt_logmel_fused_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(lengths), C.byref(params),
                                      PRECISION[precision], h.h)
     if check_status:

@@ -47,6 +47,10 @@ struct LogMelArgs {
     int N, W, H, center, reflect, power;
     int K, M;
     float minRatio, mult, invRef;
+    int ntf;               // output [n][frames][hStride] (else [n][M][hStride]); never with the packed layout
+    uint32_t frameCap;     // frames a clip's output holds (hStride for NFT, the descriptor's rows for NTF)
+    int *clipMax;          // per-clip maximum of the mel power (float bits) when the reference is the
+                           // clip maximum: the kernel then stores power and a second pass converts to dB
     const int *tileStart;  // [B + 1]
     // packed layout (both null for the [n][nStride] / [n][M][hStride] descriptors): clip b's samples
     // at src + srcOff[b], its dB rows at dst + dstOff[b] as a dense [M][frames_b] block
@@ -91,7 +95,7 @@ struct ClipView {
     const float *x;
     float *out;
     uint32_t hs;
-    int L, F;
+    int L, F, b;
 };
 
 __device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
@@ -111,10 +115,24 @@ __device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
     } else {
         v.out = a.dst + static_cast<size_t>(b) * a.dstNStride;
         v.hs = a.hStride;
-        F = min(F, static_cast<int>(a.hStride));
+        F = min(F, static_cast<int>(a.frameCap));
     }
     v.F = F;
+    v.b = b;
     return v;
+}
+
+// Address of mel row m, frame f of a clip's output.
+__device__ __forceinline__ float *out_at(const LogMelArgs &a, const ClipView &v, int m, int f)
+{
+    return a.ntf ? v.out + static_cast<size_t>(f) * v.hs + m : v.out + static_cast<size_t>(m) * v.hs + f;
+}
+
+// Folds a thread's running maximum of stored mel power into the clip's (whole wave active).
+__device__ __forceinline__ void clip_max_update(const LogMelArgs &a, int b, float mx)
+{
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&a.clipMax[b], __float_as_int(mx));
 }
 
 // Pre-emphasised sample at signal index j of a clip of length L, with the spectrogram's padding
@@ -164,8 +182,10 @@ __device__ __forceinline__ float db_exact(float v, const LogMelArgs &a)
 // prefix sum into tileStart[0..B], total in tileStart[B].
 __global__ void __launch_bounds__(1024)
 plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, uint32_t hStride, int W, int H,
-                  int center, int FT, int packed, int *__restrict__ tileStart)
+                  int center, int FT, int packed, int *__restrict__ tileStart, int *__restrict__ clipMax)
 {
+    if (clipMax)  // the fused kernel raises these with atomicMax (float bits; stored power is never below -FLT_MAX)
+        for (int b = threadIdx.x; b < B; b += blockDim.x) clipMax[b] = __float_as_int(-3.40282347e38f);
     __shared__ int waveSum[16];
     __shared__ int carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -229,6 +249,7 @@ __device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, 
                                  bool exactLog)
 {
     const int nOut = a.M * FT;
+    float mx = -3.40282347e38f;
     for (int e = threadIdx.x; e < nOut; e += kThreads) {
         const int m = e / FT, f = e % FT;
         if (f0 + f >= v.F) continue;
@@ -242,7 +263,33 @@ __device__ void mel_gather_store(const float *pw, int pws, const LogMelArgs &a, 
             const float p = pw[f * pws + k] * w;
             acc = acc + p;
         }
-        v.out[static_cast<size_t>(m) * v.hs + f0 + f] = exactLog ? db_exact(acc, a) : db_fast(acc, a);
+        if (a.clipMax) {
+            *out_at(a, v, m, f0 + f) = acc;
+            mx = fmaxf(mx, acc);
+        } else {
+            *out_at(a, v, m, f0 + f) = exactLog ? db_exact(acc, a) : db_fast(acc, a);
+        }
+    }
+    if (a.clipMax) clip_max_update(a, v.b, mx);
+}
+
+// Second pass when the dB reference is the clip maximum (cpu/kernel/to_decibels.cpp:58-75): power
+// stored by the fused kernel becomes mult * log2(max(minRatio, x / max)), 1 / max read as 1 for a
+// zero maximum.  blockIdx.y strides over the clips.
+__global__ void __launch_bounds__(256) db_from_clip_max_kernel(LogMelArgs a, int exactLog)
+{
+    for (int b = blockIdx.y; b < a.B; b += gridDim.y) {
+        const ClipView v = clip_view(a, b);
+        const float ref = __int_as_float(a.clipMax[b]);
+        LogMelArgs c = a;
+        c.invRef = ref != 0.0f ? 1.0f / ref : 1.0f;
+        const long long n = static_cast<long long>(a.M) * v.F;
+        for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+             e += static_cast<long long>(gridDim.x) * blockDim.x) {
+            const int m = static_cast<int>(e / v.F), f = static_cast<int>(e % v.F);
+            float *o = out_at(a, v, m, f);
+            *o = exactLog ? db_exact(*o, c) : db_fast(*o, c);
+        }
     }
 }
 
@@ -875,6 +922,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             };
             if (n > 0) loadW(0, 0);
             if (n > 1) loadW(1, 1);
+            float melMax = -3.40282347e38f;
             // two partial sums (even and odd entries) keep consecutive products independent
             f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int e0 = 0; e0 < n; e0 += 2) {
@@ -896,11 +944,14 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                         if (e + 1 == n || (seq[e + 1] >> 8) != tt) {
                             // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
                             if (f0 + fr < v.F) {
-                                float *out = v.out + f0 + fr;
 #pragma unroll
                                 for (int r = 0; r < 4; r++) {
                                     const int m = mt * 16 + 4 * (lane >> 4) + r;
-                                    if (m < a.M) out[static_cast<size_t>(m) * v.hs] = db_fast(c0[r] + c1[r], a);
+                                    if (m < a.M) {
+                                        const float pm = c0[r] + c1[r];
+                                        if (a.clipMax) melMax = fmaxf(melMax, pm);
+                                        *out_at(a, v, m, f0 + fr) = a.clipMax ? pm : db_fast(pm, a);
+                                    }
                                 }
                             }
                             c0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -909,6 +960,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                     }
                 }
             }
+            if (a.clipMax) clip_max_update(a, v.b, melMax);
         }
         __syncthreads();  // the power planes are reused by the next tile's segment
     }
@@ -923,7 +975,7 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
         return RPP_ERROR_INVALID_ARGUMENTS;
     if (p->maxFreq < 0 || p->maxFreq > p->sampleRate / 2 || p->minFreq < 0 || p->minFreq > p->sampleRate / 2)
         return RPP_ERROR_INVALID_ARGUMENTS;
-    if (!(p->referenceMagnitude > 0.0f)) return RPP_ERROR_NOT_IMPLEMENTED;  // per-clip max needs a second pass
+    if (p->referenceMagnitude < 0.0f) return RPP_ERROR_INVALID_ARGUMENTS;
     if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
     // the folded MFMA path: nfft 256, 512, 768 or 1024, window length a multiple of 16 (its half a
     // multiple of 8), hop a multiple of 8, at most 256 mel filters
@@ -937,7 +989,7 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
 // Validated parameters + buffers -> planner and fused kernel on h->stream.
 RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPrecision precision, const float *src,
                         uint32_t srcStride, const long long *srcOff, const int *lengths, int B, float *dst,
-                        uint32_t dstNStride, uint32_t hStride, const long long *dstOff)
+                        uint32_t dstNStride, uint32_t hStride, const long long *dstOff, int ntf, uint32_t frameCap)
 {
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
@@ -947,8 +999,9 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
                                          p->numFilter, p->normalize ? 1 : 0);
     if (!dt || !mt) return RPP_ERROR;
-    int *tileStart = static_cast<int *>(workspace(h, sizeof(int) * (B + 1)));
+    int *tileStart = static_cast<int *>(workspace(h, sizeof(int) * (2 * B + 1)));   // + per-clip maxima
     if (!tileStart) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+    const bool clipRef = p->referenceMagnitude == 0.0f;
 
     LogMelArgs a{};
     a.src = src;
@@ -975,7 +1028,10 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     if (minRatio == 0.0f) minRatio = std::nextafter(0.0f, 1.0f);
     a.minRatio = minRatio;
     a.mult = p->multiplier * 0.3010299956639812f;
-    a.invRef = 1.f / p->referenceMagnitude;
+    a.invRef = clipRef ? 1.0f : 1.f / p->referenceMagnitude;
+    a.ntf = ntf;
+    a.frameCap = frameCap;
+    a.clipMax = clipRef ? tileStart + B + 1 : nullptr;
     a.tileStart = tileStart;
     a.basis = dt->basis;
     a.cols = dt->cols;
@@ -1001,8 +1057,8 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
-    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.hStride,
-                       a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart);
+    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.frameCap,
+                       a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax);
     const int pws = a.KP + kSkew;
     if (!mfma) {
         const int span = (kFT32 - 1) * a.H + a.W;
@@ -1045,6 +1101,10 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
 #undef RPPA_LAUNCH_ONE
     }
     RPPA_HIP(hipGetLastError());
+    if (clipRef) {
+        hipLaunchKernelGGL(db_from_clip_max_kernel, dim3(32, std::min(B, 65535)), dim3(256), 0, h->stream, a, mfma ? 0 : 1);
+        RPPA_HIP(hipGetLastError());
+    }
     return RPP_SUCCESS;
 }
 
@@ -1081,17 +1141,23 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
 {
     if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
     if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
-    if (dstDescPtr->layout != NFT) return RPP_ERROR_INVALID_DST_LAYOUT;
+    if (dstDescPtr->layout != NFT && dstDescPtr->layout != NTF) return RPP_ERROR_INVALID_DST_LAYOUT;
     if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
     if (!srcPtr || !dstPtr || !srcLengthTensor) return RPP_ERROR_INVALID_ARGUMENTS;
     RppStatus st = check_params(p, precision);
     if (st != RPP_SUCCESS) return st;
     const unsigned B = srcDescPtr->n;
     if (B == 0) return RPP_SUCCESS;
-    if (dstDescPtr->h < static_cast<unsigned>(p->numFilter)) return RPP_ERROR_INVALID_DST_DIMS;
+    // NFT: rows are filters (h >= numFilter), a row holds up to hStride frames.
+    // NTF: rows are frames (at most h of them are written), a row holds hStride >= numFilter values.
+    const int ntf = dstDescPtr->layout == NTF ? 1 : 0;
+    if (ntf ? dstDescPtr->strides.hStride < static_cast<unsigned>(p->numFilter)
+            : dstDescPtr->h < static_cast<unsigned>(p->numFilter))
+        return RPP_ERROR_INVALID_DST_DIMS;
+    const uint32_t frameCap = ntf ? dstDescPtr->h : dstDescPtr->strides.hStride;
     return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
                          srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
-                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr);
+                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap);
 }
 
 // Autotuner: times every tile shape the launch supports on the caller's batch (events on the
@@ -1145,5 +1211,5 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_packed_gpu(const Rpp32f *srcPtr, const l
     if (st != RPP_SUCCESS) return st;
     if (batchSize == 0) return RPP_SUCCESS;
     return logmel_launch(h, p, precision, srcPtr, 0, srcOffsets, srcLengthTensor, batchSize, dstPtr, 0, 0,
-                         dstOffsets);
+                         dstOffsets, 0, 0);
 }

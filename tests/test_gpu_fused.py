@@ -156,8 +156,8 @@ def test_fused_status_codes(gpu, handle):
     out = torch.zeros((2, 80, 1), device="cuda")
     p = gpu.logmel_params(multiplier=0.0)
     assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -4
-    p = gpu.logmel_params(reference_magnitude=0.0)
-    assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -6
+    p = gpu.logmel_params(reference_magnitude=-1.0)
+    assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -2
     p = gpu.logmel_params(max_freq=9000.0)
     assert gpu.logmel_fused(handle, x, out, L, p, "bf16x3", check_status=False) == -2
 
@@ -210,3 +210,61 @@ def test_fused_mfma_shapes(gpu, handle, orc, nfft, W, H, nmel, sr):
     err = max_db_err(out.cpu().numpy(), ref, fr)
     print(f"fp16x3 nfft {nfft} W {W} H {H} M {nmel}: max |dB err| vs FP64 = {err:.3e}")
     assert err <= DB_TOL_MIXED
+
+
+def test_fused_clip_max_reference(gpu, handle, orc):
+    """referenceMagnitude 0 takes each clip's maximum mel power as the dB reference
+    (cpu/kernel/to_decibels.cpp:58-75): FP32 mode equals the chain with the same setting bit for
+    bit on every multi-frame clip and meets the FP32 bar against the oracle on all of them; fp16x3
+    meets 0.05 dB against the FP64 oracle.  The chain reads a one-frame clip's [numFilter x 1]
+    mel output as 1-D data (width 1, cpu/kernel/to_decibels.cpp:63, :77), the fused launch as the 2-D block it
+    is, like the oracle's log-mel chain, so that clip is held to the oracle only."""
+    bt = synth.workload1(batch=20, max_seconds=5.0)
+    L = bt.lengths
+    L[3] = 1   # a one-frame clip
+    feat = bt.params
+    p = gpu.logmel_params(nfft=feat["nfft"], window_length=feat["W"], window_step=feat["H"],
+                          sample_rate=feat["sample_rate"], min_freq=feat["fmin"], max_freq=feat["fmax"],
+                          num_filter=feat["nmel"], reference_magnitude=0.0)
+    frames = L // feat["H"] + 1
+    db_chain, _ = gpu.logmel_chain(handle, dev(bt.x), dev(L), ref=0.0)
+    d = db_chain.cpu().numpy()
+    ref64, _, _ = orc.logmel_f64(bt.x, L, ref=0.0, nthreads=8)
+    ref32, _ = orc.logmel_f32(bt.x, L, ref=0.0, nthreads=8)
+    for precision in ("fp32", "fp16x3"):
+        out = torch.full((len(L), feat["nmel"], int(frames.max())), float("nan"), device="cuda")
+        gpu.logmel_fused(handle, dev(bt.x), out, dev(L), p, precision)
+        torch.cuda.synchronize()
+        o = out.cpu().numpy()
+        err = max_db_err(o, ref64, frames)
+        print(f"{precision} ref=clip max: max |dB err| vs FP64 = {err:.3e}")
+        if precision == "fp32":
+            for b, F in enumerate(frames):
+                if F > 1:
+                    assert np.array_equal(o[b, :, :F], d[b, :, :F]), b
+            assert fp32_bar(o, ref32, frames) <= 1.0
+        else:
+            assert err <= DB_TOL_MIXED
+        assert np.nanmax(o) <= 0.0   # every clip's loudest output is at its own reference
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
+def test_fused_ntf_layout(gpu, handle, precision):
+    """NTF output (frames as rows) holds exactly the NFT output transposed, also with a row stride
+    above numFilter and fewer rows than the longest clip's frames (those frames are not written)."""
+    bt = synth.workload1(batch=16, max_seconds=4.0)
+    L = bt.lengths
+    feat = bt.params
+    M = feat["nmel"]
+    p = params_for(gpu, feat)
+    frames = L // feat["H"] + 1
+    cap = int(frames.max()) - 7
+    nft = torch.full((len(L), M, cap), float("nan"), device="cuda")
+    ntf = torch.full((len(L), cap, M + 3), float("nan"), device="cuda")
+    gpu.logmel_fused(handle, dev(bt.x), nft, dev(L), p, precision)
+    gpu.logmel_fused(handle, dev(bt.x), ntf, dev(L), p, precision, layout=gpu.NTF)
+    torch.cuda.synchronize()
+    a, t = nft.cpu().numpy(), ntf.cpu().numpy()
+    for b, F in enumerate(np.minimum(frames, cap)):
+        assert np.array_equal(t[b, :F, :M], a[b, :, :F].T), b
+        assert np.isnan(t[b, F:, :]).all() and np.isnan(t[b, :, M:]).all(), b
