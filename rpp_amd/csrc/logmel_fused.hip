@@ -730,6 +730,9 @@ __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pw
 // Mel work list of one wave: entries (16-filter x 16-frame tile, 32-bin step of its band), packed
 // as tile << 8 | step; every tile has at least one step.  Capacity per wave:
 constexpr int kMelSeq = 320;
+// Mel weight fragments in flight per wave (work-list entries loaded ahead).
+constexpr int kWA = 2;
+static_assert(kWA >= 2, "the tile-end test reads entry e + 1 from the next slot");
 
 // Fused MFMA kernel.  Block = 4 waves; wave w owns CT 32-bin column tiles (bins w*CT*32 .. +CT*32,
 // each a cos and a sin tile of the folded basis), so 4*CT*32 = N/2, for all FT = 32*RT frames of a
@@ -737,7 +740,7 @@ constexpr int kMelSeq = 320;
 // products.  Per tile: stage the pre-emphasised segment; per 16-row step of the folded
 // contraction, the products of step s run from one buffer of the operand ring while the block
 // builds step s+1 into the other (one barrier per step); power in lane into bf16 hi/lo planes;
-// banded mel on MFMA with the weights loaded two steps ahead; dB out.
+// banded mel on MFMA, each weight fragment applied to every frame group of the tile; dB out.
 template <int PREC, int FT, int CT>
 __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
 {
@@ -749,6 +752,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     __shared__ float nyqPart[NW * 64];
     __shared__ uint16_t melSeq[NW][kMelSeq];
     __shared__ int melSeqLen[NW];
+    __shared__ uint16_t melBand[16];                     // filter tile's band: first step << 8 | end step
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
     u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2 kSPB][FT] operand slices
     const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
@@ -757,29 +761,45 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     const int halfN = a.N / 2;
     const int nSteps = a.Wp / 16;
     const float invH = 1.0f / static_cast<float>(a.H);
-    const int nMT = a.Mpad / 16, nMelTiles = nMT * (FT / 16);
+    const int nMT = a.Mpad / 16;
     float *wH = lds + wOff;                              // window half (the Nyquist sin term), past the aliased region
     for (int j = threadIdx.x; j < a.W / 2; j += blockDim.x) wH[j] = a.wHalf[j];
 
-    // each wave's mel work list (the same for every tile): bands of the filter tiles it owns
-    if ((threadIdx.x & 63) == 0) {
-        const int w = threadIdx.x >> 6;
-        int n = 0;
-        for (int tt = w; tt < nMelTiles; tt += NW) {
-            const int mt = tt % nMT;
-            int kb = a.KP, ke = 0;
-            for (int i = 0; i < 16; i++) {
-                const int m = mt * 16 + i;
-                if (m < a.M) {
-                    kb = min(kb, a.filterBins[2 * m]);
-                    ke = max(ke, a.filterBins[2 * m + 1]);
-                }
+    // mel work lists (the same for every tile): whole 16-filter tiles, largest band first, each to
+    // the wave with the fewest band steps so far; an entry (filter tile, 32-bin step of its band)
+    // is applied to every 16-frame group of the tile, so one weight fragment serves FT/16 products
+    if (threadIdx.x < nMT) {
+        const int mt = threadIdx.x;
+        int kb = a.KP, ke = 0;
+        for (int i = 0; i < 16; i++) {
+            const int m = mt * 16 + i;
+            if (m < a.M) {
+                kb = min(kb, a.filterBins[2 * m]);
+                ke = max(ke, a.filterBins[2 * m + 1]);
             }
-            kb = kb >= ke ? 0 : kb / 32 * 32;        // an empty band still takes one (all-zero) step
-            ke = max(min(a.KP, (ke + 31) / 32 * 32), kb + 32);
-            for (int k = kb; k < ke && n < kMelSeq; k += 32) melSeq[w][n++] = static_cast<uint16_t>((tt << 8) | (k / 32));
         }
-        melSeqLen[w] = n;
+        kb = kb >= ke ? 0 : kb / 32 * 32;        // an empty band still takes one (all-zero) step
+        ke = max(min(a.KP, (ke + 31) / 32 * 32), kb + 32);
+        melBand[mt] = static_cast<uint16_t>((kb / 32) << 8 | (ke / 32));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int load[NW] = {0, 0, 0, 0}, cnt[NW] = {0, 0, 0, 0};
+        uint32_t taken = 0;
+        for (int it = 0; it < nMT; it++) {
+            int best = -1, bestSteps = -1;
+            for (int mt = 0; mt < nMT; mt++) {
+                const int steps = (melBand[mt] & 255) - (melBand[mt] >> 8);
+                if (!(taken >> mt & 1) && steps > bestSteps) { best = mt; bestSteps = steps; }
+            }
+            taken |= 1u << best;
+            int w = 0;
+            for (int q = 1; q < NW; q++) if (load[q] < load[w]) w = q;
+            for (int k = melBand[best] >> 8; k < (melBand[best] & 255); k++)
+                melSeq[w][cnt[w]++] = static_cast<uint16_t>((best << 8) | k);
+            load[w] += bestSteps;
+        }
+        for (int q = 0; q < NW; q++) melSeqLen[q] = cnt[q];
     }
 
     // the first tile's segment is fetched here; every later one while its predecessor's mel runs
@@ -896,6 +916,27 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         }
         __syncthreads();
 
+        // ---- mel work list of this wave; the weight fragments of its first kWA entries are
+        //      requested before the next tile's samples.  Slot u of the ring holds entry e with
+        //      e % kWA == u; every refill is unconditional (an index past the end reloads the last
+        //      entry, never used), so each slot stays in one set of registers and a product waits
+        //      only for its own fragments.
+        const int n = melSeqLen[wave];
+        const uint16_t *seq = melSeq[wave];
+        u16x8 wh[kWA], wl[kWA];
+        int ent[kWA];
+        auto loadW = [&](int slot, int e) {
+            const int en = seq[min(e, n - 1)], mt = en >> 8, k0 = (en & 255) * 32;
+            ent[slot] = en;
+            const size_t o = (static_cast<size_t>(k0 / 8 + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
+            wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
+            wl[slot] = *reinterpret_cast<const u16x8 *>(a.mLo + o);
+        };
+        if (n > 0) {
+#pragma unroll
+            for (int u = 0; u < kWA; u++) loadW(u, u);
+        }
+
         // ---- the next tile's samples load while the mel product runs (the accumulators are dead)
         {
             const int nt = tile + gridDim.x;
@@ -908,56 +949,57 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         }
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
-        //      16-filter tile's band of bins; the wave walks its work list with the weight
-        //      fragments loaded two entries ahead
-        {
-            const int n = melSeqLen[wave];
-            const uint16_t *seq = melSeq[wave];
-            u16x8 wh[2], wl[2];
-            auto loadW = [&](int slot, int e) {
-                const int ent = seq[e], tt = ent >> 8, k0 = (ent & 255) * 32;
-                const size_t o = (static_cast<size_t>(k0 / 8 + (lane >> 4)) * a.Mpad + (tt % nMT) * 16 + (lane & 15)) * 8;
-                wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
-                wl[slot] = *reinterpret_cast<const u16x8 *>(a.mLo + o);
-            };
-            if (n > 0) loadW(0, 0);
-            if (n > 1) loadW(1, 1);
+        //      16-filter tile's band of bins, for the FT/16 frame groups at once
+        if (n > 0) {
+            constexpr int FG = FT / 16;
             float melMax = -3.40282347e38f;
-            // two partial sums (even and odd entries) keep consecutive products independent
-            f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int e0 = 0; e0 < n; e0 += 2) {
+            f32x4 acc[FG];
 #pragma unroll
-                for (int u = 0; u < 2; u++) {
+            for (int g = 0; g < FG; g++) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int e0 = 0; e0 < n; e0 += kWA) {
+#pragma unroll
+                for (int u = 0; u < kWA; u++) {
                     const int e = e0 + u;
                     if (e < n) {
-                        const int ent = seq[e], tt = ent >> 8, k0 = (ent & 255) * 32;
+                        const int mt = ent[u] >> 8, k0 = (ent[u] & 255) * 32;
                         const bf16x8 w0 = __builtin_bit_cast(bf16x8, wh[u]), w1 = __builtin_bit_cast(bf16x8, wl[u]);
-                        if (e + 2 < n) loadW(u, e + 2);
-                        const int mt = tt % nMT, fr = (tt / nMT) * 16 + (lane & 15);
-                        const int q = fr * pws2 + k0 + 8 * (lane >> 4);
-                        const bf16x8 ph = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q]));
-                        const bf16x8 pl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q]));
-                        f32x4 &c = u ? c1 : c0;
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph, c, 0, 0, 0);
-                        if (e + 1 == n || (seq[e + 1] >> 8) != tt) {
-                            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame fr
-                            if (f0 + fr < v.F) {
+                        const int q0 = (lane & 15) * pws2 + k0 + 8 * (lane >> 4);
+                        bf16x8 ph[FG], pl[FG];
 #pragma unroll
-                                for (int r = 0; r < 4; r++) {
-                                    const int m = mt * 16 + 4 * (lane >> 4) + r;
-                                    if (m < a.M) {
-                                        const float pm = c0[r] + c1[r];
-                                        if (a.clipMax) melMax = fmaxf(melMax, pm);
-                                        *out_at(a, v, m, f0 + fr) = a.clipMax ? pm : db_fast(pm, a);
+                        for (int g = 0; g < FG; g++) {
+                            ph[g] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q0 + g * 16 * pws2]));
+                            pl[g] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q0 + g * 16 * pws2]));
+                        }
+#pragma unroll
+                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph[g], acc[g], 0, 0, 0);
+#pragma unroll
+                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl[g], acc[g], 0, 0, 0);
+#pragma unroll
+                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph[g], acc[g], 0, 0, 0);
+                        // slot (u + 1) % kWA already holds entry e + 1
+                        if (e + 1 == n || (ent[(u + 1) % kWA] >> 8) != mt) {
+                            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame g*16 + lane%16
+                            const int m0 = mt * 16 + 4 * (lane >> 4);
+                            const size_t rStride = a.ntf ? 1 : v.hs;   // between consecutive rows m
+#pragma unroll
+                            for (int g = 0; g < FG; g++) {
+                                const int fr = g * 16 + (lane & 15);
+                                if (f0 + fr < v.F) {
+                                    float *o = out_at(a, v, m0, f0 + fr);
+#pragma unroll
+                                    for (int r = 0; r < 4; r++) {
+                                        if (m0 + r < a.M) {
+                                            const float pm = acc[g][r];
+                                            if (a.clipMax) melMax = fmaxf(melMax, pm);
+                                            o[r * rStride] = a.clipMax ? pm : db_fast(pm, a);
+                                        }
                                     }
                                 }
+                                acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                             }
-                            c0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                            c1 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                         }
                     }
+                    loadW(u, e + kWA);
                 }
             }
             if (a.clipMax) clip_max_update(a, v.b, melMax);
@@ -1076,7 +1118,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int span = (FT - 1) * a.H + a.W;
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
-        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + 4) + sizeof(uint16_t) * 4 * kMelSeq;
+        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + 4) + sizeof(uint16_t) * (4 * kMelSeq + 16);
         const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * kSPB * FT, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
         const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);
