@@ -83,6 +83,17 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
+// Maximum over the wave, with the exchange addresses formed from a fresh opaque lane index (the
+// plain shuffles' addresses are loop-invariant and would be hoisted into live registers).
+__device__ __forceinline__ float wave_max(float v)
+{
+    const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ o) << 2, __float_as_int(v))));
+    return v;
+}
+
 // seg_pos for 0 <= i < 2^20 without an integer division: with i = qH + r, (i + 0.5)/H lies at least
 // 0.5/H away from an integer, far beyond the FP32 rounding of the product.
 __device__ __forceinline__ int seg_pos_r(int i, float invH)
@@ -131,7 +142,7 @@ __device__ __forceinline__ float *out_at(const LogMelArgs &a, const ClipView &v,
 // Folds a thread's running maximum of stored mel power into the clip's (whole wave active).
 __device__ __forceinline__ void clip_max_update(const LogMelArgs &a, int b, float mx)
 {
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    mx = wave_max(mx);
     if ((threadIdx.x & 63) == 0) atomicMax(&a.clipMax[b], __float_as_int(mx));
 }
 
@@ -680,6 +691,70 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
     }
 }
 
+// The same work as prep_step for one unit u of the 2*kSPB*2*FT units of a step pair, split so
+// the loads can be issued ahead of the products they overlap: prep_load reads the segment and
+// window (rows past W/2 read the last real rows and are zeroed in prep_finish), prep_finish
+// forms, splits and stores.
+struct PrepIn {
+    float4 up, dn, w;
+    bool live;
+};
+
+template <int FT>
+__device__ __forceinline__ void prep_load(PrepIn &p, const float *seg, const float *wHalf, const LogMelArgs &a, int s,
+                                          int u, float invH)
+{
+    const int f = u % FT, qd = u / FT;
+    const int half = a.W / 2;
+    const int j = 16 * s + 4 * qd;
+    p.live = j < half;
+    const int jc = p.live ? j : half - 4;
+    const int fb = f * (a.H + kSkew);
+    p.up = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half + jc, invH)]);
+    p.dn = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half - 4 - jc, invH)]);
+    p.w = *reinterpret_cast<const float4 *>(&wHalf[jc]);
+}
+
+template <int PREC, int FT>
+__device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u, int buf, float segScale, float &nyq)
+{
+    constexpr int GB = 2 * kSPB;
+    const int f = u % FT, qd = u / FT;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 up = p.live ? p.up : z, dn = p.live ? p.dn : z, w = p.w;   // selects, not a branch
+    const float e0 = up.x + dn.w, o0 = up.x - dn.w;
+    const float e1 = up.y + dn.z, o1 = up.y - dn.z;
+    const float e2 = up.z + dn.y, o2 = up.z - dn.y;
+    const float e3 = up.w + dn.x, o3 = up.w - dn.x;
+    nyq = fmaf(o0, w.x, nyq);                  // (-1)^row; a zero o leaves nyq unchanged
+    nyq = fmaf(o1, -w.y, nyq);
+    nyq = fmaf(o2, w.z, nyq);
+    nyq = fmaf(o3, -w.w, nyq);
+    uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
+    split2<PREC>(e0, e1, segScale, eh0, el0);
+    split2<PREC>(e2, e3, segScale, eh1, el1);
+    split2<PREC>(o0, o1, segScale, oh0, ol0);
+    split2<PREC>(o2, o3, segScale, oh1, ol1);
+    uint2 *dst = reinterpret_cast<uint2 *>(ring);
+    const int slot = (buf * 4 * GB + (qd >> 1)) * FT + f, hh = qd & 1;
+    dst[(slot + 0 * GB * FT) * 2 + hh] = make_uint2(eh0, eh1);
+    dst[(slot + 1 * GB * FT) * 2 + hh] = make_uint2(el0, el1);
+    dst[(slot + 2 * GB * FT) * 2 + hh] = make_uint2(oh0, oh1);
+    dst[(slot + 3 * GB * FT) * 2 + hh] = make_uint2(ol0, ol1);
+}
+
+// Scheduling hint for the region before it: NMFMA products, each followed by up to three vector
+// ALU instructions (the operand build of the next step pair runs while the products execute).
+template <int NMFMA>
+__device__ __forceinline__ void interleave_valu()
+{
+#pragma unroll
+    for (int i = 0; i < NMFMA; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);   // VALU
+    }
+}
+
 // bf16 hi+lo split of one power value into the two planes the mel product reads (same rounding
 // as split_bf16).
 __device__ __forceinline__ void put_power(uint16_t *pwH, uint16_t *pwL, int idx, float p)
@@ -812,14 +887,15 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         seg_prefetch(pf, cur, span, tid);
     }
     for (; tile < T; tile += gridDim.x) {
-        const int tid = opaque(threadIdx.x), wave = tid >> 6, lane = tid & 63;
-        const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
+        // thread-index values are re-derived from a fresh opaque copy in every phase, so none of
+        // them is computed once and kept live across the others
+        int tid = opaque(threadIdx.x), wave = tid >> 6, lane = tid & 63;
         const ClipView v = cur.v;
         const int f0 = cur.f0;
 
         // ---- stage the pre-emphasised segment; block max for the fp16 scale
         float mx = seg_store(seg, pf, a, cur, span, invH, tid);
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        mx = wave_max(mx);
         if (lane == 0) red[wave] = mx;
         __syncthreads();
         mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
@@ -827,6 +903,8 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));  // 2*max*scale < 2^11
 
         // ---- folded DFT
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
+        const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
         f32x16 accC[RT][CT], accS[RT][CT];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
@@ -855,46 +933,60 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) loadB(0, ct, 0);
         __syncthreads();
-        const int nBlk = (nSteps + kSPB - 1) / kSPB;
-        for (int sb = 0; sb < nBlk; sb++) {
-            const int rb = sb & 1;                      // operand ring buffer of these steps
+        // one step's products from ring buffer rb, pair half h, basis buffer buf; the basis
+        // refills of step sn (always a real step here) follow each column tile's products
+        auto step_products = [&](int rb, int h, int buf, int sn, bool refill) {
+            __builtin_amdgcn_sched_barrier(0);
+            u16x8 ah[RT][2], oh[RT][2];
 #pragma unroll
-            for (int h = 0; h < kSPB; h++) {
-                const int s = sb * kSPB + h;
-                const int buf = NB == 2 ? (h & 1) : 0;   // basis buffer holding step s (kSPB is even)
-                if (s < nSteps) {
-                    if (NB == 2 && s + 1 < nSteps) loadB(buf ^ 1, 0, s + 1);
-                    u16x8 ah[RT][2], oh[RT][2];
-#pragma unroll
-                    for (int rt = 0; rt < RT; rt++) {
-                        const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + rt * 32 + (lane & 31);
-                        ah[rt][0] = ring[slot + 0 * GB * FT];
-                        ah[rt][1] = ring[slot + 1 * GB * FT];
-                        oh[rt][0] = ring[slot + 2 * GB * FT];
-                        oh[rt][1] = ring[slot + 3 * GB * FT];
-                    }
-                    // keep the LDS reads together ahead of the products (left alone, the scheduler
-                    // serialises each read with its first product)
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int ct = 0; ct < CT; ct++) {
-#pragma unroll
-                        for (int rt = 0; rt < RT; rt++) {
-                            // X^T = basis (A: rows = bins) x frames (B: columns = frames)
-                            mfma3<PREC>(accC[rt][ct], bc[buf][ct][0], bc[buf][ct][1], ah[rt][0], ah[rt][1]);
-                            mfma3<PREC>(accS[rt][ct], bs[buf][ct][0], bs[buf][ct][1], oh[rt][0], oh[rt][1]);
-                        }
-                        if (NB == 1 && s + 1 < nSteps) loadB(0, ct, s + 1);
-                    }
-                }
+            for (int rt = 0; rt < RT; rt++) {
+                const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + rt * 32 + (lane & 31);
+                ah[rt][0] = ring[slot + 0 * GB * FT];
+                ah[rt][1] = ring[slot + 1 * GB * FT];
+                oh[rt][0] = ring[slot + 2 * GB * FT];
+                oh[rt][1] = ring[slot + 3 * GB * FT];
             }
-            if (sb + 1 < nBlk) prep_step<PREC, FT>(ring, seg, wH, a, (sb + 1) * kSPB, rb ^ 1, segScale, invH, nyq, tid);
-            __syncthreads();                            // next steps built; buffer rb free again
+            // keep the LDS reads together ahead of the products (left alone, the scheduler
+            // serialises each read with its first product)
+            __builtin_amdgcn_sched_barrier(0);
+            if (NB == 2 && refill) loadB(buf ^ 1, 0, sn);
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    // X^T = basis (A: rows = bins) x frames (B: columns = frames)
+                    mfma3<PREC>(accC[rt][ct], bc[buf][ct][0], bc[buf][ct][1], ah[rt][0], ah[rt][1]);
+                    mfma3<PREC>(accS[rt][ct], bs[buf][ct][0], bs[buf][ct][1], oh[rt][0], oh[rt][1]);
+                }
+                if (NB == 1 && refill) loadB(0, ct, sn);
+            }
+        };
+        constexpr int UPT = 2 * GB * FT / 256;            // operand units per thread and step pair
+        static_assert(kSPB == 2, "the loop below runs step pairs");
+        // full pairs: straight-line, the next pair's operand build beside the products (unit i of
+        // it loaded before step s0 + i and finished after it); the last pair's build and refills
+        // past the end are harmless (zero rows into the idle buffer, a clamped refill)
+        const int nPairs = nSteps / 2;
+        for (int sb = 0; sb < nPairs; sb++) {
+            const int rb = sb & 1;
+            const int s0 = 2 * sb;
+            PrepIn pin;
+            prep_load<FT>(pin, seg, wH, a, s0 + 2, tid, invH);
+            step_products(rb, 0, 0, s0 + 1, true);
+            prep_finish<PREC, FT>(ring, pin, tid, rb ^ 1, segScale, nyq);
+            if (UPT == 2) prep_load<FT>(pin, seg, wH, a, s0 + 2, tid + 256, invH);
+            interleave_valu<(PREC == kPrecBF16 ? 2 : 6) * RT * CT>();
+            step_products(rb, 1, NB == 2 ? 1 : 0, min(s0 + 2, nSteps - 1), true);
+            if (UPT == 2) prep_finish<PREC, FT>(ring, pin, tid + 256, rb ^ 1, segScale, nyq);
+            interleave_valu<(PREC == kPrecBF16 ? 2 : 6) * RT * CT>();
+            __syncthreads();                            // next pair built; buffer rb free again
         }
+        if (nSteps & 1) step_products(nPairs & 1, 0, 0, 0, false);   // odd last step
         nyqPart[tid] = nyq;
         __syncthreads();  // segment and operand ring are dead: the power planes may overwrite them
 
         // ---- power = cos^2 + sin^2 in lane, bin N/2 from the Nyquist partials, stored as bf16 hi and
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         //      lo planes [FT][pws2] for the mel product (each value split once)
         uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
         uint16_t *pwL = pwH + FT * pws2;
@@ -917,6 +1009,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         __syncthreads();
 
         // ---- mel work list of this wave; the weight fragments of its first kWA entries are
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         //      requested before the next tile's samples.  Slot u of the ring holds entry e with
         //      e % kWA == u; every refill is unconditional (an index past the end reloads the last
         //      entry, never used), so each slot stays in one set of registers and a product waits
@@ -938,6 +1031,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         }
 
         // ---- the next tile's samples load while the mel product runs (the accumulators are dead)
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         {
             const int nt = tile + gridDim.x;
             if (nt < T) {
@@ -949,6 +1043,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         }
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         //      16-filter tile's band of bins, for the FT/16 frame groups at once
         if (n > 0) {
             constexpr int FG = FT / 16;
