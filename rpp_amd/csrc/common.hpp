@@ -41,12 +41,14 @@ constexpr int kWave = 64;
 struct DftTables {
     int nfft = 0, W = 0, cols = 0;
     float *basis = nullptr;        // [W][cols] FP32
-    // Folded split basis for the MFMA path (even nfft and W, symmetric window).  With the frame
-    // centre c = (W-1)/2, e_j = a[c+0.5+j] + a[c-0.5-j] and o_j = a[c+0.5+j] - a[c-0.5-j]
+    // Folded split basis for the MFMA path (nfft a multiple of 4, even W, symmetric window).  With
+    // the frame centre c = (W-1)/2, e_j = a[c+0.5+j] + a[c-0.5-j] and o_j = a[c+0.5+j] - a[c-0.5-j]
     // (j < W/2) give X_k = sum_j e_j w_j cos(2 pi k (j+0.5)/N) - i sum_j o_j w_j sin(...): half
-    // the contraction length.  Columns: [0, N/2) cos of bins 0..N/2-1, [N/2, N) sin of the same
-    // bins; bin N/2 (cos term zero) is formed separately from wHalf.  Element (j, c) sits at
-    // ((j / 8) * N + c) * 8 + j % 8 -- the operand order of a 32x32x16 MFMA B fragment.
+    // the contraction length.  Only bins k < N/4 are stored: bin N/2 - k has
+    // cos = (-1)^j sin_k and sin = (-1)^j cos_k at row j, so the same columns with the odd rows
+    // negated give it; bin N/4 is formed separately from wHalf.  Columns: [0, N/4) w cos of bins
+    // 0..N/4-1, [N/4, N/2) -w sin of the same bins.  Element (j, c) sits at
+    // ((j / 8) * (N/2) + c) * 8 + j % 8 -- the operand order of a 32x32x16 MFMA fragment.
     uint16_t *foldHi = nullptr, *foldLo = nullptr;
     float *wHalf = nullptr;        // [W/2] symmetrised window half, w_j = w[W/2 + j]
     int Jp = 0;                    // W/2 padded to a multiple of 16 (zero rows)
@@ -62,9 +64,13 @@ struct MelTables {
     float *normFactors = nullptr;   // [M]
     int *filterBins = nullptr;      // [M][2] first bin / one-past-last bin of filter m (gather form)
     float *dense = nullptr;         // [M][numBins] dense normalised weights (FP32)
-    // bf16 hi/lo split of `dense` in MFMA operand order: element (m, k) at ((k / 8) * Mpad + m) * 8 + k % 8
+    // bf16 hi/lo split of `dense` in MFMA operand order over power-plane positions p: element
+    // (m, p) at ((p / 8) * Mpad + m) * 8 + p % 8.  Position p holds bin p for p <= Q and bin p - 3
+    // for p >= Q + 4 (Q = (numBins - 1) / 2, i.e. N/4), positions Q+1..Q+3 hold nothing: the fused
+    // kernel stores the upper half of the spectrum three places up so its 4-bin groups stay aligned.
     uint16_t *denseHi = nullptr, *denseLo = nullptr;
-    int KP = 0;                     // numBins padded to a multiple of 32
+    int KP = 0;                     // numBins padded to a multiple of 32 (holds every position p above
+                                    // when numBins - 1 is a multiple of 128)
     int Mpad = 0;                   // M padded to a multiple of 16
 };
 

@@ -31,6 +31,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 constexpr int kSkew = 4;
@@ -649,20 +650,36 @@ __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi
 // Folded, split DFT operand of SPB 16-row contraction steps starting at step s, built into buffer
 // `buf` of the slice ring while the waves run the previous steps' products: slot
 // ((buf * 4 + arr) * 2SPB + g) * FT + f holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7,
-// for arr = 0 e-hi, 1 e-lo, 2 o-hi, 3 o-lo -- the lane order of a 32x32x16 MFMA A fragment.  A work
-// unit forms four rows of one frame (one 8-byte half of a slot); `nyq` gathers the thread's share
-// of bin N/2 (sin term only).
+// for arr = 0 e-hi, 1 e-lo, 2 o-hi, 3 o-lo -- the lane order of a 32x32x16 MFMA fragment.  A work
+// unit forms four rows of one frame (one 8-byte half of a slot).  q4 gathers the thread's share of
+// bin N/4: re += e_j w_j c_j, im += o_j w_j s_j with (c_j, s_j) = sqrt(2) (cos, sin)(pi (2j+1)/4),
+// i.e. c = +1 -1 -1 +1 and s = +1 +1 -1 -1 over j mod 4 (the sqrt(2)/2 factors are applied to the
+// power).
 constexpr int kSPB = 2;   // contraction steps per barrier
+
+__device__ __forceinline__ void q4_accumulate(float2 &q4, float e0, float e1, float e2, float e3, float o0, float o1,
+                                              float o2, float o3, float4 w)
+{
+    q4.x = fmaf(e0, w.x, q4.x);
+    q4.x = fmaf(e1, -w.y, q4.x);
+    q4.x = fmaf(e2, -w.z, q4.x);
+    q4.x = fmaf(e3, w.w, q4.x);
+    q4.y = fmaf(o0, w.x, q4.y);
+    q4.y = fmaf(o1, w.y, q4.y);
+    q4.y = fmaf(o2, -w.z, q4.y);
+    q4.y = fmaf(o3, -w.w, q4.y);
+}
+
 template <int PREC, int FT>
 __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const float *wHalf, const LogMelArgs &a,
-                                          int s, int buf, float segScale, float invH, float &nyq, int tid)
+                                          int s, int buf, float segScale, float invH, float2 &q4, int tid)
 {
     constexpr int GB = 2 * kSPB;                         // 8-row groups per buffer
     const int half = a.W / 2;
     uint2 *dst = reinterpret_cast<uint2 *>(ring);
     for (int u = tid; u < 2 * GB * FT; u += blockDim.x) {
         const int f = u % FT, qd = u / FT;
-        const int j = 16 * s + 4 * qd;                 // rows j .. j+3 (j even)
+        const int j = 16 * s + 4 * qd;                 // rows j .. j+3 (j a multiple of 4)
         float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
         if (j < half) {
             const int fb = f * (a.H + kSkew);           // seg_pos(f*H + t) = f*(H + skew) + seg_pos(t)
@@ -672,11 +689,7 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
             e1 = up.y + dn.z; o1 = up.y - dn.z;
             e2 = up.z + dn.y; o2 = up.z - dn.y;
             e3 = up.w + dn.x; o3 = up.w - dn.x;
-            const float4 w = *reinterpret_cast<const float4 *>(&wHalf[j]);
-            nyq = fmaf(o0, w.x, nyq);                  // (-1)^row
-            nyq = fmaf(o1, -w.y, nyq);
-            nyq = fmaf(o2, w.z, nyq);
-            nyq = fmaf(o3, -w.w, nyq);
+            q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, *reinterpret_cast<const float4 *>(&wHalf[j]));
         }
         uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
         split2<PREC>(e0, e1, segScale, eh0, el0);
@@ -716,7 +729,7 @@ __device__ __forceinline__ void prep_load(PrepIn &p, const float *seg, const flo
 }
 
 template <int PREC, int FT>
-__device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u, int buf, float segScale, float &nyq)
+__device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u, int buf, float segScale, float2 &q4)
 {
     constexpr int GB = 2 * kSPB;
     const int f = u % FT, qd = u / FT;
@@ -726,10 +739,7 @@ __device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u,
     const float e1 = up.y + dn.z, o1 = up.y - dn.z;
     const float e2 = up.z + dn.y, o2 = up.z - dn.y;
     const float e3 = up.w + dn.x, o3 = up.w - dn.x;
-    nyq = fmaf(o0, w.x, nyq);                  // (-1)^row; a zero o leaves nyq unchanged
-    nyq = fmaf(o1, -w.y, nyq);
-    nyq = fmaf(o2, w.z, nyq);
-    nyq = fmaf(o3, -w.w, nyq);
+    q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, w);   // zero e and o leave q4 unchanged
     uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
     split2<PREC>(e0, e1, segScale, eh0, el0);
     split2<PREC>(e2, e3, segScale, eh1, el1);
@@ -765,38 +775,60 @@ __device__ __forceinline__ void put_power(uint16_t *pwH, uint16_t *pwL, int idx,
     pwL[idx] = __builtin_bit_cast(uint16_t, l);
 }
 
-// power = cos^2 + sin^2 of the lane's accumulators (|X| when SQRT), split into the planes.
-// The DFT accumulators hold X^T: row = bin, column = frame, so a lane owns frame rt*32 + lane%32
-// and, per group of four registers, four consecutive bins.  power = cos^2 + sin^2 (|X| when SQRT)
-// of those four bins, split into bf16 hi and lo, goes out as one 8-byte store per plane.
-template <bool SQRT, int RT, int CT>
-__device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&accC)[RT][CT],
-                                            const f32x16 (&accS)[RT][CT], float unscale, int binTile0, int lane)
+// A basis fragment with its odd rows negated (rows j..j+7 of one lane: the odd j are the high
+// halves of its four dwords), the operand of the mirror bins N/2 - k.
+__device__ __forceinline__ u16x8 flip_odd(const u16x8 v)
+{
+    return __builtin_bit_cast(u16x8, __builtin_bit_cast(u32x4, v) ^ 0x80000000u);
+}
+
+// bf16 hi and lo of four power values as two 8-byte plane stores.
+__device__ __forceinline__ void put_power4(uint16_t *pwH, uint16_t *pwL, int idx, const float (&p)[4])
+{
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const __bf16 hb = static_cast<__bf16>(p[k]);
+        const __bf16 lb = static_cast<__bf16>(p[k] - static_cast<float>(hb));
+        h[k] = __builtin_bit_cast(uint16_t, hb);
+        l[k] = __builtin_bit_cast(uint16_t, lb);
+    }
+    *reinterpret_cast<uint2 *>(&pwH[idx]) =
+        make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
+    *reinterpret_cast<uint2 *>(&pwL[idx]) =
+        make_uint2(l[0] | (static_cast<uint32_t>(l[1]) << 16), l[2] | (static_cast<uint32_t>(l[3]) << 16));
+}
+
+// Power (|X| when SQRT) of the wave's units into the planes.  The DFT accumulators hold X^T: row =
+// bin, column = frame, so a lane owns frame rt*32 + lane%32 and, per group of four registers,
+// direct bins k0 .. k0+3 (k0 = ct*32 + 8q + 4(lane/32)): acc 0/1 give bin k (cos x e, sin x o),
+// acc 2/3 bin N/2 - k (odd-negated sin x e, cos x o).  Bin k sits at plane position k, bin
+// N/2 - k at N/2 + 3 - k, so both groups are 8-byte aligned runs (the mirror run reversed).
+template <bool SQRT, int RTW, int CTM>
+__device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&acc)[RTW][CTM][4],
+                                            float unscale, int ct0, int rt0, int nct, int halfN, int lane)
 {
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) {
+    for (int r = 0; r < RTW; r++) {
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-            const int base = (rt * 32 + (lane & 31)) * pws2 + binTile0 + ct * 32 + 4 * (lane >> 5);
+        for (int ct = 0; ct < CTM; ct++) {
+            if (ct0 + ct >= nct) continue;   // a repeated tile (uneven split over the waves)
+            const int row = ((rt0 + r) * 32 + (lane & 31)) * pws2;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                uint16_t h[4], l[4];
+                const int k0 = (ct0 + ct) * 32 + 8 * q + 4 * (lane >> 5);
+                float pd[4], pm[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int r = 4 * q + k;
-                    const float re = accC[rt][ct][r] * unscale, im = accS[rt][ct][r] * unscale;
-                    float p = re * re + im * im;
-                    if (SQRT) p = sqrtf(p);
-                    const __bf16 hb = static_cast<__bf16>(p);
-                    const __bf16 lb = static_cast<__bf16>(p - static_cast<float>(hb));
-                    h[k] = __builtin_bit_cast(uint16_t, hb);
-                    l[k] = __builtin_bit_cast(uint16_t, lb);
+                for (int i = 0; i < 4; i++) {
+                    const float c = acc[r][ct][0][4 * q + i] * unscale, sn = acc[r][ct][1][4 * q + i] * unscale;
+                    const float mc = acc[r][ct][2][4 * q + i] * unscale, ms = acc[r][ct][3][4 * q + i] * unscale;
+                    float d = c * c + sn * sn, m = mc * mc + ms * ms;
+                    if (SQRT) { d = sqrtf(d); m = sqrtf(m); }
+                    pd[i] = d;
+                    pm[3 - i] = m;
                 }
-                const int idx = base + 8 * q;   // bins binTile0 + ct*32 + 8q + 4(lane/32) + 0..3
-                *reinterpret_cast<uint2 *>(&pwH[idx]) =
-                    make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
-                *reinterpret_cast<uint2 *>(&pwL[idx]) =
-                    make_uint2(l[0] | (static_cast<uint32_t>(l[1]) << 16), l[2] | (static_cast<uint32_t>(l[3]) << 16));
+                put_power4(pwH, pwL, row + k0, pd);
+                put_power4(pwH, pwL, row + halfN - k0, pm);
             }
         }
     }
@@ -809,22 +841,26 @@ constexpr int kMelSeq = 320;
 constexpr int kWA = 2;
 static_assert(kWA >= 2, "the tile-end test reads entry e + 1 from the next slot");
 
-// Fused MFMA kernel.  Block = 4 waves; wave w owns CT 32-bin column tiles (bins w*CT*32 .. +CT*32,
-// each a cos and a sin tile of the folded basis), so 4*CT*32 = N/2, for all FT = 32*RT frames of a
-// tile.  Two blocks share a CU, so one block's staging and power/mel phases overlap the other's
-// products.  Per tile: stage the pre-emphasised segment; per 16-row step of the folded
-// contraction, the products of step s run from one buffer of the operand ring while the block
-// builds step s+1 into the other (one barrier per step); power in lane into bf16 hi/lo planes;
-// banded mel on MFMA, each weight fragment applied to every frame group of the tile; dB out.
-template <int PREC, int FT, int CT>
+// Fused MFMA kernel.  Block = 4 waves.  The DFT of a tile (FT frames = NRT row tiles of 32) runs
+// over N/128 column tiles of 32 direct bins k < N/4, each giving bins k and N/2 - k; the
+// (column tile, row tile) units are split evenly over the waves, CTM x RTW per wave, four 32x32
+// accumulator tiles per unit.  Two blocks share a CU, so one block's staging and power/mel phases
+// overlap the other's products.  Per tile: stage the pre-emphasised segment; per pair of 16-row
+// steps of the folded contraction, the products run from one buffer of the operand ring while
+// the block builds the next pair into the other (one barrier per pair); power in lane into bf16
+// hi/lo planes; banded mel on MFMA, each weight fragment applied to every frame group of the
+// tile; dB out.
+template <int PREC, int FT, int CTM, int RTW>
 __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
 {
-    constexpr int RT = FT / 32;
+    constexpr int NRT = FT / 32;
+    constexpr int U = CTM * RTW;                         // units per wave
     constexpr int NW = 4;
-    static_assert(RT * CT <= 4, "accumulator budget: eight 32x32 tiles per wave");
+    static_assert(U <= 2, "accumulator budget: eight 32x32 tiles per wave");
+    static_assert(RTW <= NRT, "a wave's row tiles lie in one tile");
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[NW];
-    __shared__ float nyqPart[NW * 64];
+    __shared__ float2 q4Part[NW * 64];                  // bin N/4 partial sums (re, im)
     __shared__ uint16_t melSeq[NW][kMelSeq];
     __shared__ int melSeqLen[NW];
     __shared__ uint16_t melBand[16];                     // filter tile's band: first step << 8 | end step
@@ -837,7 +873,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     const int nSteps = a.Wp / 16;
     const float invH = 1.0f / static_cast<float>(a.H);
     const int nMT = a.Mpad / 16;
-    float *wH = lds + wOff;                              // window half (the Nyquist sin term), past the aliased region
+    float *wH = lds + wOff;                              // window half (bin N/4 sums), past the aliased region
     for (int j = threadIdx.x; j < a.W / 2; j += blockDim.x) wH[j] = a.wHalf[j];
 
     // mel work lists (the same for every tile): whole 16-filter tiles, largest band first, each to
@@ -852,6 +888,12 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                 kb = min(kb, a.filterBins[2 * m]);
                 ke = max(ke, a.filterBins[2 * m + 1]);
             }
+        }
+        // plane positions: bin b at b, or b + 3 above N/4
+        const int Q = a.N / 4;
+        if (kb < ke) {
+            kb = kb <= Q ? kb : kb + 3;
+            ke = (ke - 1 <= Q ? ke - 1 : ke + 2) + 1;
         }
         kb = kb >= ke ? 0 : kb / 32 * 32;        // an empty band still takes one (all-zero) step
         ke = max(min(a.KP, (ke + 31) / 32 * 32), kb + 32);
@@ -904,22 +946,28 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
 
         // ---- folded DFT
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
-        const int bin0 = wave * (CT * 32) + (lane & 31);    // cos column == bin; sin column = halfN + bin
-        f32x16 accC[RT][CT], accS[RT][CT];
+        const int nct = a.N / 128;                       // column tiles of direct bins
+        const int ct0 = wave * U / NRT, rt0 = wave * U % NRT;
+        f32x16 acc[RTW][CTM][4];
 #pragma unroll
-        for (int rt = 0; rt < RT; rt++)
+        for (int r = 0; r < RTW; r++)
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++)
+            for (int ct = 0; ct < CTM; ct++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) { accC[rt][ct][r] = 0.0f; accS[rt][ct][r] = 0.0f; }
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int i = 0; i < 16; i++) acc[r][ct][q][i] = 0.0f;
         // basis fragments [buffer][col tile][hi, lo]: one column tile per wave double-buffers whole
-        // steps; more refill each tile's fragments right after its own products (the other tiles'
-        // products cover the latency), which keeps the register budget
-        constexpr int NB = CT == 1 ? 2 : 1;
-        u16x8 bc[NB][CT][2], bs[NB][CT][2];
+        // steps; two refill each tile's fragments right after its own products (the other tile's
+        // products cover the latency), which keeps the register budget.  A tile index past the
+        // last (four waves over six tiles) repeats the last tile; its power is never stored.
+        constexpr int NB = CTM == 1 ? 2 : 1;
+        u16x8 bc[NB][CTM][2], bs[NB][CTM][2];
+        const int NC = a.N / 2;
         auto loadB = [&](int buf, int ct, int s) {
-            const size_t oc = (static_cast<size_t>(2 * s + (lane >> 5)) * a.N + bin0 + ct * 32) * 8;
-            const size_t os = oc + static_cast<size_t>(halfN) * 8;
+            const int c = min(ct0 + ct, nct - 1) * 32 + (lane & 31);
+            const size_t oc = (static_cast<size_t>(2 * s + (lane >> 5)) * NC + c) * 8;
+            const size_t os = oc + static_cast<size_t>(NC / 2) * 8;
             bc[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
             bs[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
             if (PREC != kPrecBF16) {
@@ -927,41 +975,51 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
                 bs[buf][ct][1] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
             }
         };
-        float nyq = 0.0f;
+        float2 q4 = make_float2(0.0f, 0.0f);
         constexpr int GB = 2 * kSPB;
-        prep_step<PREC, FT>(ring, seg, wH, a, 0, 0, segScale, invH, nyq, tid);
+        prep_step<PREC, FT>(ring, seg, wH, a, 0, 0, segScale, invH, q4, tid);
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) loadB(0, ct, 0);
+        for (int ct = 0; ct < CTM; ct++) loadB(0, ct, 0);
         __syncthreads();
         // one step's products from ring buffer rb, pair half h, basis buffer buf; the basis
-        // refills of step sn (always a real step here) follow each column tile's products
+        // refills of step sn follow each column tile's products
         auto step_products = [&](int rb, int h, int buf, int sn, bool refill) {
             __builtin_amdgcn_sched_barrier(0);
-            u16x8 ah[RT][2], oh[RT][2];
+            u16x8 ah[RTW][2], oh[RTW][2];
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + rt * 32 + (lane & 31);
-                ah[rt][0] = ring[slot + 0 * GB * FT];
-                ah[rt][1] = ring[slot + 1 * GB * FT];
-                oh[rt][0] = ring[slot + 2 * GB * FT];
-                oh[rt][1] = ring[slot + 3 * GB * FT];
+            for (int r = 0; r < RTW; r++) {
+                const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + (rt0 + r) * 32 + (lane & 31);
+                ah[r][0] = ring[slot + 0 * GB * FT];
+                ah[r][1] = ring[slot + 1 * GB * FT];
+                oh[r][0] = ring[slot + 2 * GB * FT];
+                oh[r][1] = ring[slot + 3 * GB * FT];
             }
-            // keep the LDS reads together ahead of the products (left alone, the scheduler
-            // serialises each read with its first product)
-            __builtin_amdgcn_sched_barrier(0);
+            // the other buffer's refill goes out here, a whole step ahead of its use
             if (NB == 2 && refill) loadB(buf ^ 1, 0, sn);
+            // keep the LDS reads (and that refill) ahead of the products (left alone, the
+            // scheduler serialises each read with its first product and sinks the refill)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
+            for (int ct = 0; ct < CTM; ct++) {
+                // X^T = basis (A: rows = bins) x frames (B: columns = frames); bin k from the
+                // columns as stored, bin N/2 - k from the same columns with odd rows negated
 #pragma unroll
-                for (int rt = 0; rt < RT; rt++) {
-                    // X^T = basis (A: rows = bins) x frames (B: columns = frames)
-                    mfma3<PREC>(accC[rt][ct], bc[buf][ct][0], bc[buf][ct][1], ah[rt][0], ah[rt][1]);
-                    mfma3<PREC>(accS[rt][ct], bs[buf][ct][0], bs[buf][ct][1], oh[rt][0], oh[rt][1]);
-                }
+                for (int r = 0; r < RTW; r++)
+                    mfma3<PREC>(acc[r][ct][0], bc[buf][ct][0], bc[buf][ct][1], ah[r][0], ah[r][1]);
+#pragma unroll
+                for (int r = 0; r < RTW; r++)
+                    mfma3<PREC>(acc[r][ct][1], bs[buf][ct][0], bs[buf][ct][1], oh[r][0], oh[r][1]);
+                const u16x8 mc0 = flip_odd(bc[buf][ct][0]), mc1 = flip_odd(bc[buf][ct][1]);
+                const u16x8 ms0 = flip_odd(bs[buf][ct][0]), ms1 = flip_odd(bs[buf][ct][1]);
+#pragma unroll
+                for (int r = 0; r < RTW; r++) mfma3<PREC>(acc[r][ct][2], ms0, ms1, ah[r][0], ah[r][1]);
+#pragma unroll
+                for (int r = 0; r < RTW; r++) mfma3<PREC>(acc[r][ct][3], mc0, mc1, oh[r][0], oh[r][1]);
                 if (NB == 1 && refill) loadB(0, ct, sn);
             }
         };
         constexpr int UPT = 2 * GB * FT / 256;            // operand units per thread and step pair
+        constexpr int MPS = (PREC == kPrecBF16 ? 4 : 12) * U;   // products per step and wave
         static_assert(kSPB == 2, "the loop below runs step pairs");
         // full pairs: straight-line, the next pair's operand build beside the products (unit i of
         // it loaded before step s0 + i and finished after it); the last pair's build and refills
@@ -973,47 +1031,58 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
             PrepIn pin;
             prep_load<FT>(pin, seg, wH, a, s0 + 2, tid, invH);
             step_products(rb, 0, 0, s0 + 1, true);
-            prep_finish<PREC, FT>(ring, pin, tid, rb ^ 1, segScale, nyq);
+            prep_finish<PREC, FT>(ring, pin, tid, rb ^ 1, segScale, q4);
             if (UPT == 2) prep_load<FT>(pin, seg, wH, a, s0 + 2, tid + 256, invH);
-            interleave_valu<(PREC == kPrecBF16 ? 2 : 6) * RT * CT>();
+            interleave_valu<MPS>();
             step_products(rb, 1, NB == 2 ? 1 : 0, min(s0 + 2, nSteps - 1), true);
-            if (UPT == 2) prep_finish<PREC, FT>(ring, pin, tid + 256, rb ^ 1, segScale, nyq);
-            interleave_valu<(PREC == kPrecBF16 ? 2 : 6) * RT * CT>();
+            if (UPT == 2) prep_finish<PREC, FT>(ring, pin, tid + 256, rb ^ 1, segScale, q4);
+            interleave_valu<MPS>();
             __syncthreads();                            // next pair built; buffer rb free again
         }
         if (nSteps & 1) step_products(nPairs & 1, 0, 0, 0, false);   // odd last step
-        nyqPart[tid] = nyq;
+        q4Part[tid] = q4;
         __syncthreads();  // segment and operand ring are dead: the power planes may overwrite them
 
-        // ---- power = cos^2 + sin^2 in lane, bin N/2 from the Nyquist partials, stored as bf16 hi and
+        // ---- power in lane, bin N/4 from the partial sums, stored as bf16 hi and lo planes
+        //      [FT][pws2] for the mel product (each value split once)
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
-        //      lo planes [FT][pws2] for the mel product (each value split once)
         uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
         uint16_t *pwL = pwH + FT * pws2;
         const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
-        const int binTile0 = wave * (CT * 32);
-        if (a.power == 1) store_power<true, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, binTile0, lane);
-        else store_power<false, RT, CT>(pwH, pwL, pws2, accC, accS, unscale, binTile0, lane);
-        if (tid < FT) {
-            float n = 0.0f;
-            for (int q = tid; q < NW * 64; q += FT) n += nyqPart[q];
-            float p = n * n;
-            if (a.power == 1) p = sqrtf(p);
-            put_power(pwH, pwL, tid * pws2 + halfN, p);
+        {
+            const int c0 = wave * U / NRT, r0 = wave * U % NRT;
+            if (a.power == 1) store_power<true, RTW, CTM>(pwH, pwL, pws2, acc, unscale, c0, r0, a.N / 128, halfN, lane);
+            else store_power<false, RTW, CTM>(pwH, pwL, pws2, acc, unscale, c0, r0, a.N / 128, halfN, lane);
         }
-        for (int e = tid; e < FT * (a.KP - a.K); e += NW * 64) {  // padding bins the mel product reads
-            const int f = e / (a.KP - a.K), k = a.K + e % (a.KP - a.K);
-            pwH[f * pws2 + k] = 0;
-            pwL[f * pws2 + k] = 0;
+        const int Q = a.N / 4;
+        if (tid < FT) {
+            float re = 0.0f, im = 0.0f;
+            for (int q = tid; q < NW * 64; q += FT) {
+                re += q4Part[q].x;
+                im += q4Part[q].y;
+            }
+            float p = 0.5f * (re * re + im * im);
+            if (a.power == 1) p = sqrtf(p);
+            put_power(pwH, pwL, tid * pws2 + Q, p);
+        }
+        {
+            // positions the mel product reads that hold no bin: Q+1..Q+3 and N/2+4 onwards
+            const int nz = 3 + a.KP - (halfN + 4);
+            for (int e = tid; e < FT * nz; e += NW * 64) {
+                const int f = e / nz, k = e % nz;
+                const int pos = k < 3 ? Q + 1 + k : halfN + 4 + (k - 3);
+                pwH[f * pws2 + pos] = 0;
+                pwL[f * pws2 + pos] = 0;
+            }
         }
         __syncthreads();
 
         // ---- mel work list of this wave; the weight fragments of its first kWA entries are
-        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         //      requested before the next tile's samples.  Slot u of the ring holds entry e with
         //      e % kWA == u; every refill is unconditional (an index past the end reloads the last
         //      entry, never used), so each slot stays in one set of registers and a product waits
         //      only for its own fragments.
+        tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         const int n = melSeqLen[wave];
         const uint16_t *seq = melSeq[wave];
         u16x8 wh[kWA], wl[kWA];
@@ -1213,23 +1282,24 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int span = (FT - 1) * a.H + a.W;
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
-        const size_t staticBytes = sizeof(float) * (4 + 4 * 64 + 4) + sizeof(uint16_t) * (4 * kMelSeq + 16);
+        const size_t staticBytes = sizeof(float) * (4 + 2 * 4 * 64 + 4) + sizeof(uint16_t) * (4 * kMelSeq + 16);
         const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * kSPB * FT, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
         const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
-#define RPPA_LAUNCH_ONE(PR, F, C)                                                                               \
+#define RPPA_LAUNCH_ONE(PR, F, C, R)                                                                            \
         do {                                                                                                    \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, F, C>),            \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, F, C, R>),         \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));     \
-            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats, wOff); \
+            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C, R>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats, wOff); \
         } while (0)
+        // units (32-bin column tile of direct bins, 32-frame row tile) per wave: nfft 256: 2 x 2
+        // over 4 waves; 512: 4 x 2; 768: 6 x 1 (two waves repeat a tile); 1024: 8 x 1
 #define RPPA_LAUNCH(PR)                                                                                         \
         do {                                                                                                    \
-            if (CT == 1) RPPA_LAUNCH_ONE(PR, 64, 1);                                                            \
-            else if (CT == 2) RPPA_LAUNCH_ONE(PR, 64, 2);                                                       \
-            else if (CT == 3) RPPA_LAUNCH_ONE(PR, 32, 3);                                                       \
-            else RPPA_LAUNCH_ONE(PR, 32, 4);                                                                    \
+            if (CT == 1) RPPA_LAUNCH_ONE(PR, 64, 1, 1);                                                         \
+            else if (CT == 2) RPPA_LAUNCH_ONE(PR, 64, 1, 2);                                                    \
+            else RPPA_LAUNCH_ONE(PR, 32, 2, 1);                                                                 \
         } while (0)
         if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
         else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);

@@ -77,14 +77,15 @@ __global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict_
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
-    if (c >= N || j >= Jp) return;
+    const int NC = N / 2, Q = N / 4;
+    if (c >= NC || j >= Jp) return;
     const int half = W / 2;
     double v = 0.0;
     if (j < half) {
         const double w = 0.5 * (static_cast<double>(win[half + j]) + static_cast<double>(win[half - 1 - j]));
         if (c == 0) wHalf[j] = static_cast<float>(w);
-        const bool isSin = c >= N / 2;
-        const long long k = isSin ? c - N / 2 : c;
+        const bool isSin = c >= Q;
+        const long long k = isSin ? c - Q : c;
         const long long m = (k * (2LL * j + 1)) % (2LL * N);
         const double th = kPi * static_cast<double>(m) / N;
         v = isSin ? -w * sin(th) : w * cos(th);
@@ -98,7 +99,7 @@ __global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict_
         h = f32_to_f16_rne(static_cast<float>(s));
         l = f32_to_f16_rne(static_cast<float>(s - f16_to_f32(h)));
     }
-    const size_t o = (static_cast<size_t>(j / 8) * N + c) * 8 + (j % 8);
+    const size_t o = (static_cast<size_t>(j / 8) * NC + c) * 8 + (j % 8);
     hi[o] = h;
     lo[o] = l;
 }
@@ -108,9 +109,11 @@ __global__ void mel_split_kernel(const float *__restrict__ dense, int M, int num
                                  uint16_t *__restrict__ hi, uint16_t *__restrict__ lo)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = blockIdx.y;
+    int k = blockIdx.y;   // power-plane position
     if (m >= Mpad || k >= KP) return;
-    float v = (m < M && k < numBins) ? dense[static_cast<size_t>(m) * numBins + k] : 0.0f;
+    const int Q = (numBins - 1) / 2;
+    const int bin = k <= Q ? k : (k >= Q + 4 ? k - 3 : -1);
+    float v = (m < M && bin >= 0 && bin < numBins) ? dense[static_cast<size_t>(m) * numBins + bin] : 0.0f;
     uint16_t h = f32_to_bf16_rne(v);
     uint16_t l = f32_to_bf16_rne(v - bf16_to_f32(h));
     size_t o = (static_cast<size_t>(k / 8) * Mpad + m) * 8 + (k % 8);
@@ -248,13 +251,13 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
     if (splitKind >= 0) {
         if (t.foldHi) { (void)hipFree(t.foldHi); (void)hipFree(t.foldLo); (void)hipFree(t.wHalf); }
         t.Jp = (W / 2 + 15) / 16 * 16;
-        const size_t n = static_cast<size_t>(t.Jp) * nfft;
+        const size_t n = static_cast<size_t>(t.Jp) * (nfft / 2);
         if (hipMalloc(&t.foldHi, n * 2) != hipSuccess || hipMalloc(&t.foldLo, n * 2) != hipSuccess ||
             hipMalloc(&t.wHalf, sizeof(float) * (W / 2)) != hipSuccess)
             return nullptr;
         t.splitKind = splitKind;
         const float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
-        hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft, 256), t.Jp), dim3(256), 0, h->stream, nfft, W,
+        hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft / 2, 256), t.Jp), dim3(256), 0, h->stream, nfft, W,
                            t.Jp, win, t.foldHi, t.foldLo, t.wHalf, splitKind, splitScale);
     }
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
