@@ -706,11 +706,11 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
 
 // The same work as prep_step for one unit u of the 2*kSPB*2*FT units of a step pair, split so
 // the loads can be issued ahead of the products they overlap: prep_load reads the segment and
-// window (rows past W/2 read the last real rows and are zeroed in prep_finish), prep_finish
-// forms, splits and stores.
+// window, prep_finish forms, splits and stores.  Rows past W/2 take the samples of the last real
+// rows (finite values that meet all-zero basis rows) and the four zeros stored after the window
+// half (so they add nothing to the bin N/4 sums): no masking.
 struct PrepIn {
     float4 up, dn, w;
-    bool live;
 };
 
 template <int FT>
@@ -720,12 +720,11 @@ __device__ __forceinline__ void prep_load(PrepIn &p, const float *seg, const flo
     const int f = u % FT, qd = u / FT;
     const int half = a.W / 2;
     const int j = 16 * s + 4 * qd;
-    p.live = j < half;
-    const int jc = p.live ? j : half - 4;
+    const int jc = min(j, half - 4);
     const int fb = f * (a.H + kSkew);
     p.up = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half + jc, invH)]);
     p.dn = *reinterpret_cast<const float4 *>(&seg[fb + seg_pos_r(half - 4 - jc, invH)]);
-    p.w = *reinterpret_cast<const float4 *>(&wHalf[jc]);
+    p.w = *reinterpret_cast<const float4 *>(&wHalf[min(j, half)]);
 }
 
 template <int PREC, int FT>
@@ -733,13 +732,12 @@ __device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u,
 {
     constexpr int GB = 2 * kSPB;
     const int f = u % FT, qd = u / FT;
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 up = p.live ? p.up : z, dn = p.live ? p.dn : z, w = p.w;   // selects, not a branch
+    const float4 up = p.up, dn = p.dn, w = p.w;
     const float e0 = up.x + dn.w, o0 = up.x - dn.w;
     const float e1 = up.y + dn.z, o1 = up.y - dn.z;
     const float e2 = up.z + dn.y, o2 = up.z - dn.y;
     const float e3 = up.w + dn.x, o3 = up.w - dn.x;
-    q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, w);   // zero e and o leave q4 unchanged
+    q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, w);   // zero w past W/2 leaves q4 unchanged
     uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
     split2<PREC>(e0, e1, segScale, eh0, el0);
     split2<PREC>(e2, e3, segScale, eh1, el1);
@@ -779,7 +777,17 @@ __device__ __forceinline__ void put_power(uint16_t *pwH, uint16_t *pwL, int idx,
 // halves of its four dwords), the operand of the mirror bins N/2 - k.
 __device__ __forceinline__ u16x8 flip_odd(const u16x8 v)
 {
-    return __builtin_bit_cast(u16x8, __builtin_bit_cast(u32x4, v) ^ 0x80000000u);
+    // one 32-bit XOR per dword (written out: left to itself the compiler splits it into 16-bit
+    // pieces, two instructions per dword)
+    const u32x4 w = __builtin_bit_cast(u32x4, v);
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t x;
+        __asm__("v_xor_b32 %0, 0x80000000, %1" : "=v"(x) : "v"(w[i]));
+        r[i] = x;
+    }
+    return __builtin_bit_cast(u16x8, r);
 }
 
 // bf16 hi and lo of four power values as two 8-byte plane stores.
@@ -874,7 +882,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     const float invH = 1.0f / static_cast<float>(a.H);
     const int nMT = a.Mpad / 16;
     float *wH = lds + wOff;                              // window half (bin N/4 sums), past the aliased region
-    for (int j = threadIdx.x; j < a.W / 2; j += blockDim.x) wH[j] = a.wHalf[j];
+    for (int j = threadIdx.x; j < a.W / 2 + 4; j += blockDim.x) wH[j] = j < a.W / 2 ? a.wHalf[j] : 0.0f;
 
     // mel work lists (the same for every tile): whole 16-filter tiles, largest band first, each to
     // the wave with the fewest band steps so far; an entry (filter tile, 32-bin step of its band)
@@ -964,15 +972,18 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         constexpr int NB = CTM == 1 ? 2 : 1;
         u16x8 bc[NB][CTM][2], bs[NB][CTM][2];
         const int NC = a.N / 2;
+        // 32-bit byte offsets from the table bases (the tables stay far below 4 GB), so the loads
+        // take a scalar base and a vector offset
+        const char *bHi = reinterpret_cast<const char *>(a.bHi), *bLo = reinterpret_cast<const char *>(a.bLo);
         auto loadB = [&](int buf, int ct, int s) {
-            const int c = min(ct0 + ct, nct - 1) * 32 + (lane & 31);
-            const size_t oc = (static_cast<size_t>(2 * s + (lane >> 5)) * NC + c) * 8;
-            const size_t os = oc + static_cast<size_t>(NC / 2) * 8;
-            bc[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + oc);
-            bs[buf][ct][0] = *reinterpret_cast<const u16x8 *>(a.bHi + os);
+            const uint32_t c = min(ct0 + ct, nct - 1) * 32 + (lane & 31);
+            const uint32_t oc = (static_cast<uint32_t>(2 * s + (lane >> 5)) * NC + c) * 16u;
+            const uint32_t os = oc + static_cast<uint32_t>(NC / 2) * 16u;
+            bc[buf][ct][0] = *reinterpret_cast<const u16x8 *>(bHi + oc);
+            bs[buf][ct][0] = *reinterpret_cast<const u16x8 *>(bHi + os);
             if (PREC != kPrecBF16) {
-                bc[buf][ct][1] = *reinterpret_cast<const u16x8 *>(a.bLo + oc);
-                bs[buf][ct][1] = *reinterpret_cast<const u16x8 *>(a.bLo + os);
+                bc[buf][ct][1] = *reinterpret_cast<const u16x8 *>(bLo + oc);
+                bs[buf][ct][1] = *reinterpret_cast<const u16x8 *>(bLo + os);
             }
         };
         float2 q4 = make_float2(0.0f, 0.0f);
@@ -1285,7 +1296,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const size_t staticBytes = sizeof(float) * (4 + 2 * 4 * 64 + 4) + sizeof(uint16_t) * (4 * kMelSeq + 16);
         const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * kSPB * FT, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
-        const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 3) / 4 * 4);
+        const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 4 + 3) / 4 * 4);   // window half + 4 zeros
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
 #define RPPA_LAUNCH_ONE(PR, F, C, R)                                                                            \
         do {                                                                                                    \
