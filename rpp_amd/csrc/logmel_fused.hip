@@ -503,6 +503,7 @@ __device__ __forceinline__ TileInfo tile_info(const LogMelArgs &a, int tile, int
 // of a thread's first kPF rounds are loaded ahead, into registers, while the previous tile's mel
 // product runs.
 constexpr int kPF = 11;
+constexpr int kBlockMfma = 256;   // threads of a fused MFMA block (the launch always uses this)
 struct SegPrefetch {
     float4 v[kPF];
     float prev[kPF];
@@ -537,7 +538,7 @@ __device__ __forceinline__ void seg_prefetch(SegPrefetch &p, const TileInfo &t, 
         seg_prefetch_none(p);
         return;
     }
-    const int nt = blockDim.x;
+    const int nt = kBlockMfma;
 #pragma unroll
     for (int q = 0; q < kPF; q++) {
         const int i = min(max(4 * (tid + q * nt), t.iLo), t.iHi);
@@ -572,7 +573,7 @@ __device__ __forceinline__ float4 edge_group(const LogMelArgs &a, const TileInfo
 __device__ __forceinline__ float seg_store(float *seg, const SegPrefetch &p, const LogMelArgs &a, const TileInfo &t,
                                            int span, float invH, int tid)
 {
-    const int groups = (span + 3) / 4, nt = blockDim.x;
+    const int groups = (span + 3) / 4, nt = kBlockMfma;
     float mx = 0.0f;
     auto put = [&](int i, float4 y) {
         *reinterpret_cast<float4 *>(&seg[seg_pos_r(i, invH)]) = y;
@@ -677,7 +678,7 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
     constexpr int GB = 2 * kSPB;                         // 8-row groups per buffer
     const int half = a.W / 2;
     uint2 *dst = reinterpret_cast<uint2 *>(ring);
-    for (int u = tid; u < 2 * GB * FT; u += blockDim.x) {
+    for (int u = tid; u < 2 * GB * FT; u += kBlockMfma) {
         const int f = u % FT, qd = u / FT;
         const int j = 16 * s + 4 * qd;                 // rows j .. j+3 (j a multiple of 4)
         float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
@@ -807,14 +808,15 @@ __device__ __forceinline__ void put_power4(uint16_t *pwH, uint16_t *pwL, int idx
         make_uint2(l[0] | (static_cast<uint32_t>(l[1]) << 16), l[2] | (static_cast<uint32_t>(l[3]) << 16));
 }
 
-// Power (|X| when SQRT) of the wave's units into the planes.  The DFT accumulators hold X^T: row =
+// Power (|X| when SQRT) of the wave's units into the planes (unscale2: the square of the operand
+// scaling to undo).  The DFT accumulators hold X^T: row =
 // bin, column = frame, so a lane owns frame rt*32 + lane%32 and, per group of four registers,
 // direct bins k0 .. k0+3 (k0 = ct*32 + 8q + 4(lane/32)): acc 0/1 give bin k (cos x e, sin x o),
 // acc 2/3 bin N/2 - k (odd-negated sin x e, cos x o).  Bin k sits at plane position k, bin
 // N/2 - k at N/2 + 3 - k, so both groups are 8-byte aligned runs (the mirror run reversed).
 template <bool SQRT, int RTW, int CTM>
 __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&acc)[RTW][CTM][4],
-                                            float unscale, int ct0, int rt0, int nct, int halfN, int lane)
+                                            float unscale2, int ct0, int rt0, int nct, int halfN, int lane)
 {
 #pragma unroll
     for (int r = 0; r < RTW; r++) {
@@ -828,9 +830,9 @@ __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pw
                 float pd[4], pm[4];
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const float c = acc[r][ct][0][4 * q + i] * unscale, sn = acc[r][ct][1][4 * q + i] * unscale;
-                    const float mc = acc[r][ct][2][4 * q + i] * unscale, ms = acc[r][ct][3][4 * q + i] * unscale;
-                    float d = c * c + sn * sn, m = mc * mc + ms * ms;
+                    const float c = acc[r][ct][0][4 * q + i], sn = acc[r][ct][1][4 * q + i];
+                    const float mc = acc[r][ct][2][4 * q + i], ms = acc[r][ct][3][4 * q + i];
+                    float d = fmaf(sn, sn, c * c) * unscale2, m = fmaf(ms, ms, mc * mc) * unscale2;
                     if (SQRT) { d = sqrtf(d); m = sqrtf(m); }
                     pd[i] = d;
                     pm[3 - i] = m;
@@ -859,7 +861,7 @@ static_assert(kWA >= 2, "the tile-end test reads entry e + 1 from the next slot"
 // hi/lo planes; banded mel on MFMA, each weight fragment applied to every frame group of the
 // tile; dB out.
 template <int PREC, int FT, int CTM, int RTW>
-__global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
+__global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
 {
     constexpr int NRT = FT / 32;
     constexpr int U = CTM * RTW;                         // units per wave
@@ -882,7 +884,7 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
     const float invH = 1.0f / static_cast<float>(a.H);
     const int nMT = a.Mpad / 16;
     float *wH = lds + wOff;                              // window half (bin N/4 sums), past the aliased region
-    for (int j = threadIdx.x; j < a.W / 2 + 4; j += blockDim.x) wH[j] = j < a.W / 2 ? a.wHalf[j] : 0.0f;
+    for (int j = threadIdx.x; j < a.W / 2 + 4; j += kBlockMfma) wH[j] = j < a.W / 2 ? a.wHalf[j] : 0.0f;
 
     // mel work lists (the same for every tile): whole 16-filter tiles, largest band first, each to
     // the wave with the fewest band steps so far; an entry (filter tile, 32-bin step of its band)
@@ -1062,8 +1064,9 @@ __global__ void __launch_bounds__(256, 2) logmel_mfma_kernel(LogMelArgs a, int s
         const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
         {
             const int c0 = wave * U / NRT, r0 = wave * U % NRT;
-            if (a.power == 1) store_power<true, RTW, CTM>(pwH, pwL, pws2, acc, unscale, c0, r0, a.N / 128, halfN, lane);
-            else store_power<false, RTW, CTM>(pwH, pwL, pws2, acc, unscale, c0, r0, a.N / 128, halfN, lane);
+            const float u2 = unscale * unscale;   // an exact power of two for fp16x3
+            if (a.power == 1) store_power<true, RTW, CTM>(pwH, pwL, pws2, acc, u2, c0, r0, a.N / 128, halfN, lane);
+            else store_power<false, RTW, CTM>(pwH, pwL, pws2, acc, u2, c0, r0, a.N / 128, halfN, lane);
         }
         const int Q = a.N / 4;
         if (tid < FT) {
@@ -1302,7 +1305,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         do {                                                                                                    \
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_mfma_kernel<PR, F, C, R>),         \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));     \
-            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C, R>), dim3(grid), dim3(256), bytes, h->stream, a, segFloats, wOff); \
+            hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C, R>), dim3(grid), dim3(kBlockMfma), bytes, h->stream, a, segFloats, wOff); \
         } while (0)
         // units (32-bin column tile of direct bins, 32-frame row tile) per wave: nfft 256: 2 x 2
         // over 4 waves; 512: 4 x 2; 768: 6 x 1 (two waves repeat a tile); 1024: 8 x 1
