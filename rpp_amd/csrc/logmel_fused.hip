@@ -28,6 +28,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
@@ -633,12 +634,13 @@ template <int PREC>
 __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi, uint32_t &lo)
 {
     if (PREC == kPrecFP16x3) {
+        // packed conversions (one round-to-nearest instruction per pair)
         const float u0 = v0 * s, u1 = v1 * s;
-        const _Float16 h0 = static_cast<_Float16>(u0), h1 = static_cast<_Float16>(u1);
-        const _Float16 l0 = static_cast<_Float16>(u0 - static_cast<float>(h0));
-        const _Float16 l1 = static_cast<_Float16>(u1 - static_cast<float>(h1));
-        hi = __builtin_bit_cast(uint16_t, h0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
-        lo = __builtin_bit_cast(uint16_t, l0) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, l1)) << 16);
+        const f16x2 h = {static_cast<_Float16>(u0), static_cast<_Float16>(u1)};
+        const f16x2 l = {static_cast<_Float16>(u0 - static_cast<float>(h.x)),
+                         static_cast<_Float16>(u1 - static_cast<float>(h.y))};
+        hi = __builtin_bit_cast(uint32_t, h);
+        lo = __builtin_bit_cast(uint32_t, l);
     } else {
         const __bf16 h0 = static_cast<__bf16>(v0), h1 = static_cast<__bf16>(v1);
         const __bf16 l0 = static_cast<__bf16>(v0 - static_cast<float>(h0));
