@@ -888,9 +888,8 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
     float *wH = lds + wOff;                              // window half (bin N/4 sums), past the aliased region
     for (int j = threadIdx.x; j < a.W / 2 + 4; j += kBlockMfma) wH[j] = j < a.W / 2 ? a.wHalf[j] : 0.0f;
 
-    // mel work lists (the same for every tile): whole 16-filter tiles, largest band first, each to
-    // the wave with the fewest band steps so far; an entry (filter tile, 32-bin step of its band)
-    // is applied to every 16-frame group of the tile, so one weight fragment serves FT/16 products
+    // mel work lists (the same for every tile): an entry (filter tile, half of the frame groups,
+    // 32-bin step of the band) applies one weight fragment to FT/32 frame groups
     if (threadIdx.x < nMT) {
         const int mt = threadIdx.x;
         int kb = a.KP, ke = 0;
@@ -913,20 +912,25 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        // work items: (filter tile, half of the tile's frame groups), costing the band's steps plus
+        // one for the stores at the item's end; largest first, each to the least-loaded wave
         int load[NW] = {0, 0, 0, 0}, cnt[NW] = {0, 0, 0, 0};
         uint32_t taken = 0;
-        for (int it = 0; it < nMT; it++) {
-            int best = -1, bestSteps = -1;
-            for (int mt = 0; mt < nMT; mt++) {
-                const int steps = (melBand[mt] & 255) - (melBand[mt] >> 8);
-                if (!(taken >> mt & 1) && steps > bestSteps) { best = mt; bestSteps = steps; }
+        const int nItems = 2 * nMT;
+        for (int it = 0; it < nItems; it++) {
+            int best = -1, bestCost = -1;
+            for (int item = 0; item < nItems; item++) {
+                const int mt = item >> 1;
+                const int cost = (melBand[mt] & 255) - (melBand[mt] >> 8) + 1;
+                if (!(taken >> item & 1) && cost > bestCost) { best = item; bestCost = cost; }
             }
             taken |= 1u << best;
             int w = 0;
             for (int q = 1; q < NW; q++) if (load[q] < load[w]) w = q;
-            for (int k = melBand[best] >> 8; k < (melBand[best] & 255); k++)
+            const int mt = best >> 1;
+            for (int k = melBand[mt] >> 8; k < (melBand[mt] & 255); k++)
                 melSeq[w][cnt[w]++] = static_cast<uint16_t>((best << 8) | k);
-            load[w] += bestSteps;
+            load[w] += bestCost;
         }
         for (int q = 0; q < NW; q++) melSeqLen[q] = cnt[q];
     }
@@ -1104,7 +1108,7 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         u16x8 wh[kWA], wl[kWA];
         int ent[kWA];
         auto loadW = [&](int slot, int e) {
-            const int en = seq[min(e, n - 1)], mt = en >> 8, k0 = (en & 255) * 32;
+            const int en = seq[min(e, n - 1)], mt = en >> 9, k0 = (en & 255) * 32;
             ent[slot] = en;
             const size_t o = (static_cast<size_t>(k0 / 8 + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
             wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
@@ -1129,51 +1133,67 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
-        //      16-filter tile's band of bins, for the FT/16 frame groups at once
+        //      16-filter tile's band of bins, FT/32 frame groups per work item
         if (n > 0) {
-            constexpr int FG = FT / 16;
+            constexpr int FGI = FT / 32;                     // frame groups per work item
+            const bool fastLog = a.minRatio >= 1.17549435e-38f;   // max(minRatio, x) is never subnormal
             float melMax = -3.40282347e38f;
-            f32x4 acc[FG];
+            f32x4 acc[FGI];
 #pragma unroll
-            for (int g = 0; g < FG; g++) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int g = 0; g < FGI; g++) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             for (int e0 = 0; e0 < n; e0 += kWA) {
 #pragma unroll
                 for (int u = 0; u < kWA; u++) {
                     const int e = e0 + u;
                     if (e < n) {
-                        const int mt = ent[u] >> 8, k0 = (ent[u] & 255) * 32;
+                        const int item = ent[u] >> 8, mt = item >> 1, gb = (item & 1) * FGI, k0 = (ent[u] & 255) * 32;
                         const bf16x8 w0 = __builtin_bit_cast(bf16x8, wh[u]), w1 = __builtin_bit_cast(bf16x8, wl[u]);
-                        const int q0 = (lane & 15) * pws2 + k0 + 8 * (lane >> 4);
-                        bf16x8 ph[FG], pl[FG];
+                        const int q0 = (gb * 16 + (lane & 15)) * pws2 + k0 + 8 * (lane >> 4);
+                        bf16x8 ph[FGI], pl[FGI];
 #pragma unroll
-                        for (int g = 0; g < FG; g++) {
+                        for (int g = 0; g < FGI; g++) {
                             ph[g] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwH[q0 + g * 16 * pws2]));
                             pl[g] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&pwL[q0 + g * 16 * pws2]));
                         }
 #pragma unroll
-                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph[g], acc[g], 0, 0, 0);
+                        for (int g = 0; g < FGI; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, ph[g], acc[g], 0, 0, 0);
 #pragma unroll
-                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl[g], acc[g], 0, 0, 0);
+                        for (int g = 0; g < FGI; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, pl[g], acc[g], 0, 0, 0);
 #pragma unroll
-                        for (int g = 0; g < FG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph[g], acc[g], 0, 0, 0);
+                        for (int g = 0; g < FGI; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, ph[g], acc[g], 0, 0, 0);
                         // slot (u + 1) % kWA already holds entry e + 1
-                        if (e + 1 == n || (ent[(u + 1) % kWA] >> 8) != mt) {
-                            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame g*16 + lane%16
+                        if (e + 1 == n || (ent[(u + 1) % kWA] >> 8) != item) {
+                            // lane holds mel rows mt*16 + 4*(lane/16) + r of frame (gb+g)*16 + lane%16;
+                            // the value to store is chosen once per item (uniform branches)
+                            f32x4 val[FGI];
+                            if (a.clipMax) {
+#pragma unroll
+                                for (int g = 0; g < FGI; g++) {
+                                    val[g] = acc[g];
+                                    melMax = fmaxf(melMax, fmaxf(fmaxf(acc[g][0], acc[g][1]), fmaxf(acc[g][2], acc[g][3])));
+                                }
+                            } else if (fastLog) {
+#pragma unroll
+                                for (int g = 0; g < FGI; g++)
+#pragma unroll
+                                    for (int r = 0; r < 4; r++)
+                                        val[g][r] = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, acc[g][r] * a.invRef));
+                            } else {
+#pragma unroll
+                                for (int g = 0; g < FGI; g++)
+#pragma unroll
+                                    for (int r = 0; r < 4; r++) val[g][r] = db_fast(acc[g][r], a);
+                            }
                             const int m0 = mt * 16 + 4 * (lane >> 4);
                             const size_t rStride = a.ntf ? 1 : v.hs;   // between consecutive rows m
 #pragma unroll
-                            for (int g = 0; g < FG; g++) {
-                                const int fr = g * 16 + (lane & 15);
+                            for (int g = 0; g < FGI; g++) {
+                                const int fr = (gb + g) * 16 + (lane & 15);
                                 if (f0 + fr < v.F) {
                                     float *o = out_at(a, v, m0, f0 + fr);
 #pragma unroll
-                                    for (int r = 0; r < 4; r++) {
-                                        if (m0 + r < a.M) {
-                                            const float pm = acc[g][r];
-                                            if (a.clipMax) melMax = fmaxf(melMax, pm);
-                                            o[r * rStride] = a.clipMax ? pm : db_fast(pm, a);
-                                        }
-                                    }
+                                    for (int r = 0; r < 4; r++)
+                                        if (m0 + r < a.M) o[r * rStride] = val[g][r];
                                 }
                                 acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                             }
