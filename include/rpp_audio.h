@@ -197,6 +197,15 @@ RppStatus rppt_resample_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t ds
 
 /* ======================= extensions (no counterpart in the reference) ======================= */
 
+#ifdef __cplusplus
+/* rppt_down_mixing_gpu followed by rppt_resample_gpu in one launch (the down-mixed signal stays
+ * on chip): src interleaved [n][samples * channels] with srcDimsTensor[2b] = samples,
+ * [2b+1] = channels; dst mono [n][nStride].  Bit-identical to the two calls in sequence. */
+RppStatus rppt_down_mix_resample_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                     Rpp32f *inRateTensor, Rpp32f *outRateTensor, Rpp32s *srcDimsTensor,
+                                     bool normalizeWeights, RpptResamplingWindow &window, rppHandle_t rppHandle);
+#endif
+
 /* Arithmetic of the fused DFT / mel contraction. */
 typedef enum {
     RPPT_LOGMEL_FP32 = 0,      /* FP32 vector FMA throughout */

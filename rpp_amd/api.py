@@ -104,6 +104,7 @@ def lib():
                                                            C.c_int32, P]
         l.rppt_down_mixing_gpu.argtypes = [P, DP, P, DP, P, C.c_bool, P]
         l.rppt_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.POINTER(RpptResamplingWindow), P]
+        l.rppt_down_mix_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.c_bool, C.POINTER(RpptResamplingWindow), P]
         l.rppt_logmel_chain_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
@@ -265,6 +266,7 @@ def make_window(lut_dev, scale: float, center: float, lobes: int, coeffs: int) -
     w.scale, w.center, w.lobes, w.coeffs = scale, center, lobes, coeffs
     w.lookupSize = lut_dev.numel()
     w.lookupPinned = lut_dev.data_ptr()
+    w._table = lut_dev   # the struct holds a raw pointer: keep the table alive with it
     return w
 
 
@@ -276,6 +278,19 @@ def resample(h: Handle, src, dst, in_rate, out_rate, src_dims, window: RpptResam
                                  ptr(src_dims), C.byref(window), h.h)
     if check_status:
         check("rppt_resample_gpu", st)
+    return st
+
+
+def down_mix_resample(h: Handle, src, dst, in_rate, out_rate, src_dims, window: RpptResamplingWindow, normalize=False,
+                      channels=2, check_status=True) -> int:
+    """rppt_down_mix_resample_gpu: src [B, L*C] interleaved, src_dims int32 [B*2] = (L, C); dst mono [B, n]."""
+    B, S = src.shape
+    sd = desc_1d(B, S, num_dims=3, channels=channels)
+    dd = desc_1d(B, dst.shape[1])
+    st = lib().rppt_down_mix_resample_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(in_rate), ptr(out_rate),
+                                          ptr(src_dims), normalize, C.byref(window), h.h)
+    if check_status:
+        check("rppt_down_mix_resample_gpu", st)
     return st
 
 
@@ -347,19 +362,18 @@ def get_tiles(h: Handle):
 
 def front_end_logmel(h: Handle, stereo, lengths, channels, lut_dev, scale, center, lobes, coeffs, in_rate, out_rate,
                      params: RpptLogMelParams, precision="fp16x3", workspace=None):
-    """Workload-4 front end through the drop-in entry points: down-mix (interleaved channels ->
-    mono), resample in_rate -> out_rate with the caller's table, then the fused log-mel launch.
-    stereo [B, L*C] cuda; lengths int32 cuda [B] (frames per channel).  Returns (db, mono, resampled)."""
+    """Workload-4 front end: down-mix (interleaved channels -> mono) and resample in_rate ->
+    out_rate with the caller's table in one launch (rppt_down_mix_resample_gpu, bit-identical to
+    rppt_down_mixing_gpu + rppt_resample_gpu), then the fused log-mel launch.
+    stereo [B, L*C] cuda; lengths int32 cuda [B] (frames per channel).  Returns (db, resampled)."""
     B, S = stereo.shape
     dev = stereo.device
     n_mono = S // channels
     ws = workspace if workspace is not None else {}
-    if "mono" not in ws:
-        ws["mono"] = torch.empty((B, n_mono), dtype=torch.float32, device=dev)
+    if "res" not in ws:
         n_out = int(-(-n_mono * out_rate // in_rate)) + 8
         ws["res"] = torch.empty((B, n_out), dtype=torch.float32, device=dev)
         ws["dims_dm"] = torch.stack([lengths, torch.full_like(lengths, channels)], 1).reshape(-1).contiguous()
-        ws["dims_rs"] = torch.stack([lengths, torch.ones_like(lengths)], 1).reshape(-1).contiguous()
         ws["in_rate"] = torch.full((B,), in_rate, dtype=torch.float32, device=dev)
         ws["out_rate"] = torch.full((B,), out_rate, dtype=torch.float32, device=dev)
         out_len = torch.ceil(lengths.to(torch.float32) * torch.tensor(out_rate, dtype=torch.float32, device=dev)
@@ -368,10 +382,10 @@ def front_end_logmel(h: Handle, stereo, lengths, channels, lut_dev, scale, cente
         F = int(out_len.max().item()) // params.windowStep + 1
         ws["db"] = torch.empty((B, params.numFilter, F), dtype=torch.float32, device=dev)
         ws["window"] = make_window(lut_dev, scale, center, lobes, coeffs)
-    down_mixing(h, stereo, ws["mono"], ws["dims_dm"], normalize=False, channels=channels)
-    resample(h, ws["mono"], ws["res"], ws["in_rate"], ws["out_rate"], ws["dims_rs"], ws["window"])
+    down_mix_resample(h, stereo, ws["res"], ws["in_rate"], ws["out_rate"], ws["dims_dm"], ws["window"],
+                      normalize=False, channels=channels)
     logmel_fused(h, ws["res"], ws["db"], ws["res_len"], params, precision)
-    return ws["db"], ws["mono"], ws["res"]
+    return ws["db"], ws["res"]
 
 
 def logmel_chain(h: Handle, x, lengths_dev, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000.0, fmin=0.0,

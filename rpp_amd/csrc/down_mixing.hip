@@ -9,6 +9,7 @@
 // normalizeWeights and sums in a different order, hip/kernel/down_mixing.cpp:28-70.)
 // One thread per output sample; HBM-bound: 4*(C+1) bytes per sample.
 #include "common.hpp"
+#include "down_mix_math.hpp"
 
 namespace rppa {
 namespace {
@@ -28,36 +29,7 @@ down_mix_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     if (t >= S) return;
     const float *x = src + static_cast<size_t>(b) * srcStride + t * C;
     float *y = dst + static_cast<size_t>(b) * dstStride;
-    if (C == 1) { y[t] = x[0]; return; }
-    float w = 1.f / C;
-    if (normalize) {
-        float sum = 0.0f;
-        for (int c = 0; c < C; c++) sum = sum + w;
-        float inv = static_cast<float>(1.0 / sum);
-        w = w * inv;
-    }
-    int c = 0;
-    float s;
-    if (C > 7) {
-        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const int aligned = (C / 8) * 8;
-        for (; c < aligned; c += 8) {
-#pragma unroll
-            for (int l = 0; l < 8; l++) { float p = x[c + l] * w; acc[l] = acc[l] + p; }
-        }
-        float s0 = acc[0] + acc[4], s1 = acc[1] + acc[5], s2 = acc[2] + acc[6], s3 = acc[3] + acc[7];
-        s = (s0 + s2) + (s1 + s3);
-    } else {
-        float acc[4] = {0, 0, 0, 0};
-        const int aligned = (C / 4) * 4;
-        for (; c < aligned; c += 4) {
-#pragma unroll
-            for (int l = 0; l < 4; l++) { float p = x[c + l] * w; acc[l] = acc[l] + p; }
-        }
-        s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    }
-    for (; c < C; c++) { float p = x[c] * w; s = s + p; }
-    y[t] = s;
+    y[t] = down_mix_sample(x, C, down_mix_weight(C, normalize));
 }
 
 }  // namespace
