@@ -190,9 +190,9 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     const int spanLen = min(kSpan, static_cast<int>(ceil(kBlockOut * scale)) + 2 * lobes + 4);
     float *sp = span + wave * kSpan;
     if (staged && ob < outEnd) {
-        // eight samples per lane in flight: the loads of a batch are issued before any is used
+        // sixteen samples per lane in flight: the loads of a batch are issued before any is used
         // (mono and stereo; other channel counts down-mix sample by sample)
-        constexpr int kU = 8;
+        constexpr int kU = 16;
         const bool pairs = C == 2 && (reinterpret_cast<uintptr_t>(x) & 7) == 0;   // 8-byte stereo loads
         if (C == 1 || pairs) {
             for (int j0 = lane; j0 < spanLen; j0 += kU * kWave) {
