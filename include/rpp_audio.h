@@ -238,6 +238,15 @@ RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
                                 Rpp32s *srcLengthTensor, const RpptLogMelParams *params,
                                 RpptLogMelPrecision precision, rppHandle_t rppHandle);
 
+/* The fused launch over a region of each row: samples [srcBeginTensor[b], + srcLengthTensor[b])
+ * (begin clamped to the row, the length to what remains of it), e.g. the begin/length pair
+ * rppt_non_silent_region_detection_gpu writes.  Same output as rppt_logmel_fused_gpu on the
+ * trimmed clips, with no copy of them.  Both arrays device or pinned. */
+RppStatus rppt_logmel_fused_region_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                       RpptDescPtr dstDescPtr, Rpp32s *srcBeginTensor, Rpp32s *srcLengthTensor,
+                                       const RpptLogMelParams *params, RpptLogMelPrecision precision,
+                                       rppHandle_t rppHandle);
+
 /* Packed variant for serving: clip b's samples at srcPtr + srcOffsets[b] (srcLengthTensor[b] of
  * them), its dB output as a dense [numFilter][frames_b] block at dstPtr + dstOffsets[b], with
  * frames_b = L/H + 1 (centered) or (L-W)/H + 1.  Offsets and lengths in device or pinned memory. */

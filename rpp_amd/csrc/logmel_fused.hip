@@ -41,6 +41,7 @@ struct LogMelArgs {
     const float *src;
     uint32_t srcStride;
     const int *lengths;
+    const int *begin;      // first sample of each clip's region in its row (rows start at 0 if null)
     int B;
     float *dst;
     uint32_t dstNStride, hStride;
@@ -111,6 +112,14 @@ struct ClipView {
     int L, F, b;
 };
 
+// Region of clip b's row: samples [start, start + length) with start = begin[b] (0 without a
+// begin array) clamped to the row, and the length clamped so the region never leaves the row.
+__device__ __forceinline__ int row_region(const int *begin, const int *lengths, int b, uint32_t srcStride, int &start)
+{
+    start = begin ? min(max(begin[b], 0), static_cast<int>(srcStride)) : 0;
+    return max(min(lengths[b], static_cast<int>(srcStride) - start), 0);
+}
+
 __device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
 {
     ClipView v;
@@ -118,8 +127,9 @@ __device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
         v.L = max(a.lengths[b], 0);
         v.x = a.src + a.srcOff[b];
     } else {
-        v.L = max(min(a.lengths[b], static_cast<int>(a.srcStride)), 0);  // never read past the clip's row
-        v.x = a.src + static_cast<size_t>(b) * a.srcStride;
+        int start;
+        v.L = row_region(a.begin, a.lengths, b, a.srcStride, start);   // never read past the clip's row
+        v.x = a.src + static_cast<size_t>(b) * a.srcStride + start;
     }
     int F = max(num_windows(v.L, a.W, a.H, a.center != 0), 0);
     if (a.dstOff) {
@@ -194,8 +204,9 @@ __device__ __forceinline__ float db_exact(float v, const LogMelArgs &a)
 // Single block: tiles per clip = ceil(F_b / FT) with F_b = min(frames(L_b), hStride); exclusive
 // prefix sum into tileStart[0..B], total in tileStart[B].
 __global__ void __launch_bounds__(1024)
-plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, uint32_t hStride, int W, int H,
-                  int center, int FT, int packed, int *__restrict__ tileStart, int *__restrict__ clipMax)
+plan_tiles_kernel(const int *__restrict__ lengths, const int *__restrict__ begin, int B, uint32_t srcStride,
+                  uint32_t hStride, int W, int H, int center, int FT, int packed, int *__restrict__ tileStart,
+                  int *__restrict__ clipMax)
 {
     if (clipMax)  // the fused kernel raises these with atomicMax (float bits; stored power is never below -FLT_MAX)
         for (int b = threadIdx.x; b < B; b += blockDim.x) clipMax[b] = __float_as_int(-3.40282347e38f);
@@ -208,7 +219,8 @@ plan_tiles_kernel(const int *__restrict__ lengths, int B, uint32_t srcStride, ui
         const int b = base + tid;
         int n = 0;
         if (b < B) {
-            int L = packed ? lengths[b] : min(lengths[b], static_cast<int>(srcStride));
+            int start;
+            int L = packed ? lengths[b] : row_region(begin, lengths, b, srcStride, start);
             int F = max(0, num_windows(max(L, 0), W, H, center != 0));
             if (!packed) F = min(F, static_cast<int>(hStride));
             n = (F + FT - 1) / FT;
@@ -1231,7 +1243,8 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
 // Validated parameters + buffers -> planner and fused kernel on h->stream.
 RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPrecision precision, const float *src,
                         uint32_t srcStride, const long long *srcOff, const int *lengths, int B, float *dst,
-                        uint32_t dstNStride, uint32_t hStride, const long long *dstOff, int ntf, uint32_t frameCap)
+                        uint32_t dstNStride, uint32_t hStride, const long long *dstOff, int ntf, uint32_t frameCap,
+                        const int *begin = nullptr)
 {
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
@@ -1249,6 +1262,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.src = src;
     a.srcStride = srcStride;
     a.lengths = lengths;
+    a.begin = begin;
     a.B = B;
     a.dst = dst;
     a.dstNStride = dstNStride;
@@ -1299,8 +1313,8 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
-    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, a.B, a.srcStride, a.frameCap,
-                       a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax);
+    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, begin, a.B, a.srcStride,
+                       a.frameCap, a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax);
     const int pws = a.KP + kSkew;
     if (!mfma) {
         const int span = (kFT32 - 1) * a.H + a.W;
@@ -1377,10 +1391,12 @@ RPPA_EXPORT RppStatus rppt_logmel_get_tiles(rppHandle_t h, RpptLogMelTiles *t)
     return RPP_SUCCESS;
 }
 
-RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
-                                            RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
-                                            const RpptLogMelParams *p, RpptLogMelPrecision precision,
-                                            rppHandle_t h)
+namespace rppa {
+namespace {
+// Validation and launch shared by the two strided entry points (begin null: whole rows).
+RppStatus fused_entry(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                      const Rpp32s *srcBeginTensor, Rpp32s *srcLengthTensor, const RpptLogMelParams *p,
+                      RpptLogMelPrecision precision, rppHandle_t h)
 {
     if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
     if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
@@ -1400,7 +1416,31 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDesc
     const uint32_t frameCap = ntf ? dstDescPtr->h : dstDescPtr->strides.hStride;
     return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
                          srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
-                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap);
+                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap,
+                         srcBeginTensor);
+}
+}  // namespace
+}  // namespace rppa
+
+RPPA_EXPORT RppStatus rppt_logmel_fused_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                            RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                            const RpptLogMelParams *p, RpptLogMelPrecision precision,
+                                            rppHandle_t h)
+{
+    return rppa::fused_entry(srcPtr, srcDescPtr, dstPtr, dstDescPtr, nullptr, srcLengthTensor, p, precision, h);
+}
+
+// The fused launch over a region of each clip's row: samples [srcBeginTensor[b], + srcLengthTensor[b])
+// (begin clamped to the row, the length to what remains of it), e.g. the begin/length pair
+// rppt_non_silent_region_detection_gpu writes.  Same output as rppt_logmel_fused_gpu on the
+// trimmed clips, with no copy of them.
+RPPA_EXPORT RppStatus rppt_logmel_fused_region_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                                   RpptDescPtr dstDescPtr, Rpp32s *srcBeginTensor,
+                                                   Rpp32s *srcLengthTensor, const RpptLogMelParams *p,
+                                                   RpptLogMelPrecision precision, rppHandle_t h)
+{
+    if (!srcBeginTensor) return RPP_ERROR_INVALID_ARGUMENTS;
+    return rppa::fused_entry(srcPtr, srcDescPtr, dstPtr, dstDescPtr, srcBeginTensor, srcLengthTensor, p, precision, h);
 }
 
 // Autotuner: times every tile shape the launch supports on the caller's batch (events on the

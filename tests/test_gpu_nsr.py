@@ -79,3 +79,41 @@ def test_nsr_status(gpu, handle):
     o = torch.zeros(2, dtype=torch.int32, device="cuda")
     assert gpu.non_silent_region_detection(handle, x, L, o, o, -60.0, 0, 0.0, 16, check_status=False) == -2
     assert gpu.non_silent_region_detection(handle, x, L, o, o, -60.0, 8, 0.0, 0, check_status=False) == -2
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "fp32"])
+def test_nsr_regions_feed_fused_logmel(gpu, handle, precision):
+    """Non-silent regions detected on the GPU go straight into the fused launch
+    (rppt_logmel_fused_region_gpu): the output equals the fused launch on copies of the trimmed
+    clips, bit for bit; begin/length pairs that leave the row are clamped to it."""
+    rng = np.random.default_rng(7)
+    L = np.array([40000, 16000, 2049, 90000, 800, 5], np.int32)
+    S = int(L.max()) + 16
+    x = bursty(rng, L, S)
+    xs = torch.from_numpy(x).cuda()
+    Ls = torch.from_numpy(L).cuda()
+    b = torch.zeros(len(L), dtype=torch.int32, device="cuda")
+    n = torch.zeros(len(L), dtype=torch.int32, device="cuda")
+    gpu.non_silent_region_detection(handle, xs, Ls, b, n, -30.0, 1024, 0.0, 8192)
+    p = gpu.logmel_params()
+    nb, nn = b.cpu().numpy(), n.cpu().numpy()
+    # add two pairs that reach past the row: the launch clamps them
+    nb = np.append(nb, [S - 100, -5]).astype(np.int32)
+    nn = np.append(nn, [5000, 300]).astype(np.int32)
+    x2 = np.concatenate([x, x[:2]], 0)
+    starts = np.clip(nb, 0, S)
+    lens = np.maximum(np.minimum(nn, S - starts), 0).astype(np.int32)
+    frames = lens // 160 + 1
+    cap = int(frames.max())
+    out = torch.full((len(nb), 80, cap), float("nan"), device="cuda")
+    gpu.logmel_fused_region(handle, torch.from_numpy(x2).cuda(), out, torch.from_numpy(nb).cuda(),
+                            torch.from_numpy(nn).cuda(), p, precision)
+    trimmed = np.zeros((len(nb), S), np.float32)
+    for i in range(len(nb)):
+        trimmed[i, :lens[i]] = x2[i, starts[i]:starts[i] + lens[i]]
+    ref = torch.full((len(nb), 80, cap), float("nan"), device="cuda")
+    gpu.logmel_fused(handle, torch.from_numpy(trimmed).cuda(), ref, torch.from_numpy(lens).cuda(), p, precision)
+    torch.cuda.synchronize()
+    o, r = out.cpu().numpy(), ref.cpu().numpy()
+    for i, F in enumerate(frames):
+        assert np.array_equal(o[i, :, :F], r[i, :, :F], equal_nan=True), i
