@@ -940,7 +940,9 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
             int w = 0;
             for (int q = 1; q < NW; q++) if (load[q] < load[w]) w = q;
             const int mt = best >> 1;
-            for (int k = melBand[mt] >> 8; k < (melBand[mt] & 255); k++)
+            // at most 2 * 16 tiles x 17 steps = 544 entries in all (numFilter <= 256, nfft <= 1024),
+            // so the least-loaded wave stays below 544 / 4 + 17 <= kMelSeq; the bound is checked anyway
+            for (int k = melBand[mt] >> 8; k < (melBand[mt] & 255) && cnt[w] < kMelSeq; k++)
                 melSeq[w][cnt[w]++] = static_cast<uint16_t>((best << 8) | k);
             load[w] += bestCost;
         }
