@@ -140,6 +140,8 @@ def main() -> int:
                     help="first run a device-to-device copy of this many MiB (known-size kernel for HBM counters)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="duration of the served-stream measurement (Poisson arrivals, 4000 clips/s); 0 skips it")
+    ap.add_argument("--front-end-only", action="store_true",
+                    help="only the workload-4 front end (down-mix + resample, then fused log-mel; profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus > 1 is not supported "
@@ -150,6 +152,9 @@ def main() -> int:
     from rpp_amd import api, synth
 
     torch.cuda.set_device(0)
+    if a.front_end_only:
+        print(json.dumps({"front_end": front_end_measurement(api, synth, torch, a.precision, steps=a.steps)}))
+        return 0
     if a.copy_calibration_mb:
         n = a.copy_calibration_mb * (1 << 20) // 4
         src_c = torch.rand(n, device="cuda")
