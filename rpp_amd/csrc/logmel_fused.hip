@@ -319,9 +319,9 @@ __global__ void __launch_bounds__(256) db_from_clip_max_kernel(LogMelArgs a, int
 }
 
 // ------------------------------------------------------------------------- FP32 kernel
-// FT = 32 frames per tile; each thread owns 4 frames x 16 columns of a 512-column pass; the K
-// loop order (t ascending, fmaf) and every later step match spectrogram.hip / mel_filter_bank.hip /
-// to_decibels.hip exactly, so the output equals the 4-call chain bit for bit.
+// FT = 32 frames per tile; each thread owns 4 frames x 8 column pairs (cg, cg + 32, ...) of a
+// 512-column pass; the K loop order (t ascending, fmaf) and every later step match spectrogram.hip /
+// mel_filter_bank.hip / to_decibels.hip exactly, so the output equals the 4-call chain bit for bit.
 constexpr int kFT32 = 32;
 constexpr int kKC = 16;
 constexpr int kCC = 512;
@@ -371,7 +371,11 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
                     for (int i = 0; i < 4; i++) av[i] = sA[kk][fg * 4 + i];
                     float bv[16];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) bv[j] = sB[kk * kCC + cg * 16 + j];
+                    for (int j = 0; j < 8; j++) {   // column pair cg + 32 j: consecutive lanes, consecutive pairs
+                    const float2 b2 = *reinterpret_cast<const float2 *>(&sB[kk * kCC + 2 * (cg + 32 * j)]);
+                    bv[2 * j] = b2.x;
+                    bv[2 * j + 1] = b2.y;
+                }
 #pragma unroll
                     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -386,7 +390,7 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const float re = acc[i][2 * j], im = acc[i][2 * j + 1];
-                    const int p = pass * (kCC / 2) + cg * 8 + j;
+                    const int p = pass * (kCC / 2) + cg + 32 * j;
                     if (p == 0) {
                         float v = re * re, nq = im * im;
                         if (a.power == 1) { v = sqrtf(v); nq = sqrtf(nq); }

@@ -14,7 +14,8 @@
 //
 // Tiling (one block = 32 frames of one clip, 256 threads = 4 waves):
 //   per K step of 16 samples the block stages A[16][32] and basis[16][512] in LDS; each thread
-//   owns 4 frames x 16 columns (8 re/im pairs) of the 32 x 512 output tile in registers;
+//   owns 4 frames x 8 re/im column pairs (pairs cg, cg + 32, ... so consecutive lanes read
+//   consecutive LDS words) of the 32 x 512 output tile in registers;
 //   power is formed in registers, staged in LDS and stored coalesced along frames (NFT) or
 //   bins (NTF).  Columns beyond 512 (nfft > 512) are processed in further 512-column passes.
 #include "common.hpp"
@@ -61,7 +62,7 @@ spectrogram_fp32_kernel(SpecArgs a)
     const float *x = a.src + static_cast<size_t>(b) * a.srcStride;
     const int off = a.center ? a.W / 2 : 0;
     const int tid = threadIdx.x;
-    const int cg = tid & 31;        // column group: 16 columns
+    const int cg = tid & 31;        // column pairs cg, cg + 32, ..., cg + 224 of a pass
     const int fg = tid >> 5;        // frame group: 4 frames
     const int K = a.N / 2 + 1;
     const int nPasses = (a.cols + kCC - 1) / kCC;
@@ -98,7 +99,11 @@ spectrogram_fp32_kernel(SpecArgs a)
                 for (int i = 0; i < 4; i++) av[i] = sA[kk][fg * 4 + i];
                 float bv[16];
 #pragma unroll
-                for (int j = 0; j < 16; j++) bv[j] = sB[kk * kCC + cg * 16 + j];
+                for (int j = 0; j < 8; j++) {   // column pair cg + 32 j: consecutive lanes, consecutive pairs
+                    const float2 b2 = *reinterpret_cast<const float2 *>(&sB[kk * kCC + 2 * (cg + 32 * j)]);
+                    bv[2 * j] = b2.x;
+                    bv[2 * j + 1] = b2.y;
+                }
 #pragma unroll
                 for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -114,7 +119,7 @@ spectrogram_fp32_kernel(SpecArgs a)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 float re = acc[i][2 * j], im = acc[i][2 * j + 1];
-                int p = cg * 8 + j;
+                int p = cg + 32 * j;
                 float v;
                 float extra = -1.0f;
                 if (pass == 0 && p == 0) {
