@@ -33,6 +33,25 @@ def test_down_mixing(gpu, handle, orc, C, normalize):
         assert np.all(y[b, m:] == 9.0)
 
 
+@pytest.mark.parametrize("normalize", [False, True])
+def test_down_mixing_stereo_vectorised(gpu, handle, orc, normalize):
+    """Rows with 16-byte aligned, multiple-of-4 strides take the four-samples-per-thread stereo
+    kernel (tails of 1-3 samples included); a mono row in the same batch takes the generic one."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    L = np.array([n, 4001 - 1, 1, 6, 2999, 3000], np.int32)
+    C = np.array([2, 2, 2, 2, 1, 2], np.int32)
+    x = rng.uniform(-1, 1, (len(L), 2 * n)).astype(np.float32)
+    dims = np.stack([L, C], 1).reshape(-1).astype(np.int32)
+    y = torch.full((len(L), n), 9.0, device="cuda")
+    gpu.down_mixing(handle, dev(x), y, dev(dims), normalize=normalize, channels=2)
+    y = y.cpu().numpy()
+    for b, (m, c) in enumerate(zip(L, C)):
+        ref = orc.down_mix_f32(x[b:b + 1], L[b:b + 1], [c], normalize)
+        assert np.array_equal(y[b, :m], ref[0, :m]), b
+        assert np.all(y[b, m:] == 9.0), b
+
+
 def window_for(coeffs, lobes):
     from oracle import oracle as O
     lut, sc, ce = O.windowed_sinc_lut(coeffs, lobes)
