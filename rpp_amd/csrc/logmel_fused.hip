@@ -55,6 +55,8 @@ struct LogMelArgs {
     int *clipMax;          // per-clip maximum of the mel power (float bits) when the reference is the
                            // clip maximum: the kernel then stores power and a second pass converts to dB
     const int *tileStart;  // [B + 1]
+    const int4 *tileRec;   // [tiles] clip, first frame, region length, region start of each tile (null:
+                           // the MFMA kernel searches tileStart instead)
     // packed layout (both null for the [n][nStride] / [n][M][hStride] descriptors): clip b's samples
     // at src + srcOff[b], its dB rows at dst + dstOff[b] as a dense [M][frames_b] block
     const long long *srcOff, *dstOff;
@@ -65,7 +67,7 @@ struct LogMelArgs {
     const uint16_t *bHi, *bLo;      // folded split basis (DftTables::foldHi/foldLo)
     const float *wHalf;
     int Wp;                         // folded contraction length, padded to 16
-    float basisScale;
+    float basisInv;                 // 1 / the scale of the fp16 basis (an exact power of two)
     // mel
     const float *weightsDown;
     const int *intervals;
@@ -112,28 +114,31 @@ struct ClipView {
     int L, F, b;
 };
 
+// Element i of an array that stays unchanged while the kernel runs (the per-clip inputs, the tile
+// plan).  Read through the constant address space: with a wave-uniform index the compiler issues a
+// scalar load, which neither waits for nor holds up the vector loads in flight (a divergent index
+// still gets a vector load).
+template <typename T>
+__device__ __forceinline__ T ld_const(const T *p, int i)
+{
+    typedef const __attribute__((address_space(4))) T *ConstPtr;
+    return ((ConstPtr)p)[i];
+}
+
 // Region of clip b's row: samples [start, start + length) with start = begin[b] (0 without a
 // begin array) clamped to the row, and the length clamped so the region never leaves the row.
 __device__ __forceinline__ int row_region(const int *begin, const int *lengths, int b, uint32_t srcStride, int &start)
 {
-    start = begin ? min(max(begin[b], 0), static_cast<int>(srcStride)) : 0;
-    return max(min(lengths[b], static_cast<int>(srcStride) - start), 0);
+    start = begin ? min(max(ld_const(begin, b), 0), static_cast<int>(srcStride)) : 0;
+    return max(min(ld_const(lengths, b), static_cast<int>(srcStride) - start), 0);
 }
 
-__device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
+// Frame count and output placement of clip b, whose samples (length v.L) are already set in v.
+__device__ __forceinline__ ClipView clip_view_out(const LogMelArgs &a, ClipView v, int b)
 {
-    ClipView v;
-    if (a.srcOff) {
-        v.L = max(a.lengths[b], 0);
-        v.x = a.src + a.srcOff[b];
-    } else {
-        int start;
-        v.L = row_region(a.begin, a.lengths, b, a.srcStride, start);   // never read past the clip's row
-        v.x = a.src + static_cast<size_t>(b) * a.srcStride + start;
-    }
     int F = max(num_windows(v.L, a.W, a.H, a.center != 0), 0);
     if (a.dstOff) {
-        v.out = a.dst + a.dstOff[b];
+        v.out = a.dst + ld_const(a.dstOff, b);
         v.hs = static_cast<uint32_t>(F);
     } else {
         v.out = a.dst + static_cast<size_t>(b) * a.dstNStride;
@@ -143,6 +148,20 @@ __device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
     v.F = F;
     v.b = b;
     return v;
+}
+
+__device__ __forceinline__ ClipView clip_view(const LogMelArgs &a, int b)
+{
+    ClipView v;
+    if (a.srcOff) {
+        v.L = max(ld_const(a.lengths, b), 0);
+        v.x = a.src + ld_const(a.srcOff, b);
+    } else {
+        int start;
+        v.L = row_region(a.begin, a.lengths, b, a.srcStride, start);   // never read past the clip's row
+        v.x = a.src + static_cast<size_t>(b) * a.srcStride + start;
+    }
+    return clip_view_out(a, v, b);
 }
 
 // Address of mel row m, frame f of a clip's output.
@@ -202,11 +221,12 @@ __device__ __forceinline__ float db_exact(float v, const LogMelArgs &a)
 
 // ------------------------------------------------------------------------- planner
 // Single block: tiles per clip = ceil(F_b / FT) with F_b = min(frames(L_b), hStride); exclusive
-// prefix sum into tileStart[0..B], total in tileStart[B].
+// prefix sum into tileStart[0..B], total in tileStart[B]; with tileRec (strided layout only), one
+// record per tile (clip, first frame, region length, region start).
 __global__ void __launch_bounds__(1024)
 plan_tiles_kernel(const int *__restrict__ lengths, const int *__restrict__ begin, int B, uint32_t srcStride,
                   uint32_t hStride, int W, int H, int center, int FT, int packed, int *__restrict__ tileStart,
-                  int *__restrict__ clipMax)
+                  int *__restrict__ clipMax, int4 *__restrict__ tileRec)
 {
     if (clipMax)  // the fused kernel raises these with atomicMax (float bits; stored power is never below -FLT_MAX)
         for (int b = threadIdx.x; b < B; b += blockDim.x) clipMax[b] = __float_as_int(-3.40282347e38f);
@@ -217,10 +237,9 @@ plan_tiles_kernel(const int *__restrict__ lengths, const int *__restrict__ begin
     __syncthreads();
     for (int base = 0; base < B; base += 1024) {
         const int b = base + tid;
-        int n = 0;
+        int n = 0, L = 0, start = 0;
         if (b < B) {
-            int start;
-            int L = packed ? lengths[b] : row_region(begin, lengths, b, srcStride, start);
+            L = packed ? lengths[b] : row_region(begin, lengths, b, srcStride, start);
             int F = max(0, num_windows(max(L, 0), W, H, center != 0));
             if (!packed) F = min(F, static_cast<int>(hStride));
             n = (F + FT - 1) / FT;
@@ -234,7 +253,12 @@ plan_tiles_kernel(const int *__restrict__ lengths, const int *__restrict__ begin
         __syncthreads();
         int before = 0;
         for (int w = 0; w < wave; w++) before += waveSum[w];
-        if (b < B) tileStart[b] = carry + before + v - n;
+        if (b < B) {
+            const int ts = carry + before + v - n;
+            tileStart[b] = ts;
+            if (tileRec)
+                for (int k = 0; k < n; k++) tileRec[ts + k] = make_int4(b, k * FT, L, start);
+        }
         __syncthreads();
         if (tid == 1023) carry += before + v;
         __syncthreads();
@@ -499,9 +523,20 @@ struct TileInfo {
 __device__ __forceinline__ TileInfo tile_info(const LogMelArgs &a, int tile, int FT, int span, int lane)
 {
     TileInfo t;
-    const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
-    t.v = clip_view(a, b);
-    t.f0 = (tile - a.tileStart[b]) * FT;
+    if (a.tileRec) {
+        // one uniform 16-byte load instead of the search and the clip's own loads
+        typedef int i32x4 __attribute__((ext_vector_type(4)));
+        const i32x4 r = ld_const(reinterpret_cast<const i32x4 *>(a.tileRec), tile);
+        ClipView v;
+        v.L = r.z;
+        v.x = a.src + static_cast<size_t>(r.x) * a.srcStride + r.w;
+        t.v = clip_view_out(a, v, r.x);
+        t.f0 = r.y;
+    } else {
+        const int b = owner_clip_wave(a.tileStart, a.B, tile, lane);
+        t.v = clip_view(a, b);
+        t.f0 = (tile - ld_const(a.tileStart, b)) * FT;
+    }
     t.s0 = t.f0 * a.H - (a.center ? a.W / 2 : 0);
     const int nfr = max(0, t.v.F - t.f0);
     t.valid = (nfr > 0 && t.v.L > 0) ? min(span, (nfr - 1) * a.H + a.W) : 0;
@@ -536,14 +571,19 @@ __device__ __forceinline__ void load_group(const TileInfo &t, int i, float4 &x4,
     prev = x[-1];
 }
 
-// Fresh (zero) contents: the registers of the previous prefetch must not stay live across a tile
-// whose successor loads nothing.
+// Fresh contents: the registers of the previous prefetch must not stay live across a tile whose
+// successor loads nothing.
 __device__ __forceinline__ void seg_prefetch_none(SegPrefetch &p)
 {
 #pragma unroll
     for (int q = 0; q < kPF; q++) {
-        p.v[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        p.prev[q] = 0.0f;
+        // an empty asm statement defines the registers without an instruction (zero-filling them
+        // would cost 55 moves per tile; the values are never read)
+        float4 u;
+        float w;
+        __asm__("" : "=v"(u.x), "=v"(u.y), "=v"(u.z), "=v"(u.w), "=v"(w));
+        p.v[q] = u;
+        p.prev[q] = w;
     }
 }
 
@@ -601,7 +641,9 @@ __device__ __forceinline__ float seg_store(float *seg, const SegPrefetch &p, con
         const int g = tid + q * nt, i = 4 * g;
         if (g < groups && interior_group(t, i)) put(i, pe_group(p.v[q], p.prev[q], a.coeff));
     }
-    for (int g = tid, q = 0; g < groups; g += nt, q++) {  // edge groups, and rounds past the prefetch
+    // edge groups, and rounds past the prefetch (none when every group is interior and prefetched)
+    const bool allPrefetched = t.iLo == 0 && t.iHi >= 4 * (groups - 1) && groups <= kPF * nt;
+    for (int g = allPrefetched ? groups : tid, q = 0; g < groups; g += nt, q++) {
         const int i = 4 * g;
         const bool in = interior_group(t, i);
         if (in && q < kPF) continue;
@@ -976,7 +1018,8 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         __syncthreads();
         mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         float segScale = 1.0f;
-        if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, 9 - ilogbf(mx));  // 2*max*scale < 2^11
+        // 2*max*scale < 2^11 (the exponent capped so the scale stays finite for near-zero segments)
+        if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, min(9 - ilogbf(mx), 126));
 
         // ---- folded DFT
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
@@ -1085,7 +1128,8 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
         uint16_t *pwL = pwH + FT * pws2;
-        const float unscale = (PREC == kPrecFP16x3) ? 1.0f / (segScale * a.basisScale) : 1.0f;
+        // 1 / (segScale * basis scale), both powers of two: exact without a division
+        const float unscale = (PREC == kPrecFP16x3) ? ldexpf(a.basisInv, -ilogbf(segScale)) : 1.0f;
         {
             const int c0 = wave * U / NRT, r0 = wave * U % NRT;
             const float u2 = unscale * unscale;   // an exact power of two for fp16x3
@@ -1104,13 +1148,16 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
             put_power(pwH, pwL, tid * pws2 + Q, p);
         }
         {
-            // positions the mel product reads that hold no bin: Q+1..Q+3 and N/2+4 onwards
+            // positions the mel product reads that hold no bin: Q+1..Q+3 and N/2+4 onwards (31 of
+            // them when nfft is a multiple of 128), 32 threads per frame
             const int nz = 3 + a.KP - (halfN + 4);
-            for (int e = tid; e < FT * nz; e += NW * 64) {
-                const int f = e / nz, k = e % nz;
-                const int pos = k < 3 ? Q + 1 + k : halfN + 4 + (k - 3);
-                pwH[f * pws2 + pos] = 0;
-                pwL[f * pws2 + pos] = 0;
+            for (int e = tid; e < FT * 32; e += NW * 64) {
+                const int f = e >> 5;
+                for (int k = e & 31; k < nz; k += 32) {
+                    const int pos = k < 3 ? Q + 1 + k : halfN + 1 + k;
+                    pwH[f * pws2 + pos] = 0;
+                    pwL[f * pws2 + pos] = 0;
+                }
             }
         }
         __syncthreads();
@@ -1260,8 +1307,26 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
                                          p->numFilter, p->normalize ? 1 : 0);
     if (!dt || !mt) return RPP_ERROR;
-    int *tileStart = static_cast<int *>(workspace(h, sizeof(int) * (2 * B + 1)));   // + per-clip maxima
-    if (!tileStart) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+    // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
+    // tiles each, 64 frames per tile up to nfft 512 and 32 above
+    const int CT = nfft / 256;
+    const int FT = mfma ? (CT <= 2 ? 64 : 32) : kFT32;
+    // per-tile records for the MFMA kernel on the strided layout, when their count is bounded by
+    // the row stride and the frame cap (every clip has at most ceil(maxFrames / FT) tiles)
+    size_t recBytes = 0;
+    if (mfma && !srcOff) {
+        const long long maxF = std::min<long long>(
+            std::max(num_windows(static_cast<int>(std::min<uint32_t>(srcStride, 0x7fffffffu)), p->windowLength,
+                                 p->windowStep, p->centerWindows), 0),
+            frameCap);
+        const size_t n = static_cast<size_t>(B) * static_cast<size_t>((maxF + FT - 1) / FT);
+        if (n * sizeof(int4) <= (static_cast<size_t>(256) << 20)) recBytes = n * sizeof(int4);
+    }
+    const size_t recOff = (sizeof(int) * (2 * B + 1) + 15) / 16 * 16;
+    char *ws = static_cast<char *>(workspace(h, recOff + recBytes));
+    if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+    int *tileStart = reinterpret_cast<int *>(ws);   // [B + 1], then the per-clip maxima [B]
+    int4 *tileRec = recBytes ? reinterpret_cast<int4 *>(ws + recOff) : nullptr;
     const bool clipRef = p->referenceMagnitude == 0.0f;
 
     LogMelArgs a{};
@@ -1295,13 +1360,14 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.frameCap = frameCap;
     a.clipMax = clipRef ? tileStart + B + 1 : nullptr;
     a.tileStart = tileStart;
+    a.tileRec = tileRec;
     a.basis = dt->basis;
     a.cols = dt->cols;
     a.bHi = dt->foldHi;
     a.bLo = dt->foldLo;
     a.wHalf = dt->wHalf;
     a.Wp = dt->Jp;
-    a.basisScale = splitKind == 1 ? 4096.0f : 1.0f;
+    a.basisInv = splitKind == 1 ? 1.0f / 4096.0f : 1.0f;
     a.weightsDown = mt->weightsDown;
     a.intervals = mt->intervals;
     a.normFactors = mt->normFactors;
@@ -1312,15 +1378,11 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
-    // tiles each, 64 frames per tile up to nfft 512 and 32 above
-    const int CT = nfft / 256;
-    const int FT = mfma ? (CT <= 2 ? 64 : 32) : kFT32;
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
 
     hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, begin, a.B, a.srcStride,
-                       a.frameCap, a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax);
+                       a.frameCap, a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax, tileRec);
     const int pws = a.KP + kSkew;
     if (!mfma) {
         const int span = (kFT32 - 1) * a.H + a.W;
