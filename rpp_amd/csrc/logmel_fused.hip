@@ -581,7 +581,7 @@ __device__ __forceinline__ void seg_prefetch_none(SegPrefetch &p)
         // would cost 55 moves per tile; the values are never read)
         float4 u;
         float w;
-        __asm__("" : "=v"(u.x), "=v"(u.y), "=v"(u.z), "=v"(u.w), "=v"(w));
+        __asm__ volatile("" : "=v"(u.x), "=v"(u.y), "=v"(u.z), "=v"(u.w), "=v"(w));
         p.v[q] = u;
         p.prev[q] = w;
     }
@@ -1168,12 +1168,14 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         //      entry, never used), so each slot stays in one set of registers and a product waits
         //      only for its own fragments.
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
-        const int n = melSeqLen[wave];
+        // the work list is the wave's own: its length and entries are wave-uniform (scalar loop
+        // control and item-end branches)
+        const int n = __builtin_amdgcn_readfirstlane(melSeqLen[wave]);
         const uint16_t *seq = melSeq[wave];
         u16x8 wh[kWA], wl[kWA];
         int ent[kWA];
         auto loadW = [&](int slot, int e) {
-            const int en = seq[min(e, n - 1)], mt = en >> 9, k0 = (en & 255) * 32;
+            const int en = __builtin_amdgcn_readfirstlane(seq[min(e, n - 1)]), mt = en >> 9, k0 = (en & 255) * 32;
             ent[slot] = en;
             const size_t o = (static_cast<size_t>(k0 / 8 + (lane >> 4)) * a.Mpad + mt * 16 + (lane & 15)) * 8;
             wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
