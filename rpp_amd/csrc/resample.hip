@@ -11,7 +11,7 @@
 //
 // Work split: one wave per 256-output block; lane l owns outputs 4l..4l+3 of the block and
 // reaches its first input position by the same sequence of FP32 additions the HOST loop makes.
-// The table sits in LDS (its reads are data-dependent gathers).  For mono sources (and the fused
+// The table sits in LDS for the general path (its reads are data-dependent gathers).  For mono sources (and the fused
 // down-mix, whose multi-channel input is reduced to mono first) with at most kMaxScale inputs per
 // output, each wave first stages the input span its block reads into LDS.  When in/out is an
 // integer k, every input position of a block is an integer, so each output whose taps are not
@@ -27,7 +27,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBlockOut = 256;
-constexpr int kMaxScale = 8;   // inputs per output up to which a block's input span is staged
+constexpr int kMaxScale = 4;   // inputs per output up to which a block's input span is staged (sized so four
+                               // blocks share a CU)
 constexpr int kMaxTaps = 64;   // 2 * lobes up to which the span is staged
 constexpr int kSpan = kBlockOut * kMaxScale + kMaxTaps + 8;   // staged floats per wave
 
@@ -147,8 +148,6 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     float *lut = lds;
     float *wts = lds + (lookupSize + 3) / 4 * 4;          // FIR taps of an unclipped output
     float *span = wts + kMaxTaps;                          // [wave][kSpan] staged input
-    for (int i = threadIdx.x; i < lookupSize; i += kThreads) lut[i] = lutG[i];
-    const Win win{lut, wScale, wCenter, lookupSize};
     const int b = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int C = max(dims[2 * b + 1], 1);
@@ -160,7 +159,6 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     const float dw = DMIX ? down_mix_weight(C, normalize) : 1.0f;
     auto sample = [&](int t) { return DMIX ? down_mix_sample(x + static_cast<size_t>(t) * C, C, dw) : x[t]; };
     const long long ob = (static_cast<long long>(blockIdx.x) * (kThreads / kWave) + wave) * kBlockOut;
-    __syncthreads();                                       // table staged
     if (ir == orr) {  // no resampling: a copy of L*C samples (or of the L down-mixed samples)
         const long long n = min(static_cast<long long>(L), static_cast<long long>(dstStride / CO)) * CO;
         for (long long i = ob * CO + lane; i < min(n, (ob + kBlockOut) * CO); i += kWave)
@@ -176,6 +174,12 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     // block-uniform choices (one clip per block)
     const bool staged = (DMIX || C == 1) && scale <= kMaxScale && 2 * lobes <= kMaxTaps;
     const bool fir = staged && static_cast<double>(kInt) == scale;
+    // the table goes to LDS for the per-tap lookups of the general path; an integer-ratio block
+    // looks up its 2*lobes window values once and reads the table in global memory only for the
+    // outputs clipped at the clip's ends (the same values either way)
+    if (!fir)
+        for (int i = threadIdx.x; i < lookupSize; i += kThreads) lut[i] = lutG[i];
+    const Win win{fir ? lutG : lut, wScale, wCenter, lookupSize};
     const int nGroups = (2 * lobes) / 4;
     if (fir && threadIdx.x < 2 * lobes) {
         const float p = static_cast<float>(static_cast<int>(threadIdx.x) - lobes);
@@ -222,7 +226,7 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
             }
         }
     }
-    __syncthreads();                                       // taps and spans staged
+    __syncthreads();                                       // table, taps and spans staged
     if (ob >= outEnd) return;
     if (fir) {
         const float *spB = span + wave * kSpan - s0;
