@@ -146,65 +146,76 @@ struct MelScale {
     }
 };
 
-// Single-thread table build following the reference's loop exactly
-// (cpu/kernel/mel_filter_bank.cpp:55-112): FP64 mel points accumulated by repeated addition,
-// FP32 scale functions, FP64 slopes rounded to FP32 weights.
-__global__ void mel_tables_kernel(int numBins, float sampleRate, float maxFreqVal, float minFreq, int formula,
-                                  int M, int normalize, float *weightsDown, int *intervals, float *normFactors,
-                                  int *filterBins, float *dense, int *binRange)
+// Table build.  The mel points and per-bin weights follow the reference's loop exactly on one
+// thread (cpu/kernel/mel_filter_bank.cpp:55-112): FP64 mel points accumulated by repeated
+// addition, FP32 scale functions, FP64 slopes rounded to FP32 weights.  The initialisation, the
+// per-filter bin ranges and the dense matrix are independent per element and run over the block.
+__global__ void __launch_bounds__(256)
+mel_tables_kernel(int numBins, float sampleRate, float maxFreqVal, float minFreq, int formula, int M, int normalize,
+                  float *weightsDown, int *intervals, float *normFactors, int *filterBins, float *dense,
+                  int *binRange)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    MelScale ms{formula};
-    float maxFreq = (maxFreqVal == 0) ? sampleRate / 2 : maxFreqVal;
-    double melLow = ms.hz_to_mel(minFreq);
-    double melHigh = ms.hz_to_mel(maxFreq);
-    double melStep = (melHigh - melLow) / (M + 1);
-    int nfft = (numBins - 1) * 2;
-    double hzStep = static_cast<double>(sampleRate) / nfft;
-    double invHzStep = 1.0 / hzStep;
-    int binStart = static_cast<int>(ceil(minFreq * invHzStep));
-    int binEnd = static_cast<int>(ceil(maxFreq * invHzStep));
-    if (binEnd > numBins) binEnd = numBins;
-    for (int m = 0; m < M; m++) normFactors[m] = 1.0f;
-    for (int k = 0; k < numBins; k++) { weightsDown[k] = 0.0f; intervals[k] = -1; }
-    int bin = binStart;
-    double mel0 = melLow, mel1 = melLow + melStep;
-    double fIter = bin * hzStep;
-    for (int iv = 0; iv < M + 1; iv++, mel0 = mel1, mel1 += melStep) {
-        double f0 = ms.mel_to_hz(static_cast<float>(mel0));
-        double f1 = ms.mel_to_hz(static_cast<float>(iv == M ? melHigh : mel1));
-        double slope = 1. / (f1 - f0);
-        if (normalize && iv < M) {
-            double f2 = ms.mel_to_hz(static_cast<float>(mel1 + melStep));
-            normFactors[iv] = static_cast<float>(2.0 / (f2 - f0));
-        }
-        for (; bin < binEnd && fIter < f1; bin++, fIter = bin * hzStep) {
-            weightsDown[bin] = static_cast<float>((f1 - fIter) * slope);
-            intervals[bin] = iv;
-        }
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ int range[2];
+    for (int m = tid; m < M; m += nt) {
+        normFactors[m] = 1.0f;
+        filterBins[2 * m] = 0x7fffffff;   // first bin (min), then last bin + 1 (max)
+        filterBins[2 * m + 1] = 0;
     }
-    binRange[0] = binStart;
-    binRange[1] = binEnd;
-    // gather ranges and the dense normalised matrix
-    for (int m = 0; m < M; m++) { filterBins[2 * m] = 0; filterBins[2 * m + 1] = 0; }
-    for (int m = 0; m < M; m++) {
-        int first = -1, last = -1;
-        for (int k = binStart; k < binEnd; k++) {
-            int up = intervals[k];
-            if (up == m || up - 1 == m) {
-                if (first < 0) first = k;
-                last = k;
+    for (int k = tid; k < numBins; k += nt) { weightsDown[k] = 0.0f; intervals[k] = -1; }
+    for (int i = tid; i < M * numBins; i += nt) dense[i] = 0.0f;
+    __syncthreads();
+    if (tid == 0) {
+        MelScale ms{formula};
+        float maxFreq = (maxFreqVal == 0) ? sampleRate / 2 : maxFreqVal;
+        double melLow = ms.hz_to_mel(minFreq);
+        double melHigh = ms.hz_to_mel(maxFreq);
+        double melStep = (melHigh - melLow) / (M + 1);
+        int nfft = (numBins - 1) * 2;
+        double hzStep = static_cast<double>(sampleRate) / nfft;
+        double invHzStep = 1.0 / hzStep;
+        int binStart = static_cast<int>(ceil(minFreq * invHzStep));
+        int binEnd = static_cast<int>(ceil(maxFreq * invHzStep));
+        if (binEnd > numBins) binEnd = numBins;
+        int bin = binStart;
+        double mel0 = melLow, mel1 = melLow + melStep;
+        double fIter = bin * hzStep;
+        for (int iv = 0; iv < M + 1; iv++, mel0 = mel1, mel1 += melStep) {
+            double f0 = ms.mel_to_hz(static_cast<float>(mel0));
+            double f1 = ms.mel_to_hz(static_cast<float>(iv == M ? melHigh : mel1));
+            double slope = 1. / (f1 - f0);
+            if (normalize && iv < M) {
+                double f2 = ms.mel_to_hz(static_cast<float>(mel1 + melStep));
+                normFactors[iv] = static_cast<float>(2.0 / (f2 - f0));
+            }
+            for (; bin < binEnd && fIter < f1; bin++, fIter = bin * hzStep) {
+                weightsDown[bin] = static_cast<float>((f1 - fIter) * slope);
+                intervals[bin] = iv;
             }
         }
-        if (first >= 0) { filterBins[2 * m] = first; filterBins[2 * m + 1] = last + 1; }
+        binRange[0] = range[0] = binStart;
+        binRange[1] = range[1] = binEnd;
     }
-    for (int i = 0; i < M * numBins; i++) dense[i] = 0.0f;
-    for (int k = binStart; k < binEnd; k++) {
-        int up = intervals[k], down = up - 1;
-        float wDown = weightsDown[k], wUp = 1.0f - weightsDown[k];
-        if (down >= 0) dense[down * numBins + k] = normalize ? wDown * normFactors[down] : wDown;
-        if (up >= 0 && up < M) dense[up * numBins + k] = normalize ? wUp * normFactors[up] : wUp;
+    __syncthreads();
+    // bin k feeds filter `up` with weight 1 - weightsDown[k] and filter up - 1 with weightsDown[k];
+    // each filter's bin range is [first, last + 1) over the bins that feed it, [0, 0) if none
+    for (int k = range[0] + tid; k < range[1]; k += nt) {
+        const int up = intervals[k], down = up - 1;
+        const float wDown = weightsDown[k], wUp = 1.0f - weightsDown[k];
+        if (down >= 0 && down < M) {
+            dense[down * numBins + k] = normalize ? wDown * normFactors[down] : wDown;
+            atomicMin(&filterBins[2 * down], k);
+            atomicMax(&filterBins[2 * down + 1], k + 1);
+        }
+        if (up >= 0 && up < M) {
+            dense[up * numBins + k] = normalize ? wUp * normFactors[up] : wUp;
+            atomicMin(&filterBins[2 * up], k);
+            atomicMax(&filterBins[2 * up + 1], k + 1);
+        }
     }
+    __syncthreads();
+    for (int m = tid; m < M; m += nt)
+        if (filterBins[2 * m] == 0x7fffffff) filterBins[2 * m] = 0;
 }
 
 uint64_t fnv1a(const void *p, size_t n)
@@ -288,7 +299,7 @@ const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, flo
         hipMalloc(&t.dense, sizeof(float) * static_cast<size_t>(M) * numBins) != hipSuccess ||
         hipMalloc(&range, sizeof(int) * 2) != hipSuccess)
         return nullptr;
-    hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(64), 0, h->stream, numBins, sampleRate, maxFreq, minFreq,
+    hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(256), 0, h->stream, numBins, sampleRate, maxFreq, minFreq,
                        formula, M, normalize, t.weightsDown, t.intervals, t.normFactors, t.filterBins, t.dense, range);
     t.KP = (numBins + 31) / 32 * 32;
     t.Mpad = (M + 15) / 16 * 16;
