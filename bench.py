@@ -91,6 +91,50 @@ def front_end_measurement(api, synth, torch, precision, batch=16, seconds=60.0, 
             "audio_s_per_s": round(audio * steps / wall, 1)}
 
 
+def workload_measurement(api, synth, torch, workload, precision, steps=20, settle_seconds=0.1):
+    """Fused launch (planner + kernel, hipGraph) on another workload's batch: audio-s/s and the
+    kernel's MFMA fraction by the same algorithmic count as the headline (events over the steps)."""
+    bt = synth.make(workload)
+    f = bt.params
+    x = torch.from_numpy(bt.x).cuda()
+    L = torch.from_numpy(bt.lengths).cuda()
+    frames = bt.lengths // f["H"] + 1
+    out = torch.empty((len(bt.lengths), f["nmel"], int(frames.max())), dtype=torch.float32, device="cuda")
+    p = api.logmel_params(nfft=f["nfft"], window_length=f["W"], window_step=f["H"], sample_rate=f["sample_rate"],
+                          min_freq=f["fmin"], max_freq=f["fmax"], num_filter=f["nmel"])
+    stream = torch.cuda.Stream()
+    h = api.Handle(len(bt.lengths), stream=stream)
+    with torch.cuda.stream(stream):
+        api.logmel_fused(h, x, out, L, p, precision)
+        tiles, _ = api.autotune(h, x, out, L, p, precision, iters=3)
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            api.logmel_fused(h, x, out, L, p, precision)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < settle_seconds:
+        g.replay()
+        torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        for _ in range(steps):
+            g.replay()
+        ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_s = ev0.elapsed_time(ev1) / 1e3 / steps
+    K = f["nfft"] // 2 + 1
+    flops = float(frames.sum()) * (2 * f["W"] * 2 * K + 2 * K * f["nmel"])
+    h.close()
+    return {"batch": len(bt.lengths), "audio_seconds": round(bt.audio_seconds, 1), "frames": int(frames.sum()),
+            "nfft": f["nfft"], "window": f["W"], "hop": f["H"], "mels": f["nmel"],
+            "tiles": {"frames_per_tile": tiles[0], "blocks_per_cu": tiles[1]},
+            "ms_per_step": round(wall / steps * 1e3, 4), "audio_s_per_s": round(bt.audio_seconds * steps / wall, 1),
+            "mfma_frac": round(flops / step_s / 1e12 / MI355X_PEAK_TFLOPS[precision], 4)}
+
+
 def serve_measurement(api, synth, params, precision, seconds, rate=4000.0):
     """Served stream: Poisson arrivals at `rate` clips/s, lengths log-uniform 0.25-30 s at 16 kHz,
     through the continuous-batching server (2 ms batch deadline, 4 device slots, admission by
@@ -300,6 +344,10 @@ def main() -> int:
     if not a.no_side:
         side["front_end_workload4"] = front_end_measurement(api, synth, torch, a.precision, steps=20)
         log(f"front end: {side['front_end_workload4']}")
+        for wl in (0, 2):
+            if wl != a.workload:
+                side[f"workload{wl}"] = workload_measurement(api, synth, torch, wl, a.precision)
+                log(f"workload {wl}: {side[f'workload{wl}']}")
     serving = None
     if not a.no_side and a.serve_seconds > 0:
         serving = serve_measurement(api, synth, params, a.precision, a.serve_seconds)
