@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBlockOut = 256;
-constexpr int kMaxScale = 4;   // inputs per output up to which a block's input span is staged (sized so four
+constexpr int kMaxScale = 3;   // inputs per output up to which a block's input span is staged (sized so five
                                // blocks share a CU)
 constexpr int kMaxTaps = 64;   // 2 * lobes up to which the span is staged
 constexpr int kSpan = kBlockOut * kMaxScale + kMaxTaps + 8;   // staged floats per wave
