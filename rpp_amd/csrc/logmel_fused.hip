@@ -692,13 +692,16 @@ template <int PREC>
 __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi, uint32_t &lo)
 {
     if (PREC == kPrecFP16x3) {
-        // packed conversions (one round-to-nearest instruction per pair)
-        const float u0 = v0 * s, u1 = v1 * s;
-        const f16x2 h = {static_cast<_Float16>(u0), static_cast<_Float16>(u1)};
-        const f16x2 l = {static_cast<_Float16>(u0 - static_cast<float>(h.x)),
-                         static_cast<_Float16>(u1 - static_cast<float>(h.y))};
-        hi = __builtin_bit_cast(uint32_t, h);
-        lo = __builtin_bit_cast(uint32_t, l);
+        // one mixed-precision FMA per half: hi = f16(v*s), lo = f16(v*s - hi).  s is a power of
+        // two, so v*s is exact in FP32 and so is v*s - hi (it has at most 13 significant bits):
+        // each result is rounded once, to nearest even, as the conversions round it
+        uint32_t h, l;
+        __asm__("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(h) : "v"(v0), "v"(s));
+        __asm__("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(h) : "v"(v1), "v"(s));
+        __asm__("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(l) : "v"(v0), "v"(s), "v"(h));
+        __asm__("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(l) : "v"(v1), "v"(s), "v"(h));
+        hi = h;
+        lo = l;
     } else {
         const __bf16 h0 = static_cast<__bf16>(v0), h1 = static_cast<__bf16>(v1);
         const __bf16 l0 = static_cast<__bf16>(v0 - static_cast<float>(h0));
