@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 # revision of the fused MFMA kernel; recorded traffic (profiles/*.md `traffic:` lines) applies only to
 # the revision it was measured on
-KERNEL_REVISION = "mirror-il2-rec-mix"
+KERNEL_REVISION = "mirror-il2-lean"
 MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 MI355X_HBM_GBS = 8000.0
 

@@ -857,29 +857,31 @@ __device__ __forceinline__ u16x8 flip_odd(const u16x8 v)
 // bf16 hi and lo of four power values as two 8-byte plane stores.
 __device__ __forceinline__ void put_power4(uint16_t *pwH, uint16_t *pwL, int idx, const float (&p)[4])
 {
-    uint16_t h[4], l[4];
+    // one packed conversion per pair for hi; the pair's FP32 values of hi are its low half shifted
+    // up and its high half masked (no second conversion), then one packed conversion for lo
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    uint32_t h[2], l[2];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const __bf16 hb = static_cast<__bf16>(p[k]);
-        const __bf16 lb = static_cast<__bf16>(p[k] - static_cast<float>(hb));
-        h[k] = __builtin_bit_cast(uint16_t, hb);
-        l[k] = __builtin_bit_cast(uint16_t, lb);
+    for (int k = 0; k < 2; k++) {
+        const uint32_t hp = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{p[2 * k], p[2 * k + 1]}, bf16x2));
+        const float h0 = __uint_as_float(hp << 16), h1 = __uint_as_float(hp & 0xffff0000u);
+        h[k] = hp;
+        l[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{p[2 * k] - h0, p[2 * k + 1] - h1}, bf16x2));
     }
-    *reinterpret_cast<uint2 *>(&pwH[idx]) =
-        make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
-    *reinterpret_cast<uint2 *>(&pwL[idx]) =
-        make_uint2(l[0] | (static_cast<uint32_t>(l[1]) << 16), l[2] | (static_cast<uint32_t>(l[3]) << 16));
+    *reinterpret_cast<uint2 *>(&pwH[idx]) = make_uint2(h[0], h[1]);
+    *reinterpret_cast<uint2 *>(&pwL[idx]) = make_uint2(l[0], l[1]);
 }
 
-// Power (|X| when SQRT) of the wave's units into the planes (unscale2: the square of the operand
-// scaling to undo).  The DFT accumulators hold X^T: row =
+// Power (|X| when SQRT) of the wave's units into the planes, in the scaled domain of the operands
+// (the mel output undoes the scaling).  The DFT accumulators hold X^T: row =
 // bin, column = frame, so a lane owns frame rt*32 + lane%32 and, per group of four registers,
 // direct bins k0 .. k0+3 (k0 = ct*32 + 8q + 4(lane/32)): acc 0/1 give bin k (cos x e, sin x o),
 // acc 2/3 bin N/2 - k (odd-negated sin x e, cos x o).  Bin k sits at plane position k, bin
 // N/2 - k at N/2 + 3 - k, so both groups are 8-byte aligned runs (the mirror run reversed).
 template <bool SQRT, int RTW, int CTM>
 __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pws2, const f32x16 (&acc)[RTW][CTM][4],
-                                            float unscale2, int ct0, int rt0, int nct, int halfN, int lane)
+                                            int ct0, int rt0, int nct, int halfN, int lane)
 {
 #pragma unroll
     for (int r = 0; r < RTW; r++) {
@@ -895,7 +897,7 @@ __device__ __forceinline__ void store_power(uint16_t *pwH, uint16_t *pwL, int pw
                 for (int i = 0; i < 4; i++) {
                     const float c = acc[r][ct][0][4 * q + i], sn = acc[r][ct][1][4 * q + i];
                     const float mc = acc[r][ct][2][4 * q + i], ms = acc[r][ct][3][4 * q + i];
-                    float d = fmaf(sn, sn, c * c) * unscale2, m = fmaf(ms, ms, mc * mc) * unscale2;
+                    float d = fmaf(sn, sn, c * c), m = fmaf(ms, ms, mc * mc);
                     if (SQRT) { d = sqrtf(d); m = sqrtf(m); }
                     pd[i] = d;
                     pm[3 - i] = m;
@@ -1021,8 +1023,9 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         __syncthreads();
         mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         float segScale = 1.0f;
-        // 2*max*scale < 2^11 (the exponent capped so the scale stays finite for near-zero segments)
-        if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, min(9 - ilogbf(mx), 126));
+        // 2*max*scale < 2^11 (the exponent capped so segScale * basis scale stays finite for
+        // near-zero segments)
+        if (PREC == kPrecFP16x3 && mx > 0.0f) segScale = ldexpf(1.0f, min(9 - ilogbf(mx), 114));
 
         // ---- folded DFT
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
@@ -1131,35 +1134,47 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
         uint16_t *pwH = reinterpret_cast<uint16_t *>(lds);
         uint16_t *pwL = pwH + FT * pws2;
-        // 1 / (segScale * basis scale), both powers of two: exact without a division
-        const float unscale = (PREC == kPrecFP16x3) ? ldexpf(a.basisInv, -ilogbf(segScale)) : 1.0f;
+        // The planes hold the power of the scaled operands (segScale * basis scale, a power of two
+        // for fp16x3, 1 otherwise); the mel output is multiplied by melScale = 1 / that scale (its
+        // square for power 2).  Scaling by a power of two is exact, so this is the value the
+        // unscaled power would give, with the multiply once per mel output instead of once per bin.
+        const int sExp = (PREC == kPrecFP16x3) ? ilogbf(segScale) - ilogbf(a.basisInv) : 0;
+        const float unscale = ldexpf(1.0f, -sExp);
+        const float melScale = a.power == 1 ? unscale : unscale * unscale;
         {
             const int c0 = wave * U / NRT, r0 = wave * U % NRT;
-            const float u2 = unscale * unscale;   // an exact power of two for fp16x3
-            if (a.power == 1) store_power<true, RTW, CTM>(pwH, pwL, pws2, acc, u2, c0, r0, a.N / 128, halfN, lane);
-            else store_power<false, RTW, CTM>(pwH, pwL, pws2, acc, u2, c0, r0, a.N / 128, halfN, lane);
+            if (a.power == 1) store_power<true, RTW, CTM>(pwH, pwL, pws2, acc, c0, r0, a.N / 128, halfN, lane);
+            else store_power<false, RTW, CTM>(pwH, pwL, pws2, acc, c0, r0, a.N / 128, halfN, lane);
         }
         const int Q = a.N / 4;
         if (tid < FT) {
+            // bin N/4 from sums of the unscaled samples: brought to the scaled domain before squaring
             float re = 0.0f, im = 0.0f;
             for (int q = tid; q < NW * 64; q += FT) {
                 re += q4Part[q].x;
                 im += q4Part[q].y;
             }
+            re = ldexpf(re, sExp);
+            im = ldexpf(im, sExp);
             float p = 0.5f * (re * re + im * im);
             if (a.power == 1) p = sqrtf(p);
             put_power(pwH, pwL, tid * pws2 + Q, p);
         }
         {
-            // positions the mel product reads that hold no bin: Q+1..Q+3 and N/2+4 onwards (31 of
-            // them when nfft is a multiple of 128), 32 threads per frame
-            const int nz = 3 + a.KP - (halfN + 4);
-            for (int e = tid; e < FT * 32; e += NW * 64) {
-                const int f = e >> 5;
-                for (int k = e & 31; k < nz; k += 32) {
-                    const int pos = k < 3 ? Q + 1 + k : halfN + 1 + k;
-                    pwH[f * pws2 + pos] = 0;
-                    pwL[f * pws2 + pos] = 0;
+            // positions the mel product reads that hold no bin: Q+1..Q+3 (a 2-byte and a 4-byte
+            // store) and N/2+4 .. KP-1 (8-byte stores: N/2+4 and KP are multiples of 4), in both
+            // planes; 16 threads per frame
+            const int nc = (a.KP - (halfN + 4)) / 4;
+            for (int e = tid; e < FT * 16; e += NW * 64) {
+                const int row = (e >> 4) * pws2, c = e & 15;
+                if (c < nc) {
+                    *reinterpret_cast<uint2 *>(&pwH[row + halfN + 4 + 4 * c]) = make_uint2(0u, 0u);
+                    *reinterpret_cast<uint2 *>(&pwL[row + halfN + 4 + 4 * c]) = make_uint2(0u, 0u);
+                } else if (c == 15) {
+                    pwH[row + Q + 1] = 0;
+                    pwL[row + Q + 1] = 0;
+                    *reinterpret_cast<uint32_t *>(&pwH[row + Q + 2]) = 0u;
+                    *reinterpret_cast<uint32_t *>(&pwL[row + Q + 2]) = 0u;
                 }
             }
         }
@@ -1239,20 +1254,20 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
                             if (a.clipMax) {
 #pragma unroll
                                 for (int g = 0; g < FGI; g++) {
-                                    val[g] = acc[g];
-                                    melMax = fmaxf(melMax, fmaxf(fmaxf(acc[g][0], acc[g][1]), fmaxf(acc[g][2], acc[g][3])));
+                                    val[g] = acc[g] * melScale;
+                                    melMax = fmaxf(melMax, fmaxf(fmaxf(val[g][0], val[g][1]), fmaxf(val[g][2], val[g][3])));
                                 }
                             } else if (fastLog) {
 #pragma unroll
                                 for (int g = 0; g < FGI; g++)
 #pragma unroll
                                     for (int r = 0; r < 4; r++)
-                                        val[g][r] = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, acc[g][r] * a.invRef));
+                                        val[g][r] = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, (acc[g][r] * melScale) * a.invRef));
                             } else {
 #pragma unroll
                                 for (int g = 0; g < FGI; g++)
 #pragma unroll
-                                    for (int r = 0; r < 4; r++) val[g][r] = db_fast(acc[g][r], a);
+                                    for (int r = 0; r < 4; r++) val[g][r] = db_fast(acc[g][r] * melScale, a);
                             }
                             const int m0 = mt * 16 + 4 * (lane >> 4);
                             const size_t rStride = a.ntf ? 1 : v.hs;   // between consecutive rows m
