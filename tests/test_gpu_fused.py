@@ -268,3 +268,39 @@ def test_fused_ntf_layout(gpu, handle, precision):
     for b, F in enumerate(np.minimum(frames, cap)):
         assert np.array_equal(t[b, :F, :M], a[b, :, :F].T), b
         assert np.isnan(t[b, F:, :]).all() and np.isnan(t[b, :, M:]).all(), b
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "fp32"])
+def test_fused_power_of_two_scale_invariance(gpu, handle, precision):
+    """With the clip maximum as reference, scaling a clip by 2^-40 changes no output bit: the fp16
+    operands are normalised by a power of two per tile and the power is rescaled exactly (far from
+    FP32 underflow at this scale).  A silent clip and one of amplitude ~1e-36 (the operand scale at
+    its cap) give finite outputs at the -80 dB floor."""
+    bt = synth.workload0(batch=3)
+    L = bt.lengths.copy()
+    feat = bt.params
+    x = bt.x.copy()
+    xs = np.ldexp(x, -40).astype(np.float32)
+    p = gpu.logmel_params(nfft=feat["nfft"], window_length=feat["W"], window_step=feat["H"],
+                          sample_rate=feat["sample_rate"], min_freq=feat["fmin"], max_freq=feat["fmax"],
+                          num_filter=feat["nmel"], reference_magnitude=0.0)
+    frames = L // feat["H"] + 1
+    outs = []
+    for src in (x, xs):
+        out = torch.full((len(L), feat["nmel"], int(frames.max())), float("nan"), device="cuda")
+        gpu.logmel_fused(handle, dev(src), out, dev(L), p, precision)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    for b, F in enumerate(frames):
+        assert np.array_equal(outs[0][b, :, :F], outs[1][b, :, :F]), b
+    quiet = np.zeros_like(x)
+    quiet[1] = (x[1] * np.float32(1e-36)).astype(np.float32)
+    p1 = params_for(gpu, feat)
+    out = torch.full((len(L), feat["nmel"], int(frames.max())), float("nan"), device="cuda")
+    gpu.logmel_fused(handle, dev(quiet), out, dev(L), p1, precision)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    for b in (0, 1):
+        F = frames[b]
+        assert np.all(np.isfinite(o[b, :, :F])), b
+        assert np.all(o[b, :, :F] == o[b, :, :F].min()) and o[b, :, :F].min() <= -79.99, b
