@@ -57,6 +57,8 @@ struct LogMelArgs {
     const int *tileStart;  // [B + 1]
     const int4 *tileRec;   // [tiles] clip, first frame, region length, region start of each tile (null:
                            // the MFMA kernel searches tileStart instead)
+    int4 *blockRec;        // when set, the MFMA kernel plans in each block (no planner launch) and keeps
+    int kPad;              // block b's records at blockRec[b * kPad + k] (kPad a multiple of 16)
     // packed layout (both null for the [n][nStride] / [n][M][hStride] descriptors): clip b's samples
     // at src + srcOff[b], its dB rows at dst + dstOff[b] as a dense [M][frames_b] block
     const long long *srcOff, *dstOff;
@@ -523,7 +525,18 @@ struct TileInfo {
 __device__ __forceinline__ TileInfo tile_info(const LogMelArgs &a, int tile, int FT, int span, int lane)
 {
     TileInfo t;
-    if (a.tileRec) {
+    if (a.blockRec) {
+        // the block's own record k (written by this block at the start of the kernel; its 256-byte
+        // lines are read by no other block, so the scalar cache cannot hold them stale)
+        typedef int i32x4 __attribute__((ext_vector_type(4)));
+        const int k = (tile - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x);
+        const i32x4 r = ld_const(reinterpret_cast<const i32x4 *>(a.blockRec), static_cast<int>(blockIdx.x) * a.kPad + k);
+        ClipView v;
+        v.L = r.z;
+        v.x = a.src + static_cast<size_t>(r.x) * a.srcStride + r.w;
+        t.v = clip_view_out(a, v, r.x);
+        t.f0 = r.y;
+    } else if (a.tileRec) {
         // one uniform 16-byte load instead of the search and the clip's own loads
         typedef int i32x4 __attribute__((ext_vector_type(4)));
         const i32x4 r = ld_const(reinterpret_cast<const i32x4 *>(a.tileRec), tile);
@@ -916,6 +929,40 @@ constexpr int kMelSeq = 320;
 constexpr int kWA = 2;
 static_assert(kWA >= 2, "the tile-end test reads entry e + 1 from the next slot");
 
+// The planner's work inside a block: tiles per clip (plan_tiles_kernel's count) and their
+// exclusive prefix sum into ts[0..B] (ts[B] = total), ws[0..3] scratch; returns the total.  Strided
+// layout only.
+__device__ int plan_tiles_block(const LogMelArgs &a, int FT, int *ts, int *ws)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < a.B; base += kBlockMfma) {
+        const int b = base + tid;
+        int n = 0;
+        if (b < a.B) {
+            int start;
+            const int L = row_region(a.begin, a.lengths, b, a.srcStride, start);
+            const int F = min(max(0, num_windows(max(L, 0), a.W, a.H, a.center != 0)), static_cast<int>(a.frameCap));
+            n = (F + FT - 1) / FT;
+        }
+        int v = n;   // inclusive wave scan
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o);
+            if (lane >= o) v += u;
+        }
+        if (lane == 63) ws[wave] = v;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; w++) before += ws[w];
+        if (b < a.B) ts[b] = carry + before + v - n;
+        carry += ws[0] + ws[1] + ws[2] + ws[3];
+        __syncthreads();   // ws is reused by the next chunk
+    }
+    if (tid == 0) ts[a.B] = carry;
+    __syncthreads();
+    return carry;
+}
+
 // Fused MFMA kernel.  Block = 4 waves.  The DFT of a tile (FT frames = NRT row tiles of 32) runs
 // over N/128 column tiles of 32 direct bins k < N/4, each giving bins k and N/2 - k; the
 // (column tile, row tile) units are split evenly over the waves, CTM x RTW per wave, four 32x32
@@ -942,7 +989,6 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
     u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2 kSPB][FT] operand slices
     const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
-    const int T = a.tileStart[a.B];
     const int span = (FT - 1) * a.H + a.W;
     const int halfN = a.N / 2;
     const int nSteps = a.Wp / 16;
@@ -998,6 +1044,35 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
             load[w] += bestCost;
         }
         for (int q = 0; q < NW; q++) melSeqLen[q] = cnt[q];
+    }
+
+    // the tile plan: computed here, in the segment region (free until the first tile is staged),
+    // with this block's own tile records written out; or the planner's
+    int T;
+    if (a.blockRec) {
+        int *ts = reinterpret_cast<int *>(lds);
+        T = plan_tiles_block(a, FT, ts, ts + a.B + 1);
+        for (int k = threadIdx.x; k < a.kPad; k += kBlockMfma) {
+            const int tile = static_cast<int>(blockIdx.x) + k * static_cast<int>(gridDim.x);
+            if (tile < T) {
+                int lo = 0, hi = a.B;   // ts[lo] <= tile < ts[hi]: the largest such lo owns the tile
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (ts[mid] <= tile) lo = mid;
+                    else hi = mid;
+                }
+                int start;
+                const int L = row_region(a.begin, a.lengths, lo, a.srcStride, start);
+                a.blockRec[static_cast<size_t>(blockIdx.x) * a.kPad + k] = make_int4(lo, (tile - ts[lo]) * FT, L, start);
+            }
+        }
+        // the records are read back through the scalar cache, which misses to this XCD's L2 (the
+        // block's own): the stores must have reached it, and a workgroup barrier alone does not
+        // wait for them
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __syncthreads();   // records stored; the segment region is free again
+    } else {
+        T = ld_const(a.tileStart, a.B);
     }
 
     // the first tile's segment is fetched here; every later one while its predecessor's mel runs
@@ -1327,6 +1402,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
                                          p->numFilter, p->normalize ? 1 : 0);
     if (!dt || !mt) return RPP_ERROR;
+    const bool clipRef = p->referenceMagnitude == 0.0f;
     // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
     // tiles each, 64 frames per tile up to nfft 512 and 32 above
     const int CT = nfft / 256;
@@ -1342,12 +1418,26 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const size_t n = static_cast<size_t>(B) * static_cast<size_t>((maxF + FT - 1) / FT);
         if (n * sizeof(int4) <= (static_cast<size_t>(256) << 20)) recBytes = n * sizeof(int4);
     }
-    const size_t recOff = (sizeof(int) * (2 * B + 1) + 15) / 16 * 16;
-    char *ws = static_cast<char *>(workspace(h, recOff + recBytes));
+    // the MFMA kernel plans in each block (no planner launch, one dependent dispatch less) when the
+    // per-tile records are bounded, the clip count fits the segment region and no per-clip maximum
+    // needs initialising; each block's records take kPad slots, 256-byte aligned
+    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
+    const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
+    const bool selfPlan = recBytes > 0 && !clipRef && B <= 4096;
+    int kPad = 0;
+    size_t blockBytes = 0;
+    if (selfPlan) {
+        const size_t tiles = recBytes / sizeof(int4);
+        kPad = static_cast<int>(((tiles + grid - 1) / grid + 15) / 16 * 16);
+        blockBytes = static_cast<size_t>(grid) * kPad * sizeof(int4);
+        recBytes = 0;
+    }
+    const size_t recOff = (sizeof(int) * (2 * B + 1) + 255) / 256 * 256;
+    char *ws = static_cast<char *>(workspace(h, recOff + recBytes + blockBytes));
     if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
     int *tileStart = reinterpret_cast<int *>(ws);   // [B + 1], then the per-clip maxima [B]
     int4 *tileRec = recBytes ? reinterpret_cast<int4 *>(ws + recOff) : nullptr;
-    const bool clipRef = p->referenceMagnitude == 0.0f;
+    int4 *blockRec = blockBytes ? reinterpret_cast<int4 *>(ws + recOff) : nullptr;
 
     LogMelArgs a{};
     a.src = src;
@@ -1381,6 +1471,8 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.clipMax = clipRef ? tileStart + B + 1 : nullptr;
     a.tileStart = tileStart;
     a.tileRec = tileRec;
+    a.blockRec = blockRec;
+    a.kPad = kPad;
     a.basis = dt->basis;
     a.cols = dt->cols;
     a.bHi = dt->foldHi;
@@ -1398,10 +1490,8 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
-    const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
-
-    hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, begin, a.B, a.srcStride,
+    if (!selfPlan)
+        hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, begin, a.B, a.srcStride,
                        a.frameCap, a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax, tileRec);
     const int pws = a.KP + kSkew;
     if (!mfma) {
