@@ -8,7 +8,8 @@ frames W=400 hop 160 nfft 512, power 2, 80 Slaney mels 0-8 kHz normalised, dB (x
 Inputs are resident in HBM before timing starts.
 
 A step is one pass of the hot path over the whole batch: the fused persistent launch
-(rppt_logmel_fused_gpu, planner + kernel) replayed from a hipGraph.  Before the warmup steps the
+(rppt_logmel_fused_gpu: the persistent fused kernel, which plans its own tiles) replayed from a
+hipGraph.  Before the warmup steps the
 graph is replayed, untimed, for --settle-seconds so the timed steps see the clocks of sustained
 load (the first tens of milliseconds on an idle GPU run up to 20% slower).  Printed as one JSON line:
 value = audio-seconds of the batch * steps / wall time of the timed steps.  Side figures (same
@@ -31,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 # revision of the fused MFMA kernel; recorded traffic (profiles/*.md `traffic:` lines) applies only to
 # the revision it was measured on
-KERNEL_REVISION = "mirror-il2-lean"
+KERNEL_REVISION = "mirror-il2-selfplan"
 MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 MI355X_HBM_GBS = 8000.0
 
@@ -92,7 +93,7 @@ def front_end_measurement(api, synth, torch, precision, batch=16, seconds=60.0, 
 
 
 def workload_measurement(api, synth, torch, workload, precision, steps=20, settle_seconds=0.1):
-    """Fused launch (planner + kernel, hipGraph) on another workload's batch: audio-s/s and the
+    """Fused launch (hipGraph) on another workload's batch: audio-s/s and the
     kernel's MFMA fraction by the same algorithmic count as the headline (events over the steps)."""
     bt = synth.make(workload)
     f = bt.params
@@ -372,7 +373,7 @@ def main() -> int:
                                 2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],
                    "batch": B, "audio_seconds": round(audio_s, 2), "frames": int(frames.sum()),
                    "nfft": feat["nfft"], "window": feat["W"], "hop": H, "mels": feat["nmel"],
-                   "precision": a.precision, "launch": "planner + persistent fused kernel, hipGraph replay",
+                   "precision": a.precision, "launch": "persistent fused kernel (self-planned tiles), hipGraph replay",
                    "tiles": {"frames_per_tile": tuned[a.precision][0][0], "blocks_per_cu": tuned[a.precision][0][1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
