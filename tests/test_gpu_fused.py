@@ -304,3 +304,22 @@ def test_fused_power_of_two_scale_invariance(gpu, handle, precision):
         F = frames[b]
         assert np.all(np.isfinite(o[b, :, :F])), b
         assert np.all(o[b, :, :F] == o[b, :, :F].min()) and o[b, :, :F].min() <= -79.99, b
+
+
+def test_fused_tile_records_follow_the_batch(gpu):
+    """One handle, two batches back to back whose clip lengths differ (the fused launch plans its
+    tiles in each block and keeps their records at the same workspace addresses): the second launch
+    equals a fresh handle's launch of the same batch bit for bit, so no record of the first batch
+    survives into the second."""
+    bt = synth.workload1(batch=24, max_seconds=6.0)
+    L1 = bt.lengths
+    L2 = np.ascontiguousarray(L1[::-1])
+    h = gpu.Handle(len(L1))
+    run_fused(gpu, h, bt.x, L1, bt.params, "fp16x3")
+    o2, frames = run_fused(gpu, h, bt.x, L2, bt.params, "fp16x3")
+    h.close()
+    h2 = gpu.Handle(len(L1))
+    r2, _ = run_fused(gpu, h2, bt.x, L2, bt.params, "fp16x3")
+    h2.close()
+    for b, F in enumerate(frames):
+        assert np.array_equal(o2[b, :, :F], r2[b, :, :F]), b
