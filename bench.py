@@ -136,6 +136,57 @@ def workload_measurement(api, synth, torch, workload, precision, steps=20, settl
             "mfma_frac": round(flops / step_s / 1e12 / MI355X_PEAK_TFLOPS[precision], 4)}
 
 
+def trim_measurement(api, synth, torch, params, precision, steps=20, settle_seconds=0.1):
+    """Workload 1 trimmed first: non-silent region detection (-60 dB cut-off, 2048-sample window,
+    reset interval 8192, reference = each clip's maximum) writing each clip's begin/length on the
+    device, then the fused launch over the detected regions (rppt_logmel_fused_region_gpu, no
+    copy); both launches replayed from one hipGraph, and the detection alone from another."""
+    bt = synth.make(1)
+    feat = bt.params
+    B = len(bt.lengths)
+    x = torch.from_numpy(bt.x).cuda()
+    L = torch.from_numpy(bt.lengths).cuda()
+    frames = bt.lengths // feat["H"] + 1
+    out = torch.empty((B, feat["nmel"], int(frames.max())), dtype=torch.float32, device="cuda")
+    begin = torch.zeros(B, dtype=torch.int32, device="cuda")
+    length = torch.zeros(B, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    h = api.Handle(B, stream=stream)
+
+    def nsr():
+        api.non_silent_region_detection(h, x, L, begin, length, -60.0, 2048, 0.0, 8192)
+
+    def both():
+        nsr()
+        api.logmel_fused_region(h, x, out, begin, length, params, precision)
+
+    graphs = {}
+    with torch.cuda.stream(stream):
+        for name, fn in (("nsr", nsr), ("both", both)):
+            fn()
+            stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=stream):
+                fn()
+            graphs[name] = g
+    res = {"batch": B, "audio_seconds": round(bt.audio_seconds, 1)}
+    for name, g in graphs.items():
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < settle_seconds:
+            g.replay()
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.replay()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        res[f"{name}_ms_per_step"] = round(wall * 1e3, 4)
+        res[f"{name}_audio_s_per_s"] = round(bt.audio_seconds / wall, 1)
+    res["kept_fraction"] = round(float(length.sum().item()) / float(bt.lengths.sum()), 4)
+    h.close()
+    return res
+
+
 def serve_measurement(api, synth, params, precision, seconds, rate=4000.0):
     """Served stream: Poisson arrivals at `rate` clips/s, lengths log-uniform 0.25-30 s at 16 kHz,
     through the continuous-batching server (2 ms batch deadline, 4 device slots, admission by
@@ -345,6 +396,8 @@ def main() -> int:
     if not a.no_side:
         side["front_end_workload4"] = front_end_measurement(api, synth, torch, a.precision, steps=20)
         log(f"front end: {side['front_end_workload4']}")
+        side["trim_workload1"] = trim_measurement(api, synth, torch, params, a.precision)
+        log(f"trim: {side['trim_workload1']}")
         for wl in (0, 2):
             if wl != a.workload:
                 side[f"workload{wl}"] = workload_measurement(api, synth, torch, wl, a.precision)

@@ -23,7 +23,8 @@
 namespace rppa {
 namespace {
 
-constexpr int kChunk = 16;          // samples per lane per prefetch step
+constexpr int kChunk = 32;          // samples per lane per prefetch step (three steps in flight)
+constexpr int kVecChunk = 16;       // the same with 16-byte loads
 constexpr int kPass1Threads = 64;   // one wave per workgroup spreads the few lanes over CUs
 constexpr int kPass2Threads = 128;
 
@@ -42,6 +43,7 @@ struct NsrArgs {
     int *blockLast;
     int *begin;
     int *length;
+    int vec;            // rows 16-byte aligned, nStride and R multiples of 4: 16-byte loads
 };
 
 __device__ __forceinline__ int clip_len(const NsrArgs &a, int b)
@@ -59,47 +61,172 @@ __device__ __forceinline__ int blocks_of(int L, int R) { return L <= 0 ? 0 : (R 
 __device__ __forceinline__ float ld(const float *x, int i, int n) { return x[min(max(i, 0), n - 1)]; }
 
 // Runs the HOST recurrence over outputs [start, end) of one clip of length n and calls
-// visit(p, mms[p]) in order.
+// visit(p, mms[p], p < end) in order, branch-free: the recurrence runs on to the end of the last
+// chunk (values past `end` are flagged invalid and never used).  The recurrence is sequential (its rounding is the HOST's), so the
+// loads run ahead of it: kDepth chunks of kChunk samples are in flight while one is consumed (the
+// three buffers rotate by name in a loop unrolled three times, so no values are copied).
 template <class Visit>
 __device__ __forceinline__ void mms_block(const float *x, int n, int start, int end, int W, float mf, Visit &&visit)
 {
     float sum = 0.0f;
-    // fresh sum of the W-1 samples before the block (HOST :129-130)
-    for (int i = max(start - W + 1, 0); i < start; i++) {
-        float q = x[i] * x[i];
-        sum = sum + q;
-    }
-    float ca[kChunk], cs[kChunk];
+    {
+        // fresh sum of the W-1 samples before the block (HOST :129-130), in order
+        const int f0 = max(start - W + 1, 0);
+        float b0[kChunk], b1[kChunk], b2[kChunk];
+        auto load = [&](float (&v)[kChunk], int i0) {
 #pragma unroll
-    for (int i = 0; i < kChunk; i++) {
-        ca[i] = ld(x, start + i, n);
-        cs[i] = ld(x, start - W + 1 + i, n);
+            for (int u = 0; u < kChunk; u++) v[u] = ld(x, i0 + u, n);
+        };
+        auto add = [&](const float (&v)[kChunk], int i0) {
+#pragma unroll
+            for (int u = 0; u < kChunk; u++)
+                {
+                    float q = v[u] * v[u];
+                    sum = sum + (i0 + u < start ? q : 0.0f);   // adding +0 keeps the sum
+                }
+        };
+        if (f0 < start) {
+            load(b0, f0);
+            load(b1, f0 + kChunk);
+            load(b2, f0 + 2 * kChunk);
+            for (int i0 = f0; i0 < start; i0 += 3 * kChunk) {
+                add(b0, i0);
+                load(b0, i0 + 3 * kChunk);
+                add(b1, i0 + kChunk);
+                load(b1, i0 + 4 * kChunk);
+                add(b2, i0 + 2 * kChunk);
+                load(b2, i0 + 5 * kChunk);
+            }
+        }
     }
-    for (int p0 = start; p0 < end; p0 += kChunk) {
-        float na[kChunk], ns[kChunk];
+    float a0[kChunk], a1[kChunk], a2[kChunk], s0[kChunk], s1[kChunk], s2[kChunk];
+    auto load2 = [&](float (&av)[kChunk], float (&sv)[kChunk], int p0) {
 #pragma unroll
         for (int i = 0; i < kChunk; i++) {
-            na[i] = ld(x, p0 + kChunk + i, n);
-            ns[i] = ld(x, p0 + kChunk - W + 1 + i, n);
+            av[i] = ld(x, p0 + i, n);
+            sv[i] = ld(x, p0 - W + 1 + i, n);
         }
+    };
+    auto run = [&](const float (&av)[kChunk], const float (&sv)[kChunk], int p0) {
 #pragma unroll
         for (int i = 0; i < kChunk; i++) {
             const int p = p0 + i;
-            if (p < end) {
-                float q = ca[i] * ca[i];
+            float q = av[i] * av[i];
+            sum = sum + q;
+            visit(p, sum * mf, p < end);
+            float r = sv[i] * sv[i];
+            sum = sum - (p - W + 1 >= 0 ? r : 0.0f);   // x - 0 is x: the HOST's skipped subtraction
+        }
+    };
+    load2(a0, s0, start);
+    load2(a1, s1, start + kChunk);
+    load2(a2, s2, start + 2 * kChunk);
+    for (int p0 = start; p0 < end; p0 += 3 * kChunk) {
+        run(a0, s0, p0);
+        load2(a0, s0, p0 + 3 * kChunk);
+        run(a1, s1, p0 + kChunk);
+        load2(a1, s1, p0 + 4 * kChunk);
+        run(a2, s2, p0 + 2 * kChunk);
+        load2(a2, s2, p0 + 5 * kChunk);
+    }
+}
+
+// The same recurrence on 16-byte loads, for rows that are 16-byte aligned with a block start that
+// is a multiple of 4: the operations and their order are mms_block's, only the loads differ (the
+// trailing samples x[p - W + 1] are elements M .. M+3 of two aligned groups, M = (start - W + 1) & 3).
+template <int M, class Visit>
+__device__ __forceinline__ void mms_main_vec(const float *x, int rowLen, int start, int end, int W, float mf,
+                                             float &sum, Visit &visit)
+{
+    constexpr int G = kVecChunk / 4;                       // 16-byte groups per chunk
+    const int tb0 = start - W + 1 - M;                     // a multiple of 4
+    auto ld4 = [&](int i) { return *reinterpret_cast<const float4 *>(x + min(max(i, 0), rowLen - 4)); };
+    auto load = [&](float4 (&A)[G], float4 (&T)[G], int p0) {
+        const int t0 = tb0 + (p0 - start);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            A[g] = ld4(p0 + 4 * g);
+            T[g] = ld4(t0 + 4 * g);
+        }
+    };
+    auto run = [&](const float4 (&A)[G], const float4 (&T)[G], const float4 &Tn, int p0) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const float4 t1 = g + 1 < G ? T[g + 1] : Tn;
+            const float a[4] = {A[g].x, A[g].y, A[g].z, A[g].w};
+            const float t8[8] = {T[g].x, T[g].y, T[g].z, T[g].w, t1.x, t1.y, t1.z, t1.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int p = p0 + 4 * g + u;
+                float q = a[u] * a[u];
                 sum = sum + q;
-                visit(p, sum * mf);
-                if (p - W + 1 >= 0) {
-                    float r = cs[i] * cs[i];
-                    sum = sum - r;
-                }
+                visit(p, sum * mf, p < end);
+                float r = t8[M + u] * t8[M + u];
+                sum = sum - (p - W + 1 >= 0 ? r : 0.0f);
             }
         }
+    };
+    float4 A0[G], A1[G], A2[G], T0[G], T1[G], T2[G];
+    load(A0, T0, start);
+    load(A1, T1, start + kVecChunk);
+    load(A2, T2, start + 2 * kVecChunk);
+    for (int p0 = start; p0 < end; p0 += 3 * kVecChunk) {
+        run(A0, T0, T1[0], p0);
+        load(A0, T0, p0 + 3 * kVecChunk);
+        run(A1, T1, T2[0], p0 + kVecChunk);
+        load(A1, T1, p0 + 4 * kVecChunk);
+        run(A2, T2, T0[0], p0 + 2 * kVecChunk);
+        load(A2, T2, p0 + 5 * kVecChunk);
+    }
+}
+
+template <class Visit>
+__device__ __forceinline__ void mms_block_vec(const float *x, int rowLen, int start, int end, int W, float mf,
+                                              Visit &&visit)
+{
+    constexpr int G = kVecChunk / 4;
+    float sum = 0.0f;
+    {
+        // fresh sum of the W-1 samples before the block, in order, from aligned 16-byte groups
+        const int f0 = max(start - W + 1, 0);
+        const int a0 = f0 & ~3;
+        auto ld4 = [&](int i) { return *reinterpret_cast<const float4 *>(x + min(max(i, 0), rowLen - 4)); };
+        auto load = [&](float4 (&v)[G], int i0) {
 #pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            ca[i] = na[i];
-            cs[i] = ns[i];
+            for (int g = 0; g < G; g++) v[g] = ld4(i0 + 4 * g);
+        };
+        auto add = [&](const float4 (&v)[G], int i0) {
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const float e[4] = {v[g].x, v[g].y, v[g].z, v[g].w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int i = i0 + 4 * g + u;
+                    float q = e[u] * e[u];
+                    sum = sum + ((i >= f0 && i < start) ? q : 0.0f);   // adding +0 keeps the sum
+                }
+            }
+        };
+        if (f0 < start) {
+            float4 b0[G], b1[G], b2[G];
+            load(b0, a0);
+            load(b1, a0 + kVecChunk);
+            load(b2, a0 + 2 * kVecChunk);
+            for (int i0 = a0; i0 < start; i0 += 3 * kVecChunk) {
+                add(b0, i0);
+                load(b0, i0 + 3 * kVecChunk);
+                add(b1, i0 + kVecChunk);
+                load(b1, i0 + 4 * kVecChunk);
+                add(b2, i0 + 2 * kVecChunk);
+                load(b2, i0 + 5 * kVecChunk);
+            }
         }
+    }
+    switch ((start - W + 1) & 3) {
+    case 0: mms_main_vec<0>(x, rowLen, start, end, W, mf, sum, visit); break;
+    case 1: mms_main_vec<1>(x, rowLen, start, end, W, mf, sum, visit); break;
+    case 2: mms_main_vec<2>(x, rowLen, start, end, W, mf, sum, visit); break;
+    default: mms_main_vec<3>(x, rowLen, start, end, W, mf, sum, visit); break;
     }
 }
 
@@ -107,8 +234,12 @@ __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, in
 {
     const long long lane = static_cast<long long>(blockIdx.x) * kPass1Threads + threadIdx.x;
     if (lane >= static_cast<long long>(B) * a.maxBlocks) return;
-    const int b = static_cast<int>(lane / a.maxBlocks);
-    const int j = static_cast<int>(lane % a.maxBlocks);
+    // consecutive lanes take the same block of consecutive clips: the lanes of a wave then read
+    // rows a clip stride apart rather than blocks a reset interval apart (a power-of-two stride
+    // that sends every lane's load to the same memory channel)
+    const int b = static_cast<int>(lane % B);
+    const int j = static_cast<int>(lane / B);
+    const long long cell = static_cast<long long>(b) * a.maxBlocks + j;
     const int L = clip_len(a, b);
     if (j >= blocks_of(L, a.R)) return;
     const int R = a.R <= 0 ? L : a.R;
@@ -119,17 +250,18 @@ __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, in
     const float cut = a.refPower * a.cutOff;
     float m = -INFINITY;
     int first = -1, last = -1;
-    mms_block(x, L, start, end, a.W, a.meanFactor, [&](int p, float v) {
-        m = fmaxf(m, v);
-        if (knownCut && v >= cut) {
-            if (first < 0) first = p;
-            last = p;
-        }
-    });
-    a.blockMax[lane] = m;
+    auto visit = [&](int p, float v, bool valid) {
+        m = fmaxf(m, valid ? v : -INFINITY);
+        const bool hit = valid && knownCut && v >= cut;
+        first = (hit && first < 0) ? p : first;
+        last = hit ? p : last;
+    };
+    if (a.vec) mms_block_vec(x, static_cast<int>(a.nStride), start, end, a.W, a.meanFactor, visit);
+    else mms_block(x, L, start, end, a.W, a.meanFactor, visit);
+    a.blockMax[cell] = m;
     if (knownCut) {
-        a.blockFirst[lane] = first;
-        a.blockLast[lane] = last;
+        a.blockFirst[cell] = first;
+        a.blockLast[cell] = last;
     }
 }
 
@@ -186,12 +318,14 @@ __global__ void __launch_bounds__(kPass2Threads) nsr_region_kernel(NsrArgs a)
             const int j = head ? sFirstBlk : sLastBlk;
             int hit = -1;
             bool done = false;
-            mms_block(x, L, j * R, static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(j) * R + R)), a.W, a.meanFactor, [&](int p, float v) {
-                if (!done && v >= cut) {
-                    hit = p;
-                    done = head;  // the head wants the first hit; the tail keeps the last
-                }
-            });
+            auto visit = [&](int p, float v, bool valid) {
+                const bool h = valid && !done && v >= cut;
+                hit = h ? p : hit;
+                done = done || (h && head);  // the head wants the first hit; the tail keeps the last
+            };
+            const int s0 = j * R, e0 = static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(j) * R + R));
+            if (a.vec) mms_block_vec(x, static_cast<int>(a.nStride), s0, e0, a.W, a.meanFactor, visit);
+            else mms_block(x, L, s0, e0, a.W, a.meanFactor, visit);
             if (head) sBegin = hit;
             else sEnd = hit;
         }
@@ -247,6 +381,7 @@ RPPA_EXPORT RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, Rppt
     a.refPower = referencePower;
     a.begin = detectedIndexTensor;
     a.length = detectionLengthTensor;
+    a.vec = (reinterpret_cast<uintptr_t>(srcPtr) & 15) == 0 && nStride % 4 == 0 && (resetInterval <= 0 || resetInterval % 4 == 0) ? 1 : 0;
     const size_t cells = static_cast<size_t>(B) * a.maxBlocks;
     char *ws = static_cast<char *>(rppa::workspace(rppHandle, cells * 12));
     if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
