@@ -13,10 +13,13 @@
 // so mms is bit-identical to that sequence and the detected indices match it exactly.  The
 // reference's HIP kernel uses per-tile prefix sums instead, whose rounding differs from HOST.
 //
-// Parallelism is the reset blocks: one lane runs one block's recurrence.  Pass 1 gives each
-// block's maximum (and, when the cut-off is known up front, its first and last hit); pass 2, one
-// workgroup per clip, reduces them and rescans only the two blocks holding the region's ends.
+// Parallelism is the reset blocks: one lane runs one block's recurrence.  The recurrence's whole
+// state is the running sum, so pass 1 records it every kSeg outputs together with the largest sum
+// of those outputs (or, when the cut-off is known up front, each block's first and last hit);
+// pass 2, one workgroup per clip, reduces the records and resumes the recurrence from the
+// recorded sums over only the two kSeg-output spans holding the region's ends.
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -25,8 +28,10 @@ namespace {
 
 constexpr int kChunk = 32;          // samples per lane per prefetch step (three steps in flight)
 constexpr int kVecChunk = 16;       // the same with 16-byte loads
+constexpr int kSeg = 192;           // outputs between checkpoints of the running sum
 constexpr int kPass1Threads = 64;   // one wave per workgroup spreads the few lanes over CUs
 constexpr int kPass2Threads = 128;
+static_assert(kSeg % (3 * kChunk) == 0 && kSeg % (3 * kVecChunk) == 0, "checkpoints fall on loop steps");
 
 struct NsrArgs {
     const float *src;
@@ -36,14 +41,20 @@ struct NsrArgs {
     float meanFactor;
     int R;              // reset length; <= 0 means the whole clip is one block
     int maxBlocks;
+    int segsPerBlock;   // checkpoints per reset block
+    int maxSegs;        // checkpoints per clip row
     float cutOff;       // 10^(cutOffDB/10)
     float refPower;     // 0: per-clip maximum
-    float *blockMax;    // [B][maxBlocks]
+    float *segMax;      // [B][maxSegs] maximum running sum over each checkpoint's outputs
+    float *segSum;      // [B][maxSegs] running sum before each checkpoint's first output
+    float *blockMax;    // [B][maxBlocks] maximum running sum over each reset block
     int *blockFirst;    // [B][maxBlocks], used when refPower != 0
     int *blockLast;
     int *begin;
     int *length;
-    int vec;            // rows 16-byte aligned, nStride and R multiples of 4: 16-byte loads
+    int vec;            // rows 16-byte aligned, nStride and R multiples of 4, batch under 2 GiB:
+                        // 16-byte buffer loads
+    int srcBytes;       // B * nStride * 4 when vec
 };
 
 __device__ __forceinline__ int clip_len(const NsrArgs &a, int b)
@@ -60,45 +71,51 @@ __device__ __forceinline__ int blocks_of(int L, int R) { return L <= 0 ? 0 : (R 
 // loaded there are never used.
 __device__ __forceinline__ float ld(const float *x, int i, int n) { return x[min(max(i, 0), n - 1)]; }
 
-// Runs the HOST recurrence over outputs [start, end) of one clip of length n and calls
-// visit(p, mms[p], p < end) in order, branch-free: the recurrence runs on to the end of the last
-// chunk (values past `end` are flagged invalid and never used).  The recurrence is sequential (its rounding is the HOST's), so the
-// loads run ahead of it: kDepth chunks of kChunk samples are in flight while one is consumed (the
-// three buffers rotate by name in a loop unrolled three times, so no values are copied).
-template <class Visit>
-__device__ __forceinline__ void mms_block(const float *x, int n, int start, int end, int W, float mf, Visit &&visit)
+// Fresh sum of the W-1 squares before output `start`, in order (HOST :129-130).
+__device__ __forceinline__ float fresh_sum(const float *x, int n, int start, int W)
 {
     float sum = 0.0f;
-    {
-        // fresh sum of the W-1 samples before the block (HOST :129-130), in order
-        const int f0 = max(start - W + 1, 0);
-        float b0[kChunk], b1[kChunk], b2[kChunk];
-        auto load = [&](float (&v)[kChunk], int i0) {
+    const int f0 = max(start - W + 1, 0);
+    float b0[kChunk], b1[kChunk], b2[kChunk];
+    auto load = [&](float (&v)[kChunk], int i0) {
 #pragma unroll
-            for (int u = 0; u < kChunk; u++) v[u] = ld(x, i0 + u, n);
-        };
-        auto add = [&](const float (&v)[kChunk], int i0) {
+        for (int u = 0; u < kChunk; u++) v[u] = ld(x, i0 + u, n);
+    };
+    auto add = [&](const float (&v)[kChunk], int i0) {
 #pragma unroll
-            for (int u = 0; u < kChunk; u++)
-                {
-                    float q = v[u] * v[u];
-                    sum = sum + (i0 + u < start ? q : 0.0f);   // adding +0 keeps the sum
-                }
-        };
-        if (f0 < start) {
-            load(b0, f0);
-            load(b1, f0 + kChunk);
-            load(b2, f0 + 2 * kChunk);
-            for (int i0 = f0; i0 < start; i0 += 3 * kChunk) {
-                add(b0, i0);
-                load(b0, i0 + 3 * kChunk);
-                add(b1, i0 + kChunk);
-                load(b1, i0 + 4 * kChunk);
-                add(b2, i0 + 2 * kChunk);
-                load(b2, i0 + 5 * kChunk);
-            }
+        for (int u = 0; u < kChunk; u++) {
+            float q = v[u] * v[u];
+            sum = sum + (i0 + u < start ? q : 0.0f);   // adding +0 keeps the sum
+        }
+    };
+    if (f0 < start) {
+        load(b0, f0);
+        load(b1, f0 + kChunk);
+        load(b2, f0 + 2 * kChunk);
+        for (int i0 = f0; i0 < start; i0 += 3 * kChunk) {
+            add(b0, i0);
+            load(b0, i0 + 3 * kChunk);
+            add(b1, i0 + kChunk);
+            load(b1, i0 + 4 * kChunk);
+            add(b2, i0 + 2 * kChunk);
+            load(b2, i0 + 5 * kChunk);
         }
     }
+    return sum;
+}
+
+// Runs the HOST recurrence over outputs [start, end) of one clip of length n from the running
+// sum `sum` and calls visit(p, sum_p, p < end) in order, where mms[p] = sum_p * (1/W) rounded;
+// mark(off, sum) is called before the outputs start + off of every loop step (off a multiple of
+// 3 kChunk) with the sum the recurrence resumes from there.  Branch-free: the recurrence runs on
+// to the end of the last chunk (values past `end` are flagged invalid and never used).  The
+// recurrence is sequential (its rounding is the HOST's), so the loads run ahead of it: three
+// chunks are in flight while one is consumed (the buffers rotate by name in a loop unrolled
+// three times, so no values are copied).
+template <class Visit, class Mark>
+__device__ __forceinline__ void mms_main(const float *x, int n, int start, int end, int W, float sum,
+                                         Visit &visit, Mark &mark)
+{
     float a0[kChunk], a1[kChunk], a2[kChunk], s0[kChunk], s1[kChunk], s2[kChunk];
     auto load2 = [&](float (&av)[kChunk], float (&sv)[kChunk], int p0) {
 #pragma unroll
@@ -113,7 +130,7 @@ __device__ __forceinline__ void mms_block(const float *x, int n, int start, int 
             const int p = p0 + i;
             float q = av[i] * av[i];
             sum = sum + q;
-            visit(p, sum * mf, p < end);
+            visit(p, sum, p < end);
             float r = sv[i] * sv[i];
             sum = sum - (p - W + 1 >= 0 ? r : 0.0f);   // x - 0 is x: the HOST's skipped subtraction
         }
@@ -122,6 +139,7 @@ __device__ __forceinline__ void mms_block(const float *x, int n, int start, int 
     load2(a1, s1, start + kChunk);
     load2(a2, s2, start + 2 * kChunk);
     for (int p0 = start; p0 < end; p0 += 3 * kChunk) {
+        mark(p0 - start, sum);
         run(a0, s0, p0);
         load2(a0, s0, p0 + 3 * kChunk);
         run(a1, s1, p0 + kChunk);
@@ -131,16 +149,32 @@ __device__ __forceinline__ void mms_block(const float *x, int n, int start, int 
     }
 }
 
-// The same recurrence on 16-byte loads, for rows that are 16-byte aligned with a block start that
-// is a multiple of 4: the operations and their order are mms_block's, only the loads differ (the
+// 16-byte loads of a clip's row through a buffer resource over the whole batch (under 2 GiB):
+// loads past the batch's end, or before its start (a negative offset wraps above 2^31), return
+// zeros instead of faulting, and loads before the row's start or past its end read neighbouring
+// rows; neither kind of value reaches an output.
+struct Row4 {
+    __amdgpu_buffer_rsrc_t rsrc;
+    unsigned rowOff;   // byte offset of the row
+
+    __device__ __forceinline__ float4 operator()(int i) const
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, rowOff + static_cast<unsigned>(i) * 4u, 0, 0);
+        return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+    }
+};
+
+// The same recurrence on 16-byte loads, for rows that are 16-byte aligned with a start that is a
+// multiple of 4: the operations and their order are mms_main's, only the loads differ (the
 // trailing samples x[p - W + 1] are elements M .. M+3 of two aligned groups, M = (start - W + 1) & 3).
-template <int M, class Visit>
-__device__ __forceinline__ void mms_main_vec(const float *x, int rowLen, int start, int end, int W, float mf,
-                                             float &sum, Visit &visit)
+// Full loop steps run unmasked; the last, partial step flags the outputs past `end`.  Clamp
+// (start < W - 1) keeps the HOST's skipped subtraction for outputs p < W - 1.
+template <int M, bool Clamp, class Visit, class Mark>
+__device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end, int W, float sum,
+                                             Visit &visit, Mark &mark)
 {
     constexpr int G = kVecChunk / 4;                       // 16-byte groups per chunk
     const int tb0 = start - W + 1 - M;                     // a multiple of 4
-    auto ld4 = [&](int i) { return *reinterpret_cast<const float4 *>(x + min(max(i, 0), rowLen - 4)); };
     auto load = [&](float4 (&A)[G], float4 (&T)[G], int p0) {
         const int t0 = tb0 + (p0 - start);
 #pragma unroll
@@ -149,7 +183,7 @@ __device__ __forceinline__ void mms_main_vec(const float *x, int rowLen, int sta
             T[g] = ld4(t0 + 4 * g);
         }
     };
-    auto run = [&](const float4 (&A)[G], const float4 (&T)[G], const float4 &Tn, int p0) {
+    auto run = [&](const float4 (&A)[G], const float4 (&T)[G], const float4 &Tn, int p0, auto masked) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
             const float4 t1 = g + 1 < G ? T[g + 1] : Tn;
@@ -160,9 +194,9 @@ __device__ __forceinline__ void mms_main_vec(const float *x, int rowLen, int sta
                 const int p = p0 + 4 * g + u;
                 float q = a[u] * a[u];
                 sum = sum + q;
-                visit(p, sum * mf, p < end);
+                visit(p, sum, decltype(masked)::value ? p < end : true);
                 float r = t8[M + u] * t8[M + u];
-                sum = sum - (p - W + 1 >= 0 ? r : 0.0f);
+                sum = sum - ((!Clamp || p - W + 1 >= 0) ? r : 0.0f);   // x - 0 is x
             }
         }
     };
@@ -170,66 +204,120 @@ __device__ __forceinline__ void mms_main_vec(const float *x, int rowLen, int sta
     load(A0, T0, start);
     load(A1, T1, start + kVecChunk);
     load(A2, T2, start + 2 * kVecChunk);
-    for (int p0 = start; p0 < end; p0 += 3 * kVecChunk) {
-        run(A0, T0, T1[0], p0);
+    int p0 = start;
+    for (; p0 + 3 * kVecChunk <= end; p0 += 3 * kVecChunk) {
+        mark(p0 - start, sum);
+        run(A0, T0, T1[0], p0, std::false_type{});
         load(A0, T0, p0 + 3 * kVecChunk);
-        run(A1, T1, T2[0], p0 + kVecChunk);
+        run(A1, T1, T2[0], p0 + kVecChunk, std::false_type{});
         load(A1, T1, p0 + 4 * kVecChunk);
-        run(A2, T2, T0[0], p0 + 2 * kVecChunk);
+        run(A2, T2, T0[0], p0 + 2 * kVecChunk, std::false_type{});
         load(A2, T2, p0 + 5 * kVecChunk);
+    }
+    if (p0 < end) {
+        mark(p0 - start, sum);
+        const float4 tn = ld4(tb0 + (p0 - start) + 3 * kVecChunk);   // the group after A2's chunk
+        run(A0, T0, T1[0], p0, std::true_type{});
+        run(A1, T1, T2[0], p0 + kVecChunk, std::true_type{});
+        run(A2, T2, tn, p0 + 2 * kVecChunk, std::true_type{});
     }
 }
 
-template <class Visit>
-__device__ __forceinline__ void mms_block_vec(const float *x, int rowLen, int start, int end, int W, float mf,
-                                              Visit &&visit)
+// fresh_sum on aligned 16-byte groups: the first group is masked below f0, the full loop steps
+// add unmasked, the last, partial step is masked at `start`.
+__device__ __forceinline__ float fresh_sum_vec(const Row4 &ld4, int start, int W)
 {
     constexpr int G = kVecChunk / 4;
     float sum = 0.0f;
-    {
-        // fresh sum of the W-1 samples before the block, in order, from aligned 16-byte groups
-        const int f0 = max(start - W + 1, 0);
-        const int a0 = f0 & ~3;
-        auto ld4 = [&](int i) { return *reinterpret_cast<const float4 *>(x + min(max(i, 0), rowLen - 4)); };
-        auto load = [&](float4 (&v)[G], int i0) {
+    const int f0 = max(start - W + 1, 0);
+    if (f0 >= start) return sum;
+    const int a0 = f0 & ~3;
+    auto load = [&](float4 (&v)[G], int i0) {
 #pragma unroll
-            for (int g = 0; g < G; g++) v[g] = ld4(i0 + 4 * g);
-        };
-        auto add = [&](const float4 (&v)[G], int i0) {
+        for (int g = 0; g < G; g++) v[g] = ld4(i0 + 4 * g);
+    };
+    auto add = [&](const float4 (&v)[G], int i0, auto masked) {
 #pragma unroll
-            for (int g = 0; g < G; g++) {
-                const float e[4] = {v[g].x, v[g].y, v[g].z, v[g].w};
+        for (int g = 0; g < G; g++) {
+            const float e[4] = {v[g].x, v[g].y, v[g].z, v[g].w};
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int i = i0 + 4 * g + u;
-                    float q = e[u] * e[u];
-                    sum = sum + ((i >= f0 && i < start) ? q : 0.0f);   // adding +0 keeps the sum
-                }
-            }
-        };
-        if (f0 < start) {
-            float4 b0[G], b1[G], b2[G];
-            load(b0, a0);
-            load(b1, a0 + kVecChunk);
-            load(b2, a0 + 2 * kVecChunk);
-            for (int i0 = a0; i0 < start; i0 += 3 * kVecChunk) {
-                add(b0, i0);
-                load(b0, i0 + 3 * kVecChunk);
-                add(b1, i0 + kVecChunk);
-                load(b1, i0 + 4 * kVecChunk);
-                add(b2, i0 + 2 * kVecChunk);
-                load(b2, i0 + 5 * kVecChunk);
+            for (int u = 0; u < 4; u++) {
+                float q = e[u] * e[u];
+                sum = sum + ((!decltype(masked)::value || i0 + 4 * g + u < start) ? q : 0.0f);   // +0 keeps the sum
             }
         }
+    };
+    {
+        const float4 v = ld4(a0);
+        const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            float q = e[u] * e[u];
+            sum = sum + ((a0 + u >= f0 && a0 + u < start) ? q : 0.0f);
+        }
     }
-    switch ((start - W + 1) & 3) {
-    case 0: mms_main_vec<0>(x, rowLen, start, end, W, mf, sum, visit); break;
-    case 1: mms_main_vec<1>(x, rowLen, start, end, W, mf, sum, visit); break;
-    case 2: mms_main_vec<2>(x, rowLen, start, end, W, mf, sum, visit); break;
-    default: mms_main_vec<3>(x, rowLen, start, end, W, mf, sum, visit); break;
+    int i0 = a0 + 4;
+    float4 b0[G], b1[G], b2[G];
+    load(b0, i0);
+    load(b1, i0 + kVecChunk);
+    load(b2, i0 + 2 * kVecChunk);
+    for (; i0 + 3 * kVecChunk <= start; i0 += 3 * kVecChunk) {
+        add(b0, i0, std::false_type{});
+        load(b0, i0 + 3 * kVecChunk);
+        add(b1, i0 + kVecChunk, std::false_type{});
+        load(b1, i0 + 4 * kVecChunk);
+        add(b2, i0 + 2 * kVecChunk, std::false_type{});
+        load(b2, i0 + 5 * kVecChunk);
+    }
+    if (i0 < start) {
+        add(b0, i0, std::true_type{});
+        add(b1, i0 + kVecChunk, std::true_type{});
+        add(b2, i0 + 2 * kVecChunk, std::true_type{});
+    }
+    return sum;
+}
+
+__device__ __forceinline__ Row4 row4(const NsrArgs &a, int b)
+{
+    return Row4{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(a.src), 0, a.srcBytes, 0x00020000),
+                static_cast<unsigned>(b) * static_cast<unsigned>(a.nStride) * 4u};
+}
+
+__device__ __forceinline__ float fresh(const NsrArgs &a, const float *x, int b, int L, int start)
+{
+    return a.vec ? fresh_sum_vec(row4(a, b), start, a.W) : fresh_sum(x, L, start, a.W);
+}
+
+template <int M, class Visit, class Mark>
+__device__ __forceinline__ void mms_vec(const Row4 &ld4, int start, int end, int W, float sum, Visit &visit, Mark &mark)
+{
+    if (start - W + 1 < 0) mms_main_vec<M, true>(ld4, start, end, W, sum, visit, mark);
+    else mms_main_vec<M, false>(ld4, start, end, W, sum, visit, mark);
+}
+
+template <class Visit, class Mark>
+__device__ __forceinline__ void mms_run(const NsrArgs &a, const float *x, int b, int L, int start, int end, float sum,
+                                        Visit &visit, Mark &mark)
+{
+    if (a.vec) {
+        const Row4 ld4 = row4(a, b);
+        switch ((start - a.W + 1) & 3) {
+        case 0: mms_vec<0>(ld4, start, end, a.W, sum, visit, mark); break;
+        case 1: mms_vec<1>(ld4, start, end, a.W, sum, visit, mark); break;
+        case 2: mms_vec<2>(ld4, start, end, a.W, sum, visit, mark); break;
+        default: mms_vec<3>(ld4, start, end, a.W, sum, visit, mark); break;
+        }
+    } else {
+        mms_main(x, L, start, end, a.W, sum, visit, mark);
     }
 }
 
+// Pass 1: one lane per reset block.  With a known cut-off (referencePower != 0) it records the
+// block's first and last hit; otherwise, every kSeg outputs, the running sum the recurrence
+// resumes from and the largest running sum of those outputs, and the block's largest running sum
+// (mms = sum * (1/W) rounded is monotone in sum, so the largest mms is the largest sum times 1/W,
+// rounded).
+template <bool KnownCut>
 __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, int B)
 {
     const long long lane = static_cast<long long>(blockIdx.x) * kPass1Threads + threadIdx.x;
@@ -239,60 +327,72 @@ __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, in
     // that sends every lane's load to the same memory channel)
     const int b = static_cast<int>(lane % B);
     const int j = static_cast<int>(lane / B);
-    const long long cell = static_cast<long long>(b) * a.maxBlocks + j;
     const int L = clip_len(a, b);
     if (j >= blocks_of(L, a.R)) return;
     const int R = a.R <= 0 ? L : a.R;
     const int start = j * R;
     const int end = static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(start) + R));
     const float *x = a.src + b * a.nStride;
-    const bool knownCut = a.refPower != 0.0f;
-    const float cut = a.refPower * a.cutOff;
-    float m = -INFINITY;
-    int first = -1, last = -1;
-    auto visit = [&](int p, float v, bool valid) {
-        m = fmaxf(m, valid ? v : -INFINITY);
-        const bool hit = valid && knownCut && v >= cut;
-        first = (hit && first < 0) ? p : first;
-        last = hit ? p : last;
-    };
-    if (a.vec) mms_block_vec(x, static_cast<int>(a.nStride), start, end, a.W, a.meanFactor, visit);
-    else mms_block(x, L, start, end, a.W, a.meanFactor, visit);
-    a.blockMax[cell] = m;
-    if (knownCut) {
+    const float sum0 = fresh(a, x, b, L, start);
+    if constexpr (KnownCut) {
+        const float mf = a.meanFactor;
+        const float cut = a.refPower * a.cutOff;
+        int first = -1, last = -1;
+        auto visit = [&](int p, float s, bool valid) {
+            const bool hit = valid && s * mf >= cut;
+            first = (hit && first < 0) ? p : first;
+            last = hit ? p : last;
+        };
+        auto mark = [](int, float) {};
+        mms_run(a, x, b, L, start, end, sum0, visit, mark);
+        const long long cell = static_cast<long long>(b) * a.maxBlocks + j;
         a.blockFirst[cell] = first;
         a.blockLast[cell] = last;
+    } else {
+        float *segMax = a.segMax + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+        float *segSum = a.segSum + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+        float m = -INFINITY, mb = -INFINITY;
+        int k = -1;
+        auto visit = [&](int, float s, bool valid) { m = fmaxf(m, valid ? s : -INFINITY); };
+        auto mark = [&](int off, float s) {
+            if (off % kSeg == 0) {
+                if (k >= 0) {
+                    segMax[k] = m;
+                    mb = fmaxf(mb, m);
+                }
+                ++k;
+                m = -INFINITY;
+                segSum[k] = s;
+            }
+        };
+        mms_run(a, x, b, L, start, end, sum0, visit, mark);
+        segMax[k] = m;   // start < end: the first loop step marked checkpoint 0
+        a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, m);
     }
 }
 
+// Pass 2: one workgroup per clip reduces pass 1's records; without a known cut-off it finds the
+// first and the last reset block holding a hit, in them the first and the last checkpoint span
+// holding one, and reruns the recurrence over only those two spans from their recorded sums.
+template <bool KnownCut>
 __global__ void __launch_bounds__(kPass2Threads) nsr_region_kernel(NsrArgs a)
 {
     const int b = blockIdx.x;
     const int L = clip_len(a, b);
     const int nb = blocks_of(L, a.R);
-    const bool knownCut = a.refPower != 0.0f;
     __shared__ float sMax[kPass2Threads];
-    __shared__ int sFirstBlk, sLastBlk, sBegin, sEnd;
-    const float *bm = a.blockMax + static_cast<long long>(b) * a.maxBlocks;
-
-    float m = -INFINITY;
-    for (int j = threadIdx.x; j < nb; j += kPass2Threads) m = fmaxf(m, bm[j]);
-    sMax[threadIdx.x] = m;
+    __shared__ int sFirstBlk, sLastBlk, sFirst, sLast, sBegin, sEnd;
     if (threadIdx.x == 0) {
-        sFirstBlk = nb;
+        sFirstBlk = 0x7fffffff;
         sLastBlk = -1;
+        sFirst = 0x7fffffff;
+        sLast = -1;
         sBegin = L;
         sEnd = -1;
     }
     __syncthreads();
-    for (int s = kPass2Threads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sMax[threadIdx.x] = fmaxf(sMax[threadIdx.x], sMax[threadIdx.x + s]);
-        __syncthreads();
-    }
-    const float base = knownCut ? a.refPower : sMax[0];
-    const float cut = base * a.cutOff;
 
-    if (knownCut) {
+    if constexpr (KnownCut) {
         // pass 1 already found every block's first and last hit
         const int *bf = a.blockFirst + static_cast<long long>(b) * a.maxBlocks;
         const int *bl = a.blockLast + static_cast<long long>(b) * a.maxBlocks;
@@ -303,31 +403,87 @@ __global__ void __launch_bounds__(kPass2Threads) nsr_region_kernel(NsrArgs a)
             }
         }
     } else {
+        const int R = a.R <= 0 ? L : a.R;
+        const float mf = a.meanFactor;
+        const float *bm = a.blockMax + static_cast<long long>(b) * a.maxBlocks;
+        float m = -INFINITY;
+        for (int j = threadIdx.x; j < nb; j += kPass2Threads) m = fmaxf(m, bm[j]);
+        sMax[threadIdx.x] = m;
+        __syncthreads();
+        for (int s = kPass2Threads / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sMax[threadIdx.x] = fmaxf(sMax[threadIdx.x], sMax[threadIdx.x + s]);
+            __syncthreads();
+        }
+        const float cut = (sMax[0] * mf) * a.cutOff;   // the largest mms times the cut-off ratio
+        // the first and the last reset block holding a hit, then the first checkpoint span of the
+        // one and the last of the other holding a hit
         for (int j = threadIdx.x; j < nb; j += kPass2Threads) {
-            if (bm[j] >= cut) {
+            if (bm[j] * mf >= cut) {
                 atomicMin(&sFirstBlk, j);
                 atomicMax(&sLastBlk, j);
             }
         }
         __syncthreads();
-        // wave 0 lane 0 rescans the first block holding a hit, wave 1 lane 0 the last one
-        const int R = a.R <= 0 ? L : a.R;
-        const float *x = a.src + b * a.nStride;
-        if ((threadIdx.x == 0 || threadIdx.x == kWave) && sLastBlk >= 0) {
-            const bool head = threadIdx.x == 0;
-            const int j = head ? sFirstBlk : sLastBlk;
-            int hit = -1;
-            bool done = false;
-            auto visit = [&](int p, float v, bool valid) {
-                const bool h = valid && !done && v >= cut;
-                hit = h ? p : hit;
-                done = done || (h && head);  // the head wants the first hit; the tail keeps the last
-            };
-            const int s0 = j * R, e0 = static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(j) * R + R));
-            if (a.vec) mms_block_vec(x, static_cast<int>(a.nStride), s0, e0, a.W, a.meanFactor, visit);
-            else mms_block(x, L, s0, e0, a.W, a.meanFactor, visit);
-            if (head) sBegin = hit;
-            else sEnd = hit;
+        if (sLastBlk >= 0) {
+            const float *sm = a.segMax + static_cast<long long>(b) * a.maxSegs;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int j = h == 0 ? sFirstBlk : sLastBlk;
+                const long long blockEnd = min(static_cast<long long>(L), static_cast<long long>(j) * R + R);
+                const int segs = static_cast<int>((blockEnd - static_cast<long long>(j) * R + kSeg - 1) / kSeg);
+                for (int k = threadIdx.x; k < segs; k += kPass2Threads) {
+                    const int g = j * a.segsPerBlock + k;
+                    if (sm[g] * mf >= cut) {
+                        if (h == 0) atomicMin(&sFirst, g);
+                        else atomicMax(&sLast, g);
+                    }
+                }
+            }
+            __syncthreads();
+            // the squares of the two spans' samples and trailing samples x[p - W + 1] go to LDS
+            // (0 where p < W - 1: x - 0 is x, the HOST's skipped subtraction); one lane per span
+            // runs the additions in order and keeps every running sum; then all threads search
+            __shared__ float sQ[2][kSeg], sR[2][kSeg], sS[2][kSeg];
+            const float *x = a.src + b * a.nStride;
+            int s0[2], n[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int g = h == 0 ? sFirst : sLast;
+                const int j = g / a.segsPerBlock;
+                s0[h] = j * R + (g - j * a.segsPerBlock) * kSeg;
+                const long long e0 = min(static_cast<long long>(L),
+                                         min(static_cast<long long>(s0[h]) + kSeg, static_cast<long long>(j) * R + R));
+                n[h] = static_cast<int>(e0 - s0[h]);
+                for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
+                    const int t = s0[h] + i - a.W + 1;
+                    const float v = x[s0[h] + i];
+                    const float w = t >= 0 ? x[t] : 0.0f;
+                    sQ[h][i] = v * v;
+                    sR[h][i] = w * w;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 || threadIdx.x == kWave) {
+                const int h = threadIdx.x == 0 ? 0 : 1;
+                float sum = a.segSum[static_cast<long long>(b) * a.maxSegs + (h == 0 ? sFirst : sLast)];
+                const int nh = n[h];
+#pragma unroll 8
+                for (int i = 0; i < nh; i++) {
+                    sum = sum + sQ[h][i];
+                    sS[h][i] = sum;
+                    sum = sum - sR[h][i];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
+                    if (sS[h][i] * mf >= cut) {
+                        if (h == 0) atomicMin(&sBegin, s0[h] + i);
+                        else atomicMax(&sEnd, s0[h] + i);
+                    }
+                }
+            }
         }
     }
     __syncthreads();
@@ -381,18 +537,35 @@ RPPA_EXPORT RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, Rppt
     a.refPower = referencePower;
     a.begin = detectedIndexTensor;
     a.length = detectionLengthTensor;
-    a.vec = (reinterpret_cast<uintptr_t>(srcPtr) & 15) == 0 && nStride % 4 == 0 && (resetInterval <= 0 || resetInterval % 4 == 0) ? 1 : 0;
+    a.vec = (reinterpret_cast<uintptr_t>(srcPtr) & 15) == 0 && nStride % 4 == 0 && (resetInterval <= 0 || resetInterval % 4 == 0) &&
+            static_cast<long long>(B) * nStride * 4 < (1LL << 31) ? 1 : 0;
+    a.srcBytes = a.vec ? static_cast<int>(static_cast<long long>(B) * nStride * 4) : 0;
+    a.segsPerBlock = static_cast<int>(((resetInterval <= 0 ? nStride : resetInterval) + rppa::kSeg - 1) / rppa::kSeg);
+    a.maxSegs = resetInterval <= 0 ? a.segsPerBlock : a.maxBlocks * a.segsPerBlock;
+    const bool knownCut = referencePower != 0.0f;
     const size_t cells = static_cast<size_t>(B) * a.maxBlocks;
-    char *ws = static_cast<char *>(rppa::workspace(rppHandle, cells * 12));
+    const size_t segCells = knownCut ? 0 : static_cast<size_t>(B) * a.maxSegs;
+    char *ws = static_cast<char *>(rppa::workspace(rppHandle, knownCut ? cells * 8 : segCells * 8 + cells * 4));
     if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
-    a.blockMax = reinterpret_cast<float *>(ws);
-    a.blockFirst = reinterpret_cast<int *>(ws + cells * 4);
-    a.blockLast = reinterpret_cast<int *>(ws + cells * 8);
+    if (knownCut) {
+        a.blockFirst = reinterpret_cast<int *>(ws);
+        a.blockLast = reinterpret_cast<int *>(ws + cells * 4);
+    } else {
+        a.segMax = reinterpret_cast<float *>(ws);
+        a.segSum = reinterpret_cast<float *>(ws + segCells * 4);
+        a.blockMax = reinterpret_cast<float *>(ws + segCells * 8);
+    }
 
     const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Threads - 1) / rppa::kPass1Threads);
-    hipLaunchKernelGGL(rppa::nsr_blocks_kernel, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
-    RPPA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(rppa::nsr_region_kernel, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
+    if (knownCut) {
+        hipLaunchKernelGGL(rppa::nsr_blocks_kernel<true>, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
+        RPPA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rppa::nsr_region_kernel<true>, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
+    } else {
+        hipLaunchKernelGGL(rppa::nsr_blocks_kernel<false>, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
+        RPPA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rppa::nsr_region_kernel<false>, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
+    }
     RPPA_HIP(hipGetLastError());
     return RPP_SUCCESS;
 }
