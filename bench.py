@@ -245,6 +245,8 @@ def main() -> int:
                     help="duration of the served-stream measurement (Poisson arrivals, 4000 clips/s); 0 skips it")
     ap.add_argument("--front-end-only", action="store_true",
                     help="only the workload-4 front end (down-mix + resample, then fused log-mel; profiling runs)")
+    ap.add_argument("--trim-only", action="store_true",
+                    help="only the workload-1 detection + region log-mel side line (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus > 1 is not supported "
@@ -257,6 +259,12 @@ def main() -> int:
     torch.cuda.set_device(0)
     if a.front_end_only:
         print(json.dumps({"front_end": front_end_measurement(api, synth, torch, a.precision, steps=a.steps)}))
+        return 0
+    if a.trim_only:
+        f = synth.make(1).params
+        p = api.logmel_params(nfft=f["nfft"], window_length=f["W"], window_step=f["H"], sample_rate=f["sample_rate"],
+                              min_freq=f["fmin"], max_freq=f["fmax"], num_filter=f["nmel"])
+        print(json.dumps({"trim_workload1": trim_measurement(api, synth, torch, p, a.precision, steps=a.steps)}))
         return 0
     if a.copy_calibration_mb:
         n = a.copy_calibration_mb * (1 << 20) // 4
