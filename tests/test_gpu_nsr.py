@@ -54,6 +54,31 @@ def test_nsr_vs_oracle_ragged(gpu, handle, orc, cutoff, W, ref, reset):
     assert np.array_equal(b, eb) and np.array_equal(n, en), (b, eb, n, en)
 
 
+@pytest.mark.parametrize("stride,offset,cutoff,W,ref,reset", [
+    (160003, 0, -40.0, 512, 0.0, 999),      # odd row stride, reset interval not a multiple of 4
+    (160008, 1, -45.0, 300, 0.0, -1),       # rows one sample off 16-byte alignment
+    (160008, 1, -30.0, 400, 0.01, 8192),    # the same with a fixed reference power
+    (160002, 0, -60.0, 2048, 0.0, 8190)])
+def test_nsr_vs_oracle_4byte_loads(gpu, handle, orc, stride, offset, cutoff, W, ref, reset):
+    """Rows that are not 16-byte aligned, or a reset interval that is not a multiple of 4, take the
+    4-byte-load recurrence (and its checkpoints) instead of the 16-byte one."""
+    rng = np.random.default_rng(abs(hash((stride, offset, cutoff, W, ref, reset))) % 2**32)
+    L = np.array([0, 1, 2, 5, 300, 2047, 2048, 2049, 8191, 8192, 8193, 30000, 77777, 160000], np.int32)
+    x = bursty(rng, L, stride)
+    flat = torch.zeros(len(L) * stride + offset, dtype=torch.float32, device="cuda")
+    xs = flat[offset:].view(len(L), stride)
+    xs.copy_(torch.from_numpy(x))
+    Ls = torch.from_numpy(L).cuda()
+    b = torch.full((len(L),), -7, dtype=torch.int32, device="cuda")
+    n = torch.full((len(L),), -7, dtype=torch.int32, device="cuda")
+    gpu.non_silent_region_detection(handle, xs, Ls, b, n, cutoff, W, ref, reset)
+    torch.cuda.synchronize()
+    b, n = b.cpu().numpy(), n.cpu().numpy()
+    eb, en = orc.non_silent_region_f32(x, L, cutoff, W, ref, reset, nthreads=8)
+    eb[L == 0], en[L == 0] = 0, 0   # HOST reads an empty buffer's maximum there; the GPU reports (0, 0)
+    assert np.array_equal(b, eb) and np.array_equal(n, en), (b, eb, n, en)
+
+
 def test_nsr_silence_and_constant(gpu, handle, orc):
     """All-zero clips report the whole clip (cut-off 0 is met everywhere); a constant burst reports
     the burst widened by W - 1 on both sides."""
@@ -71,6 +96,22 @@ def test_nsr_workload1_full(gpu, handle, orc):
     b, n = run(gpu, handle, wl.x, wl.lengths, -60.0, 2048, 0.0, 8192)
     eb, en = orc.non_silent_region_f32(wl.x, wl.lengths, -60.0, 2048, 0.0, 8192, nthreads=8)
     assert np.array_equal(b, eb) and np.array_equal(n, en)
+
+
+def test_nsr_workload1_bursts(gpu, handle, orc):
+    """The 512-clip batch with every clip made quiet (x 1e-4, -80 dB) outside one random burst, so
+    the regions begin and end inside reset blocks and inside the recorded 192-output spans."""
+    wl = synth.workload1()
+    rng = np.random.default_rng(11)
+    x = wl.x.copy()
+    for i, l in enumerate(wl.lengths):
+        a, c = sorted(rng.integers(0, l, 2))
+        x[i, :a] *= np.float32(1e-4)
+        x[i, c:l] *= np.float32(1e-4)
+    b, n = run(gpu, handle, x, wl.lengths, -60.0, 2048, 0.0, 8192)
+    eb, en = orc.non_silent_region_f32(x, wl.lengths, -60.0, 2048, 0.0, 8192, nthreads=8)
+    assert np.array_equal(b, eb) and np.array_equal(n, en)
+    assert (b > 0).mean() > 0.5 and (b + n < wl.lengths).mean() > 0.5   # interior regions
 
 
 def test_nsr_status(gpu, handle):
