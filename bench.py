@@ -250,7 +250,7 @@ def main() -> int:
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus > 1 is not supported "
-                                   "(clips shard across GPUs with no exchange; see DESIGN.md)"}))
+                                   "(clips shard across GPUs with no exchange between them)"}))
         return 2
 
     import torch

@@ -1,8 +1,8 @@
 """GPU parity of the spectrogram and the 4-call chain against the FP32 oracle (the restatement of
 RPP HOST pinned to the reference's goldens) across the parameter space of the API.
 
-Bar: spectrogram |err| / frame max <= 1e-5; mel power <= 1e-4 relative (north-star tolerance for
-the plain FP32 path); dB outputs within 4.5e-4 dB (= 1e-4 relative in power).
+Bar: spectrogram |err| / frame max <= 1e-5; mel power <= 1e-4 relative (the FP32 tolerance
+against RPP HOST); dB outputs within 4.5e-4 dB (= 1e-4 relative in power).
 """
 import numpy as np
 import pytest
