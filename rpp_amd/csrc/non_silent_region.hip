@@ -15,9 +15,9 @@
 //
 // Parallelism is the reset blocks: one lane runs one block's recurrence.  The recurrence's whole
 // state is the running sum, so pass 1 records it every kSeg outputs together with the largest sum
-// of those outputs (or, when the cut-off is known up front, each block's first and last hit);
-// pass 2, one workgroup per clip, reduces the records and resumes the recurrence from the
-// recorded sums over only the two kSeg-output spans holding the region's ends.
+// of those outputs; pass 2, one workgroup per clip, reduces the records to the cut-off (unless the
+// reference power fixes it) and resumes the recurrence from the recorded sums over only the two
+// kSeg-output spans holding the region's ends.
 #include <cmath>
 #include <type_traits>
 
@@ -48,8 +48,6 @@ struct NsrArgs {
     float *segMax;      // [B][maxSegs] maximum running sum over each checkpoint's outputs
     float *segSum;      // [B][maxSegs] running sum before each checkpoint's first output
     float *blockMax;    // [B][maxBlocks] maximum running sum over each reset block
-    int *blockFirst;    // [B][maxBlocks], used when refPower != 0
-    int *blockLast;
     int *begin;
     int *length;
     int vec;            // rows 16-byte aligned, nStride and R multiples of 4, batch under 2 GiB:
@@ -312,12 +310,10 @@ __device__ __forceinline__ void mms_run(const NsrArgs &a, const float *x, int b,
     }
 }
 
-// Pass 1: one lane per reset block.  With a known cut-off (referencePower != 0) it records the
-// block's first and last hit; otherwise, every kSeg outputs, the running sum the recurrence
+// Pass 1: one lane per reset block records, every kSeg outputs, the running sum the recurrence
 // resumes from and the largest running sum of those outputs, and the block's largest running sum
 // (mms = sum * (1/W) rounded is monotone in sum, so the largest mms is the largest sum times 1/W,
-// rounded).
-template <bool KnownCut>
+// rounded, and a block or span holds an mms >= cut exactly when its largest sum times 1/W does).
 __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, int B)
 {
     const long long lane = static_cast<long long>(blockIdx.x) * kPass1Threads + threadIdx.x;
@@ -334,47 +330,31 @@ __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, in
     const int end = static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(start) + R));
     const float *x = a.src + b * a.nStride;
     const float sum0 = fresh(a, x, b, L, start);
-    if constexpr (KnownCut) {
-        const float mf = a.meanFactor;
-        const float cut = a.refPower * a.cutOff;
-        int first = -1, last = -1;
-        auto visit = [&](int p, float s, bool valid) {
-            const bool hit = valid && s * mf >= cut;
-            first = (hit && first < 0) ? p : first;
-            last = hit ? p : last;
-        };
-        auto mark = [](int, float) {};
-        mms_run(a, x, b, L, start, end, sum0, visit, mark);
-        const long long cell = static_cast<long long>(b) * a.maxBlocks + j;
-        a.blockFirst[cell] = first;
-        a.blockLast[cell] = last;
-    } else {
-        float *segMax = a.segMax + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
-        float *segSum = a.segSum + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
-        float m = -INFINITY, mb = -INFINITY;
-        int k = -1;
-        auto visit = [&](int, float s, bool valid) { m = fmaxf(m, valid ? s : -INFINITY); };
-        auto mark = [&](int off, float s) {
-            if (off % kSeg == 0) {
-                if (k >= 0) {
-                    segMax[k] = m;
-                    mb = fmaxf(mb, m);
-                }
-                ++k;
-                m = -INFINITY;
-                segSum[k] = s;
+    float *segMax = a.segMax + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+    float *segSum = a.segSum + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+    float m = -INFINITY, mb = -INFINITY;
+    int k = -1;
+    auto visit = [&](int, float s, bool valid) { m = fmaxf(m, valid ? s : -INFINITY); };
+    auto mark = [&](int off, float s) {
+        if (off % kSeg == 0) {
+            if (k >= 0) {
+                segMax[k] = m;
+                mb = fmaxf(mb, m);
             }
-        };
-        mms_run(a, x, b, L, start, end, sum0, visit, mark);
-        segMax[k] = m;   // start < end: the first loop step marked checkpoint 0
-        a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, m);
-    }
+            ++k;
+            m = -INFINITY;
+            segSum[k] = s;
+        }
+    };
+    mms_run(a, x, b, L, start, end, sum0, visit, mark);
+    segMax[k] = m;   // start < end: the first loop step marked checkpoint 0
+    a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, m);
 }
 
-// Pass 2: one workgroup per clip reduces pass 1's records; without a known cut-off it finds the
-// first and the last reset block holding a hit, in them the first and the last checkpoint span
-// holding one, and reruns the recurrence over only those two spans from their recorded sums.
-template <bool KnownCut>
+// Pass 2: one workgroup per clip reduces pass 1's records to the cut-off (unless the reference
+// power fixes it), finds the first and the last reset block holding a hit, in them the first and
+// the last checkpoint span holding one, and reruns the recurrence over only those two spans from
+// their recorded sums.
 __global__ void __launch_bounds__(kPass2Threads) nsr_region_kernel(NsrArgs a)
 {
     const int b = blockIdx.x;
@@ -392,96 +372,85 @@ __global__ void __launch_bounds__(kPass2Threads) nsr_region_kernel(NsrArgs a)
     }
     __syncthreads();
 
-    if constexpr (KnownCut) {
-        // pass 1 already found every block's first and last hit
-        const int *bf = a.blockFirst + static_cast<long long>(b) * a.maxBlocks;
-        const int *bl = a.blockLast + static_cast<long long>(b) * a.maxBlocks;
-        for (int j = threadIdx.x; j < nb; j += kPass2Threads) {
-            if (bf[j] >= 0) {
-                atomicMin(&sBegin, bf[j]);
-                atomicMax(&sEnd, bl[j]);
-            }
-        }
-    } else {
-        const int R = a.R <= 0 ? L : a.R;
-        const float mf = a.meanFactor;
-        const float *bm = a.blockMax + static_cast<long long>(b) * a.maxBlocks;
-        float m = -INFINITY;
-        for (int j = threadIdx.x; j < nb; j += kPass2Threads) m = fmaxf(m, bm[j]);
-        sMax[threadIdx.x] = m;
+    const int R = a.R <= 0 ? L : a.R;
+    const float mf = a.meanFactor;
+    const float *bm = a.blockMax + static_cast<long long>(b) * a.maxBlocks;
+    float m = -INFINITY;
+    for (int j = threadIdx.x; j < nb; j += kPass2Threads) m = fmaxf(m, bm[j]);
+    sMax[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kPass2Threads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sMax[threadIdx.x] = fmaxf(sMax[threadIdx.x], sMax[threadIdx.x + s]);
         __syncthreads();
-        for (int s = kPass2Threads / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) sMax[threadIdx.x] = fmaxf(sMax[threadIdx.x], sMax[threadIdx.x + s]);
-            __syncthreads();
+    }
+    // the reference power, else the largest mms, times the cut-off ratio
+    const float cut = a.refPower != 0.0f ? a.refPower * a.cutOff : (sMax[0] * mf) * a.cutOff;
+    // the first and the last reset block holding a hit, then the first checkpoint span of the
+    // one and the last of the other holding a hit
+    for (int j = threadIdx.x; j < nb; j += kPass2Threads) {
+        if (bm[j] * mf >= cut) {
+            atomicMin(&sFirstBlk, j);
+            atomicMax(&sLastBlk, j);
         }
-        const float cut = (sMax[0] * mf) * a.cutOff;   // the largest mms times the cut-off ratio
-        // the first and the last reset block holding a hit, then the first checkpoint span of the
-        // one and the last of the other holding a hit
-        for (int j = threadIdx.x; j < nb; j += kPass2Threads) {
-            if (bm[j] * mf >= cut) {
-                atomicMin(&sFirstBlk, j);
-                atomicMax(&sLastBlk, j);
-            }
-        }
-        __syncthreads();
-        if (sLastBlk >= 0) {
-            const float *sm = a.segMax + static_cast<long long>(b) * a.maxSegs;
+    }
+    __syncthreads();
+    if (sLastBlk >= 0) {
+        const float *sm = a.segMax + static_cast<long long>(b) * a.maxSegs;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int j = h == 0 ? sFirstBlk : sLastBlk;
-                const long long blockEnd = min(static_cast<long long>(L), static_cast<long long>(j) * R + R);
-                const int segs = static_cast<int>((blockEnd - static_cast<long long>(j) * R + kSeg - 1) / kSeg);
-                for (int k = threadIdx.x; k < segs; k += kPass2Threads) {
-                    const int g = j * a.segsPerBlock + k;
-                    if (sm[g] * mf >= cut) {
-                        if (h == 0) atomicMin(&sFirst, g);
-                        else atomicMax(&sLast, g);
-                    }
+        for (int h = 0; h < 2; h++) {
+            const int j = h == 0 ? sFirstBlk : sLastBlk;
+            const long long blockEnd = min(static_cast<long long>(L), static_cast<long long>(j) * R + R);
+            const int segs = static_cast<int>((blockEnd - static_cast<long long>(j) * R + kSeg - 1) / kSeg);
+            for (int k = threadIdx.x; k < segs; k += kPass2Threads) {
+                const int g = j * a.segsPerBlock + k;
+                if (sm[g] * mf >= cut) {
+                    if (h == 0) atomicMin(&sFirst, g);
+                    else atomicMax(&sLast, g);
                 }
             }
-            __syncthreads();
-            // the squares of the two spans' samples and trailing samples x[p - W + 1] go to LDS
-            // (0 where p < W - 1: x - 0 is x, the HOST's skipped subtraction); one lane per span
-            // runs the additions in order and keeps every running sum; then all threads search
-            __shared__ float sQ[2][kSeg], sR[2][kSeg], sS[2][kSeg];
-            const float *x = a.src + b * a.nStride;
-            int s0[2], n[2];
+        }
+        __syncthreads();
+        // the squares of the two spans' samples and trailing samples x[p - W + 1] go to LDS
+        // (0 where p < W - 1: x - 0 is x, the HOST's skipped subtraction); one lane per span
+        // runs the additions in order and keeps every running sum; then all threads search
+        __shared__ float sQ[2][kSeg], sR[2][kSeg], sS[2][kSeg];
+        const float *x = a.src + b * a.nStride;
+        int s0[2], n[2];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int g = h == 0 ? sFirst : sLast;
-                const int j = g / a.segsPerBlock;
-                s0[h] = j * R + (g - j * a.segsPerBlock) * kSeg;
-                const long long e0 = min(static_cast<long long>(L),
-                                         min(static_cast<long long>(s0[h]) + kSeg, static_cast<long long>(j) * R + R));
-                n[h] = static_cast<int>(e0 - s0[h]);
-                for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
-                    const int t = s0[h] + i - a.W + 1;
-                    const float v = x[s0[h] + i];
-                    const float w = t >= 0 ? x[t] : 0.0f;
-                    sQ[h][i] = v * v;
-                    sR[h][i] = w * w;
-                }
+        for (int h = 0; h < 2; h++) {
+            const int g = h == 0 ? sFirst : sLast;
+            const int j = g / a.segsPerBlock;
+            s0[h] = j * R + (g - j * a.segsPerBlock) * kSeg;
+            const long long e0 = min(static_cast<long long>(L),
+                                     min(static_cast<long long>(s0[h]) + kSeg, static_cast<long long>(j) * R + R));
+            n[h] = static_cast<int>(e0 - s0[h]);
+            for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
+                const int t = s0[h] + i - a.W + 1;
+                const float v = x[s0[h] + i];
+                const float w = t >= 0 ? x[t] : 0.0f;
+                sQ[h][i] = v * v;
+                sR[h][i] = w * w;
             }
-            __syncthreads();
-            if (threadIdx.x == 0 || threadIdx.x == kWave) {
-                const int h = threadIdx.x == 0 ? 0 : 1;
-                float sum = a.segSum[static_cast<long long>(b) * a.maxSegs + (h == 0 ? sFirst : sLast)];
-                const int nh = n[h];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 || threadIdx.x == kWave) {
+            const int h = threadIdx.x == 0 ? 0 : 1;
+            float sum = a.segSum[static_cast<long long>(b) * a.maxSegs + (h == 0 ? sFirst : sLast)];
+            const int nh = n[h];
 #pragma unroll 8
-                for (int i = 0; i < nh; i++) {
-                    sum = sum + sQ[h][i];
-                    sS[h][i] = sum;
-                    sum = sum - sR[h][i];
-                }
+            for (int i = 0; i < nh; i++) {
+                sum = sum + sQ[h][i];
+                sS[h][i] = sum;
+                sum = sum - sR[h][i];
             }
-            __syncthreads();
+        }
+        __syncthreads();
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
-                    if (sS[h][i] * mf >= cut) {
-                        if (h == 0) atomicMin(&sBegin, s0[h] + i);
-                        else atomicMax(&sEnd, s0[h] + i);
-                    }
+        for (int h = 0; h < 2; h++) {
+            for (int i = threadIdx.x; i < n[h]; i += kPass2Threads) {
+                if (sS[h][i] * mf >= cut) {
+                    if (h == 0) atomicMin(&sBegin, s0[h] + i);
+                    else atomicMax(&sEnd, s0[h] + i);
                 }
             }
         }
@@ -542,30 +511,18 @@ RPPA_EXPORT RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, Rppt
     a.srcBytes = a.vec ? static_cast<int>(static_cast<long long>(B) * nStride * 4) : 0;
     a.segsPerBlock = static_cast<int>(((resetInterval <= 0 ? nStride : resetInterval) + rppa::kSeg - 1) / rppa::kSeg);
     a.maxSegs = resetInterval <= 0 ? a.segsPerBlock : a.maxBlocks * a.segsPerBlock;
-    const bool knownCut = referencePower != 0.0f;
     const size_t cells = static_cast<size_t>(B) * a.maxBlocks;
-    const size_t segCells = knownCut ? 0 : static_cast<size_t>(B) * a.maxSegs;
-    char *ws = static_cast<char *>(rppa::workspace(rppHandle, knownCut ? cells * 8 : segCells * 8 + cells * 4));
+    const size_t segCells = static_cast<size_t>(B) * a.maxSegs;
+    char *ws = static_cast<char *>(rppa::workspace(rppHandle, segCells * 8 + cells * 4));
     if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
-    if (knownCut) {
-        a.blockFirst = reinterpret_cast<int *>(ws);
-        a.blockLast = reinterpret_cast<int *>(ws + cells * 4);
-    } else {
-        a.segMax = reinterpret_cast<float *>(ws);
-        a.segSum = reinterpret_cast<float *>(ws + segCells * 4);
-        a.blockMax = reinterpret_cast<float *>(ws + segCells * 8);
-    }
+    a.segMax = reinterpret_cast<float *>(ws);
+    a.segSum = reinterpret_cast<float *>(ws + segCells * 4);
+    a.blockMax = reinterpret_cast<float *>(ws + segCells * 8);
 
     const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Threads - 1) / rppa::kPass1Threads);
-    if (knownCut) {
-        hipLaunchKernelGGL(rppa::nsr_blocks_kernel<true>, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
-        RPPA_HIP(hipGetLastError());
-        hipLaunchKernelGGL(rppa::nsr_region_kernel<true>, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
-    } else {
-        hipLaunchKernelGGL(rppa::nsr_blocks_kernel<false>, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
-        RPPA_HIP(hipGetLastError());
-        hipLaunchKernelGGL(rppa::nsr_region_kernel<false>, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
-    }
+    hipLaunchKernelGGL(rppa::nsr_blocks_kernel, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
+    RPPA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(rppa::nsr_region_kernel, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
     RPPA_HIP(hipGetLastError());
     return RPP_SUCCESS;
 }
