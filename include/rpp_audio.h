@@ -20,6 +20,9 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>   /* the reference's signatures take bool */
+#endif
 
 #ifdef __cplusplus
 #include <vector>
@@ -301,7 +304,9 @@ RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLogMelPrecision
 RppStatus rppa_server_destroy(rppaServer *server);
 /* copies `length` host samples; RPP_ERROR_NOT_ENOUGH_MEMORY when the clip can never fit a slot */
 RppStatus rppa_server_submit(rppaServer *server, const float *samples, int length, long long *id);
-/* blocks until `id` is done; out receives [numFilter][frames] dB rows when outputs are kept */
+/* blocks until `id` is done; out receives [numFilter][frames] dB rows when outputs are kept.  Several
+ * threads may wait on one id (the request is freed when the last returns).  Without kept outputs a
+ * request that already finished reports success with *frames = -1 (out must then be NULL). */
 RppStatus rppa_server_wait(rppaServer *server, long long id, float *out, long long outCapacity, int *frames,
                            double *latencyUs);
 RppStatus rppa_server_drain(rppaServer *server, int timeoutMs);

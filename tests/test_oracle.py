@@ -331,3 +331,134 @@ def test_ref_non_silent_region(ref_power, reset):
     assert r.ref_non_silent_region_detection(x, 30008, L, 4, -40.0, 512, ref_power, reset, b0, n0, 2) == 0
     b1, n1 = O.non_silent_region_f32(x, L, -40.0, 512, ref_power, reset)
     assert np.array_equal(b0, b1) and np.array_equal(n0, n1)
+
+
+# ------------------------------------------------------------------ the FP64 chain (the 0.05 dB judge)
+# Every mixed-precision bound is measured against logmel_f64 = pre_emphasis_f64 -> spectrogram_f64 ->
+# mel_f64 -> to_db_f64.  The FP64 spectrogram and mel are pinned above (numpy FFT, Parseval, filter
+# properties); these pin the two remaining stages and the composition.
+@pytest.mark.parametrize("border", ["zero", "clamp", "reflect"])
+def test_pre_emphasis_f64_pins(border):
+    """c = 0 is the identity; a constant k gives (1 - c) k after the first sample; the first sample
+    takes the border's value (CLAMP x[0], REFLECT x[1], ZERO 0: cpu/kernel/pre_emphasis_filter.cpp:46-48);
+    nothing is written past a clip; and the FP64 result agrees with the golden-pinned FP32 one to
+    within the FP32 rounding of its product and difference."""
+    rng = np.random.default_rng(20)
+    L = np.array([1, 2, 7, 300, 4093], np.int32)
+    x = rng.uniform(-1, 1, (len(L), 4096)).astype(np.float32)
+    assert np.array_equal(O.pre_emphasis_f64(x, L, 0.0, border)[valid_mask(x.shape, L)],
+                          x.astype(np.float64)[valid_mask(x.shape, L)])
+    k = np.full((1, 64), 0.75, np.float32)
+    yk = O.pre_emphasis_f64(k, [64], 0.97, border)[0]
+    c97 = float(np.float32(0.97))     # the coefficient arrives as FP32 (coeffTensor)
+    assert np.allclose(yk[1:], (1 - c97) * 0.75, rtol=1e-15, atol=0)
+    first = {"zero": 0.75, "clamp": (1 - c97) * 0.75, "reflect": (1 - c97) * 0.75}[border]
+    assert abs(yk[0] - first) < 1e-15
+    c = np.float32(0.9)
+    y = O.pre_emphasis_f64(x, L, c, border)
+    bval = {"zero": 0.0, "clamp": x[:, 0], "reflect": x[:, 1]}[border]
+    assert np.allclose(y[:, 0][L > 1], (x[:, 0] - np.float64(c) * bval)[L > 1], rtol=0, atol=1e-15)
+    for b, n in enumerate(L):
+        assert np.all(y[b, n:] == 0.0)
+        if n > 1:   # interior samples: x[t] - c x[t-1] (an index slip or sign error fails here)
+            assert np.allclose(y[b, 1:n], x[b, 1:n].astype(np.float64) - np.float64(c) * x[b, :n - 1], rtol=0,
+                               atol=1e-15)
+    y32 = O.pre_emphasis_f32(x, L, c, border)
+    m = valid_mask(x.shape, L)
+    assert np.abs(y32[m] - y[m]).max() <= 2.0 ** -23 * 2.0   # |x| <= 1, |c x| < 1: two FP32 roundings
+
+
+def test_to_db_f64_closed_form_and_clamp():
+    """10 log10(max(x, 1e-8)) for multiplier 10 / cut-off -80 (values below 1e-8 clamp to -80 dB
+    exactly); 20 log10(max(x, 1e-3)) for multiplier 20 / cut-off -60; FP32 restatement within its
+    rounding."""
+    rng = np.random.default_rng(21)
+    x = 10.0 ** rng.uniform(-12, 2, (2, 12, 30))
+    y = O.to_db_f64(x, [30, 30], [12, 12], -80.0, 10.0, 1.0)
+    assert np.allclose(y, 10 * np.log10(np.maximum(x, 1e-8)), rtol=0, atol=1e-12)
+    low = x < 1e-8
+    assert low.any() and np.all(y[low] == -80.0)
+    y20 = O.to_db_f64(x, [30, 30], [12, 12], -60.0, 20.0, 1.0)
+    assert np.allclose(y20, 20 * np.log10(np.maximum(x, 1e-3)), rtol=0, atol=1e-12)
+    # a reference magnitude r divides first: 10 log10(max(x / r, 1e-8))
+    y25 = O.to_db_f64(x, [30, 30], [12, 12], -80.0, 10.0, 2.5)
+    assert np.allclose(y25, 10 * np.log10(np.maximum(x / 2.5, 1e-8)), rtol=0, atol=1e-12)
+    y32 = O.to_db_f32(x.astype(np.float32), [30, 30], [12, 12], -80.0, 10.0, 1.0)
+    assert np.abs(y32 - y).max() < 2e-5
+
+
+def test_to_db_f64_clip_max_reference():
+    """Reference magnitude 0: each clip's maximum over its valid region maps to 0 dB and every
+    value to 10 log10(x / max) (cpu/kernel/to_decibels.cpp:58-75); cells outside a clip's
+    width x height are neither read nor written."""
+    rng = np.random.default_rng(22)
+    x = 10.0 ** rng.uniform(-6, 1, (2, 10, 16))
+    x[1, :, 9:] = 1e6          # outside clip 1's width (9): must not become its maximum
+    y = O.to_db_f64(x, [16, 9], [10, 10], -80.0, 10.0, 0.0)
+    for b, w in enumerate([16, 9]):
+        v = x[b, :, :w]
+        ref = 10 * np.log10(np.maximum(v / v.max(), 1e-8))
+        assert np.allclose(y[b, :, :w], ref, rtol=0, atol=1e-12)
+        assert y[b, :, :w].max() == 0.0
+    assert np.all(y[1, :, 9:] == 0.0)
+
+
+def test_to_db_f64_one_column_reading():
+    """A one-column clip is read as 1-D data (height values contiguous from the start,
+    cpu/kernel/to_decibels.cpp:76-81) unless force2d keeps the [row][hStride] reading."""
+    x = np.zeros((1, 5, 8))
+    x[0].flat[:5] = [1.0, 0.1, 0.01, 1e-3, 1e-4]     # what the 1-D reading sees
+    x[0, :, 0] = [1.0, 1e-9, 1e-9, 1e-9, 1e-9]      # what the 2-D reading sees (x[r*8])
+    y1 = O.to_db_f64(x, [1], [5], -80.0, 10.0, 1.0)
+    assert np.allclose(y1[0].flat[:5], [0.0, -10.0, -20.0, -30.0, -40.0], atol=1e-12)
+    y2 = O.to_db_f64(x, [1], [5], -80.0, 10.0, 1.0, force2d=True)
+    assert np.allclose(y2[0, :, 0], [0.0, -80.0, -80.0, -80.0, -80.0], atol=1e-12)
+
+
+def _features(bt):
+    p = bt.params
+    return dict(nfft=p["nfft"], W=p["W"], H=p["H"], nmel=p["nmel"], sample_rate=p["sample_rate"], fmin=p["fmin"],
+                fmax=p["fmax"])
+
+
+@pytest.mark.parametrize("workload,n", [(0, 8), (1, 24), (2, 6)])
+def test_logmel_f64_matches_pinned_f32_chain(workload, n):
+    """The FP64 chain and the golden-pinned FP32 chain agree to the FP32 parity bar (1e-4 relative
+    in mel power with a floor at 1e-6 of the frame's loudest band), with the same frame counts;
+    measured |dB| differences are a few 1e-3 at most."""
+    from helpers import fp32_bar, max_db_err
+    from rpp_amd import synth
+    bt = synth.make(workload)
+    x, L = bt.x[:n], bt.lengths[:n]
+    d32, f32 = O.logmel_f32(x, L, nthreads=4, **_features(bt))
+    d64, mel, f64 = O.logmel_f64(x, L, nthreads=4, **_features(bt))
+    assert np.array_equal(f32, f64)
+    assert fp32_bar(d32, d64, f32) <= 1.0
+    assert max_db_err(d32, d64, f32) < 1e-2
+    # the returned mel power is the dB's argument: 10 log10(max(mel, 1e-8)) == db
+    for b, F in enumerate(f64):
+        assert np.allclose(d64[b, :, :F], 10 * np.log10(np.maximum(mel[b, :, :F], 1e-8)), rtol=0, atol=1e-9)
+
+
+def test_logmel_f64_independent_composition():
+    """The FP64 chain against a composition written here from numpy alone: pre-emphasis by array
+    arithmetic, numpy reflect framing, the half-sample-shifted Hann window, numpy's rfft, the mel
+    matrix of the pinned FP64 filter bank (applied as a matrix product) and numpy's log10."""
+    rng = np.random.default_rng(23)
+    L = np.array([7000, 4001], np.int32)
+    x = rng.uniform(-1, 1, (2, 7008)).astype(np.float32)
+    nfft, W, H, M = 512, 400, 160, 80
+    db, mel, frames = O.logmel_f64(x, L, nfft=nfft, W=W, H=H, nmel=M, fmax=8000.0)
+    K = nfft // 2 + 1
+    eye = np.eye(K)[None]
+    Wm = O.mel_f64(eye, [K], M, 16000.0, 0.0, 8000.0, "slaney", True)[0]        # [M, K]
+    t = np.arange(W)
+    win = 0.5 * (1 - np.cos(2 * np.pi * (t + 0.5) / W))
+    for b, n in enumerate(L):
+        s = x[b, :n].astype(np.float64)
+        pe = s - float(np.float32(0.97)) * np.concatenate([[s[0]], s[:-1]])   # CLAMP border, FP32 coefficient
+        fr = frames_numpy(pe, int(n), W, H) * win
+        P = np.abs(np.fft.rfft(fr, n=nfft, axis=1)) ** 2                      # [F, K]
+        ref = 10 * np.log10(np.maximum(P @ Wm.T, 1e-8)).T                     # [M, F]
+        assert frames[b] == ref.shape[1]
+        assert np.allclose(db[b, :, :frames[b]], ref, rtol=0, atol=1e-9)
