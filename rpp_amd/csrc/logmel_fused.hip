@@ -526,11 +526,17 @@ __device__ __forceinline__ TileInfo tile_info(const LogMelArgs &a, int tile, int
 {
     TileInfo t;
     if (a.blockRec) {
-        // the block's own record k (written by this block at the start of the kernel; its 256-byte
-        // lines are read by no other block, so the scalar cache cannot hold them stale)
-        typedef int i32x4 __attribute__((ext_vector_type(4)));
+        // the block's own record k, written by this block earlier in this launch: a plain global load
+        // (the stores and a workgroup barrier precede it, so it sees them), made wave-uniform.  Not
+        // through the constant address space, whose loads the compiler may treat as invariant for
+        // the whole launch and move above the stores.
         const int k = (tile - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x);
-        const i32x4 r = ld_const(reinterpret_cast<const i32x4 *>(a.blockRec), static_cast<int>(blockIdx.x) * a.kPad + k);
+        const int4 g = a.blockRec[static_cast<size_t>(blockIdx.x) * a.kPad + k];
+        int4 r;
+        r.x = __builtin_amdgcn_readfirstlane(g.x);
+        r.y = __builtin_amdgcn_readfirstlane(g.y);
+        r.z = __builtin_amdgcn_readfirstlane(g.z);
+        r.w = __builtin_amdgcn_readfirstlane(g.w);
         ClipView v;
         v.L = r.z;
         v.x = a.src + static_cast<size_t>(r.x) * a.srcStride + r.w;
@@ -1066,9 +1072,8 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
                 a.blockRec[static_cast<size_t>(blockIdx.x) * a.kPad + k] = make_int4(lo, (tile - ts[lo]) * FT, L, start);
             }
         }
-        // the records are read back through the scalar cache, which misses to this XCD's L2 (the
-        // block's own): the stores must have reached it, and a workgroup barrier alone does not
-        // wait for them
+        // the records are read back by this block's waves with ordinary global loads; the stores
+        // are waited for explicitly before the barrier (once per launch)
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
         __syncthreads();   // records stored; the segment region is free again
     } else {
@@ -1274,6 +1279,9 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
             wh[slot] = *reinterpret_cast<const u16x8 *>(a.mHi + o);
             wl[slot] = *reinterpret_cast<const u16x8 *>(a.mLo + o);
         };
+        // the next tile's record first: its load is then not queued behind the weight fragments
+        const int nt = tile + gridDim.x;
+        if (nt < T) cur = tile_info(a, nt, FT, span, lane);
         if (n > 0) {
 #pragma unroll
             for (int u = 0; u < kWA; u++) loadW(u, u);
@@ -1281,15 +1289,8 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
 
         // ---- the next tile's samples load while the mel product runs (the accumulators are dead)
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
-        {
-            const int nt = tile + gridDim.x;
-            if (nt < T) {
-                cur = tile_info(a, nt, FT, span, lane);
-                seg_prefetch(pf, cur, span, tid);
-            } else {
-                seg_prefetch_none(pf);
-            }
-        }
+        if (nt < T) seg_prefetch(pf, cur, span, tid);
+        else seg_prefetch_none(pf);
 
         // ---- mel^T = Wmel [M x KP] . pw^T [KP x FT] on MFMA 16x16x32 bf16 split hi+lo, over each
         tid = opaque(threadIdx.x); wave = tid >> 6; lane = tid & 63;
@@ -1457,7 +1458,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.H = p->windowStep;
     a.center = p->centerWindows;
     a.reflect = p->reflectPadding;
-    a.power = p->power;
+    a.power = p->power == 2 ? 2 : 1;   // rppt_spectrogram_gpu's rule: 2 squared magnitude, else magnitude
     a.K = K;
     a.M = p->numFilter;
     // dB constants exactly as cpu/kernel/to_decibels.cpp:38-43 computes them

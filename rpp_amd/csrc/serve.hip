@@ -44,6 +44,7 @@ struct Request {
     int length = 0, frames = 0;
     Clock::time_point arrival, done;
     bool finished = false;
+    int waiters = 0;          // rppa_server_wait calls blocked on this request (they own its lifetime)
     std::vector<float> out;
 };
 
@@ -135,7 +136,9 @@ void finish_batch(rppaServer *s, Slot &sl)
         s->latUs.push_back(std::chrono::duration<double, std::micro>(r->done - r->arrival).count());
         s->audioSamples += r->length;
         std::vector<float>().swap(r->samples);
-        if (!s->cfg.keepOutputs) {
+        // without kept outputs nothing is left to collect: forget the request now, unless a waiter
+        // holds it (the last waiter frees it)
+        if (!s->cfg.keepOutputs && r->waiters == 0) {
             s->live.erase(r->id);
             delete r;
         }
@@ -156,7 +159,7 @@ void fail_requests(rppaServer *s, std::vector<Request *> &reqs)
     for (Request *r : reqs) {
         r->finished = true;
         r->done = Clock::now();
-        if (!s->cfg.keepOutputs) {
+        if (!s->cfg.keepOutputs && r->waiters == 0) {
             s->live.erase(r->id);
             delete r;
         }
@@ -373,15 +376,27 @@ RPPA_EXPORT RppStatus rppa_server_submit(rppaServer *s, const float *samples, in
 
 // Blocks until request `id` is done; copies its [numFilter][frames] dB rows into out (when the
 // server keeps outputs and out is non-null, up to outCapacity floats) and forgets the request.
+// A request is held alive by its waiters: the scheduler never frees one that a waiter is blocked on,
+// and the last waiter to return frees it.  Without kept outputs a request that already finished
+// (and was forgotten) reports success with *frames = -1 and no latency.
 RPPA_EXPORT RppStatus rppa_server_wait(rppaServer *s, long long id, float *out, long long outCapacity, int *frames,
                                        double *latencyUs)
 {
     if (!s) return RPP_ERROR_INVALID_ARGUMENTS;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->live.find(id);
-    if (it == s->live.end()) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (it == s->live.end()) {
+        if (!s->cfg.keepOutputs && id >= 1 && id < s->nextId) {
+            if (frames) *frames = -1;
+            if (latencyUs) *latencyUs = -1.0;
+            return out ? RPP_ERROR_INSUFFICIENT_DST_BUFFER_LENGTH : RPP_SUCCESS;
+        }
+        return RPP_ERROR_INVALID_ARGUMENTS;
+    }
     Request *r = it->second;
+    r->waiters++;
     s->cvDone.wait(lk, [&] { return r->finished; });
+    r->waiters--;
     if (frames) *frames = r->frames;
     if (latencyUs) *latencyUs = std::chrono::duration<double, std::micro>(r->done - r->arrival).count();
     RppStatus st = RPP_SUCCESS;
@@ -389,8 +404,10 @@ RPPA_EXPORT RppStatus rppa_server_wait(rppaServer *s, long long id, float *out, 
         if (static_cast<long long>(r->out.size()) > outCapacity || r->out.empty()) st = RPP_ERROR_INSUFFICIENT_DST_BUFFER_LENGTH;
         else std::memcpy(out, r->out.data(), sizeof(float) * r->out.size());
     }
-    s->live.erase(it);
-    delete r;
+    if (r->waiters == 0) {
+        s->live.erase(r->id);   // looked up again: the iterator may be stale after the wait
+        delete r;
+    }
     return st;
 }
 

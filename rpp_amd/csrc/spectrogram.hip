@@ -3,7 +3,9 @@
 // Semantics of cpu/kernel/spectrogram.cpp:71-251: F = L/H + 1 centered frames (else (L-W)/H + 1),
 // frame f starts at f*H - (center ? W/2 : 0), out-of-range samples are mirrored without
 // repeating the edge (reflect) or zero, the frame is multiplied by the window, and bin
-// k in [0, N/2] gets |X_k|^2 (power != 1) or |X_k| (power == 1).  NFT writes
+// k in [0, N/2] gets |X_k|^2 when power == 2 and |X_k| for any other value -- the rule of the
+// GPU entry point this replaces (hip/kernel/spectrogram.cpp:231-232; the HOST kernel instead
+// takes |X_k| only for power == 1, cpu/kernel/spectrogram.cpp:217-242).  NFT writes
 // dst[k*hStride + f], NTF writes dst[f*hStride + k]; only frames < F are written.
 //
 // The DFT is a dense FP32 contraction  C[f][c] = sum_t x[f*H - off + t] * basis[t][c]  against
@@ -204,7 +206,7 @@ RPPA_EXPORT RppStatus rppt_spectrogram_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescP
     a.N = nfft;
     a.W = windowLength;
     a.H = windowStep;
-    a.power = power;
+    a.power = power == 2 ? 2 : 1;   // the kernel's flag: 1 = magnitude, 2 = squared magnitude
     a.cols = t->cols;
     a.basis = t->basis;
     dim3 grid(rppa::ceil_div(maxFrames, rppa::kFT), B);

@@ -68,6 +68,14 @@ class Server:
                                                               C.byref(lat)))
         return out[:num_filter * fr.value].reshape(num_filter, fr.value), lat.value
 
+    def wait_timing(self, rid: int):
+        """Waits without collecting outputs; returns (frames, latency_us), (-1, -1.0) for a request the
+        server already forgot (outputs not kept)."""
+        fr = C.c_int()
+        lat = C.c_double()
+        api.check("rppa_server_wait", _bind().rppa_server_wait(self.s, rid, None, 0, C.byref(fr), C.byref(lat)))
+        return fr.value, lat.value
+
     def drain(self, timeout_ms=60000):
         api.check("rppa_server_drain", _bind().rppa_server_drain(self.s, timeout_ms))
 

@@ -41,6 +41,10 @@ SPEC_CASES = [
     (256, 256, 100, True, True, 2, "NTF"),
     (400, 400, 160, True, True, 2, "NFT"),     # non-power-of-two nfft
     (401, 300, 128, True, True, 2, "NFT"),     # odd nfft
+    # power other than 1 and 2 follows the GPU entry point being replaced: 2 gives |X|^2, any other
+    # value |X| (hip/kernel/spectrogram.cpp:231-232)
+    (512, 400, 160, True, True, 0, "NFT"),
+    (512, 400, 160, True, True, 3, "NTF"),
 ]
 
 
@@ -51,7 +55,8 @@ def test_spectrogram_params(gpu, handle, orc, case):
     L = np.array([1, 2, W - 1, W, W + 1, 3 * H + 7, 5000, 12345], np.int32)
     S = int(L.max()) + 8
     x = rng.uniform(-1, 1, (len(L), S)).astype(np.float32)
-    ref = orc.spectrogram_f32(x, L, nfft, W, H, center, reflect, power, layout)
+    # the oracle restates the HOST kernel, whose rule is power == 1 -> |X|, else |X|^2
+    ref = orc.spectrogram_f32(x, L, nfft, W, H, center, reflect, 2 if power == 2 else 1, layout)
     frames = np.array([orc.num_windows(int(l), W, H, center) for l in L])
     out = torch.full(ref.shape, -1.0, device="cuda")
     gpu.spectrogram(handle, dev(x), out, dev(L), nfft=nfft, window_length=W, window_step=H, center=center,

@@ -1,0 +1,189 @@
+"""Each workload at its own full size, through the C-ABI.
+
+The reference's HIP path cannot run batched workloads 1 and 2 in one call
+(N*(maxFrames*B + 2K) floats of scratch above SPECTROGRAM_MAX_SCRATCH_MEMORY, api/rppdefs.h:88,
+src/modules/handle_hip.cpp:217-229); this library has no such limit, so these tests show the whole
+batches are right:
+
+  workload 1  batch 512 ragged 0.5-20 s at 16 kHz: the autotuned fused launch replayed from a
+              hipGraph equals, bit for bit, the same clips run as eight 64-clip batches, and 64 clips
+              (the 32 longest and 32 spread over the rest) are within 0.05 dB of the FP64 oracle
+  workload 2  batch 256, 2-10 s at 44.1 kHz, nfft 1024, 128 mels: the same checks
+  workload 4  60-s stereo 48 kHz clips (6,001 frames each): down-mix + resample bit-exact against
+              the oracle, the log-mel within 0.05 dB of FP64
+  workload 3  served clips of 10-30 s: the same alone and in the mix, within 0.05 dB of FP64
+  the 1x path rppt_logmel_chain_gpu, per clip and batched, against the FP32 oracle chain
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import fp32_bar, max_db_err
+from rpp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+DB_TOL_MIXED = 0.05
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def feat_kw(p):
+    return dict(nfft=p["nfft"], W=p["W"], H=p["H"], nmel=p["nmel"], sample_rate=p["sample_rate"], fmin=p["fmin"],
+                fmax=p["fmax"])
+
+
+def params_for(gpu, p):
+    return gpu.logmel_params(nfft=p["nfft"], window_length=p["W"], window_step=p["H"], sample_rate=p["sample_rate"],
+                             min_freq=p["fmin"], max_freq=p["fmax"], num_filter=p["nmel"])
+
+
+def oracle_sample(L, n=64):
+    """The n/2 longest clips and n/2 spread evenly over the others."""
+    order = np.argsort(L)[::-1]
+    longest = order[:n // 2]
+    rest = np.sort(order[n // 2:])
+    spread = rest[np.linspace(0, len(rest) - 1, n // 2).astype(int)]
+    return np.sort(np.concatenate([longest, spread]))
+
+
+def full_batch_check(gpu, orc, bt, name):
+    p = bt.params
+    B = len(bt.lengths)
+    L = bt.lengths
+    frames = L // p["H"] + 1
+    F = int(frames.max())
+    x = dev(bt.x)
+    Ld = dev(L)
+    prm = params_for(gpu, p)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        h = gpu.Handle(B, stream=s)
+        out = torch.full((B, p["nmel"], F), float("nan"), device="cuda")
+        (ft, bpc), ms = gpu.autotune(h, x, out, Ld, prm, "fp16x3", iters=2)
+        print(f"{name}: autotuned framesPerTile {ft}, blocksPerCU {bpc}: {ms:.3f} ms per launch")
+        gpu.logmel_fused(h, x, out, Ld, prm, "fp16x3")
+        s.synchronize()
+        eager = out.clone()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            gpu.logmel_fused(h, x, out, Ld, prm, "fp16x3")
+    for i in range(3):
+        out.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(out, nan=-7.0), torch.nan_to_num(eager, nan=-7.0)), ("replay", i)
+    h.close()
+    full = eager.cpu().numpy()
+    del g, out, eager
+    # the same clips in 64-clip batches, each its own launch on a fresh handle
+    h = gpu.Handle(64)
+    for b0 in range(0, B, 64):
+        sl = slice(b0, min(B, b0 + 64))
+        Fb = int(frames[sl].max())
+        part = torch.full((sl.stop - sl.start, p["nmel"], Fb), float("nan"), device="cuda")
+        gpu.logmel_fused(h, x[sl], part, Ld[sl], prm, "fp16x3")
+        torch.cuda.synchronize()
+        pa = part.cpu().numpy()
+        for j, b in enumerate(range(sl.start, sl.stop)):
+            assert np.array_equal(pa[j, :, :frames[b]], full[b, :, :frames[b]]), ("64-clip batch", b)
+    h.close()
+    idx = oracle_sample(L)
+    ref, _, fr = orc.logmel_f64(bt.x[idx], L[idx], nthreads=8, **feat_kw(p))
+    err = max_db_err(full[idx], ref, fr)
+    print(f"{name}: {len(idx)} clips (up to {int(fr.max())} frames) max |dB err| vs FP64 = {err:.3e}")
+    assert err <= DB_TOL_MIXED
+    return full
+
+
+def test_workload1_full_batch(gpu, orc):
+    full_batch_check(gpu, orc, synth.workload1(), "workload1 (512 clips)")
+
+
+def test_workload2_full_batch(gpu, orc):
+    full_batch_check(gpu, orc, synth.workload2(), "workload2 (256 clips)")
+
+
+def test_workload4_sixty_second_clips(gpu, handle, orc):
+    """Two 60-s stereo 48 kHz clips, int16-range values: 2,880,000 samples per channel, 960,000
+    after resampling, 6,001 frames each."""
+    bt = synth.workload4(batch=2, seconds=60.0)
+    lut, sc, ce = synth.kaiser_sinc_lut()
+    p = gpu.logmel_params()
+    db, res = gpu.front_end_logmel(handle, dev(bt.x), dev(bt.lengths), 2, dev(lut), sc, ce, 16, 4091, 48000.0,
+                                   16000.0, p)
+    torch.cuda.synchronize()
+    n_out = orc.resample_out_length(int(bt.lengths[0]), 48000.0, 16000.0)
+    assert n_out == 960000
+    m_ref = orc.down_mix_f32(bt.x, bt.lengths, [2, 2], nthreads=8)
+    r_ref = orc.resample_f32(m_ref, bt.lengths, 48000.0, 16000.0, lut, sc, ce, 16, out_stride=res.shape[1],
+                             nthreads=8)
+    r = res.cpu().numpy()
+    assert np.array_equal(r[:, :n_out], r_ref[:, :n_out])
+    ref64, _, fr = orc.logmel_f64(r_ref[:, :n_out], np.full(2, n_out, np.int32), nthreads=8)
+    assert fr[0] == 6001
+    d = db.cpu().numpy()
+    err = max(float(np.abs(d[b, :, :F] - ref64[b, :, :F]).max()) for b, F in enumerate(fr))
+    print(f"workload4 60-s clips: max |dB err| vs FP64 = {err:.3e}")
+    assert err <= DB_TOL_MIXED
+
+
+def test_workload3_long_served_clips(gpu, orc):
+    """Served clips of 10-30 s (the long end of the served length range): each is the same as the
+    clip alone, and within 0.05 dB of FP64."""
+    from rpp_amd.serve import Server
+    s = synth.Stream(31)
+    L = np.round(s.uniform(6, 160000, 480000)).astype(np.int32)
+    xs = [synth.clip_for_request(i, int(n), 31) for i, n in enumerate(L)]
+    p = gpu.logmel_params()
+    with Server(p, "fp16x3", max_batch=4, deadline_us=2000, slots=2) as srv:
+        ids = [srv.submit(x) for x in xs]
+        outs = [srv.wait(r, 80, len(x) // 160 + 1)[0] for r, x in zip(ids, xs)]
+    h = gpu.Handle(1)
+    for x, o in zip(xs, outs):
+        F = len(x) // 160 + 1
+        a = torch.full((1, 80, F), float("nan"), device="cuda")
+        gpu.logmel_fused(h, dev(x[None]), a, dev(np.array([len(x)], np.int32)), p, "fp16x3")
+        torch.cuda.synchronize()
+        assert np.array_equal(o, a[0].cpu().numpy())
+    h.close()
+    pad = np.zeros((len(xs), int(L.max()) + 8), np.float32)
+    for b, x in enumerate(xs):
+        pad[b, :len(x)] = x
+    ref, _, fr = orc.logmel_f64(pad, L, nthreads=8)
+    err = max(float(np.abs(outs[b] - ref[b, :, :F]).max()) for b, F in enumerate(fr))
+    print(f"served 10-30 s clips: max |dB err| vs FP64 = {err:.3e}")
+    assert err <= DB_TOL_MIXED
+
+
+@pytest.mark.parametrize("per_clip", [1, 0])
+def test_chain_entry_point(gpu, handle, orc, per_clip):
+    """rppt_logmel_chain_gpu -- the four drop-in calls behind one entry, per clip (one launch per clip
+    per stage: the plain FP32 per-clip path) and batched -- against the FP32 oracle chain: the FP32
+    parity bar and 4.5e-4 dB (1e-4 relative in power).  Clips are at least two frames long (the
+    drop-in to_decibels reads a one-frame clip as 1-D data, as the reference does)."""
+    bt = synth.workload1(batch=24, max_seconds=8.0)
+    L = bt.lengths
+    frames = L // 160 + 1
+    out = torch.full((len(L), 80, int(frames.max())), float("nan"), device="cuda")
+    Lh = torch.from_numpy(L.copy()).pin_memory()
+    gpu.logmel_chain_gpu(handle, dev(bt.x), out, Lh, gpu.logmel_params(), per_clip=bool(per_clip))
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    ref32, fr = orc.logmel_f32(bt.x, L, nthreads=8)
+    assert np.array_equal(fr, frames)
+    bar = fp32_bar(o, ref32, fr)
+    err = max_db_err(o, ref32, fr)
+    print(f"chain perClip={per_clip}: fp32 bar {bar:.3f}, max |dB err| vs FP32 oracle {err:.3e}")
+    assert bar <= 1.0 and err <= 4.5e-4
+    # per clip and batched are the same launches on the same data: identical outputs
+    if per_clip:
+        out2 = torch.full_like(out, float("nan"))
+        gpu.logmel_chain_gpu(handle, dev(bt.x), out2, Lh, gpu.logmel_params(), per_clip=False)
+        torch.cuda.synchronize()
+        o2 = out2.cpu().numpy()
+        for b, F in enumerate(fr):
+            assert np.array_equal(o[b, :, :F], o2[b, :, :F]), b

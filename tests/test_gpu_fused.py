@@ -4,8 +4,9 @@
               oracle (the RPP HOST restatement) 1e-4 relative in mel power with a floor at 1e-6 of
               the frame's loudest band (helpers.fp32_bar)
   fp16x3      max |dB - FP64 oracle dB| <= 0.05 dB over every output (values clamp at -80 dB)
-  bf16x3      error reported against the FP64 oracle; meets 0.05 dB on workload 0 only (a two-term
-              bf16 split keeps ~16 mantissa bits), so its check on workloads 1/2 is the reported figure
+  bf16x3      meets 0.05 dB on workload 0 only (a two-term bf16 split keeps ~16 mantissa bits); on
+              workloads 1/2 it is held to its recorded error plus a margin (BF16X3_BOUND_DB), so a
+              regression fails
   bf16        one product, reported only
 """
 import numpy as np
@@ -18,6 +19,9 @@ from rpp_amd import synth
 pytestmark = pytest.mark.gpu
 
 DB_TOL_MIXED = 0.05
+# bf16x3 against the FP64 oracle: measured 7.9e-2 dB on workload 1[:24] and 1.4e-1 dB on workload
+# 2[:6]; the bound is that plus about a third
+BF16X3_BOUND_DB = {"workload1[:24]": 0.11, "workload2[:6]": 0.19}
 
 
 def dev(a):
@@ -74,6 +78,8 @@ def check_against_oracles(orc, out, x, L, precision, name, **kw):
         assert bar <= 1.0
     elif precision == "fp16x3":
         assert e64 <= DB_TOL_MIXED
+    elif precision == "bf16x3":
+        assert e64 <= BF16X3_BOUND_DB[name]
     return e64
 
 
@@ -323,3 +329,24 @@ def test_fused_tile_records_follow_the_batch(gpu):
     h2.close()
     for b, F in enumerate(frames):
         assert np.array_equal(o2[b, :, :F], r2[b, :, :F]), b
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
+def test_fused_power_rule(gpu, handle, precision):
+    """The fused launch reads `power` as rppt_spectrogram_gpu does (hip/kernel/spectrogram.cpp:231-232):
+    2 is the squared magnitude, any other value the magnitude, so power 0, 1 and 3 give the same
+    bits and differ from power 2."""
+    bt = synth.workload0(batch=3)
+    feat = bt.params
+    frames = bt.lengths // feat["H"] + 1
+    outs = {}
+    for pw in (0, 1, 2, 3):
+        p = gpu.logmel_params(power=pw)
+        out = torch.full((3, 80, int(frames.max())), float("nan"), device="cuda")
+        gpu.logmel_fused(handle, dev(bt.x), out, dev(bt.lengths), p, precision)
+        torch.cuda.synchronize()
+        outs[pw] = out.cpu().numpy()
+    for b, F in enumerate(frames):
+        assert np.array_equal(outs[0][b, :, :F], outs[1][b, :, :F])
+        assert np.array_equal(outs[3][b, :, :F], outs[1][b, :, :F])
+        assert not np.array_equal(outs[2][b, :, :F], outs[1][b, :, :F])

@@ -74,3 +74,34 @@ def test_deadline_closes_a_partial_batch(gpu):
         st = srv.stats()
     assert st["batches"] == 1                # three clips, fewer than maxBatch: one batch at the deadline
     assert min(lats) >= 2500, lats
+
+
+def test_wait_without_kept_outputs(gpu):
+    """keepOutputs = 0: several threads may wait on one request, a request that finished before its
+    wait reports frames -1, and the server never frees a request a waiter still holds."""
+    import threading
+    from rpp_amd.serve import Server
+    xs = clips(24, seed=11, hi=40000)
+    with Server(gpu.logmel_params(), "fp16x3", max_batch=8, deadline_us=2000, slots=2,
+                keep_outputs=False) as srv:
+        ids = [srv.submit(x) for x in xs]
+        got = {}
+
+        def waiter(tag):
+            got[tag] = [srv.wait_timing(r) for r in ids]
+
+        ts = [threading.Thread(target=waiter, args=(k,)) for k in range(3)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(60)
+        assert not any(t.is_alive() for t in ts)
+        srv.drain()
+        for k in range(3):
+            for (fr, lat), x in zip(got[k], xs):
+                # a waiter that arrived after the request was forgotten sees -1
+                assert fr in (-1, len(x) // 160 + 1)
+        late = srv.wait_timing(ids[0])
+        assert late == (-1, -1.0)
+        st = srv.stats()
+    assert st["completed"] == len(xs)
