@@ -135,9 +135,11 @@ extern "C" {
 
 /* ======================= handle API (api/rpp.h:132-244) ======================= */
 
-/* rppCreate: api/rpp.h:132, src/modules/handle_api.cpp:29-44.  Only RPP_HIP_BACKEND is
- * served here; stream may be NULL (null stream) or a hipStream_t.  numThreads is kept
- * for API parity and unused on the GPU. */
+/* rppCreate: api/rpp.h:132, src/modules/handle_api.cpp:29-44.  RPP_HIP_BACKEND handles serve the
+ * rppt_*_gpu entry points on the current device (stream NULL = null stream, else a hipStream_t);
+ * RPP_HOST_BACKEND handles serve the rppt_*_host entry points and touch no GPU.  numThreads: host
+ * threads of the _host entry points, at most the hardware threads, 0 = one per clip of the batch
+ * (src/modules/handle_host.cpp:54-62). */
 rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u numThreads, void *stream, RppBackend backend);
 /* rppDestroy: api/rpp.h:150, handle_api.cpp:46-60 */
 rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend);
@@ -196,6 +198,46 @@ RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, RpptDescPtr srcD
 RppStatus rppt_resample_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
                             Rpp32f *inRateTensor, Rpp32f *outRateTensor, Rpp32s *srcDimsTensor,
                             RpptResamplingWindow &window, rppHandle_t rppHandle);
+#endif
+
+/* ======================= HOST backend (buffers in host memory) ======================= */
+/* The rppt_*_host entry points of api/rppt_tensor_audio_augmentations.h on the CPU cores, with a
+ * RPP_HOST_BACKEND handle; every pointer is host memory.  Pre-emphasis, mel filter bank,
+ * to_decibels, down-mixing, resampling and region detection give RPP HOST's outputs bit for bit;
+ * the spectrogram uses this library's own FFT (RPP links FFTS), so it agrees to FP32 rounding. */
+
+/* api/rppt_tensor_audio_augmentations.h:139; cpu/kernel/pre_emphasis_filter.cpp:27-75 */
+RppStatus rppt_pre_emphasis_filter_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                        RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor, Rpp32f *coeffTensor,
+                                        RpptAudioBorderType borderType, rppHandle_t rppHandle);
+/* api/rppt_tensor_audio_augmentations.h:210; cpu/kernel/spectrogram.cpp:71-251 (power == 1: |X|,
+ * else |X|^2, the HOST rule) */
+RppStatus rppt_spectrogram_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                Rpp32s *srcLengthTensor, bool centerWindows, bool reflectPadding,
+                                Rpp32f *windowFunction, Rpp32s nfft, Rpp32s power, Rpp32s windowLength,
+                                Rpp32s windowStep, rppHandle_t rppHandle);
+/* api/rppt_tensor_audio_augmentations.h:253; cpu/kernel/mel_filter_bank.cpp:27-198 */
+RppStatus rppt_mel_filter_bank_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                    Rpp32s *srcDims, Rpp32f maxFreq, Rpp32f minFreq, RpptMelScaleFormula melFormula,
+                                    Rpp32s numFilter, Rpp32f sampleRate, bool normalize, rppHandle_t rppHandle);
+/* api/rppt_tensor_audio_augmentations.h:104; cpu/kernel/to_decibels.cpp:27-99 */
+RppStatus rppt_to_decibels_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                RpptImagePatchPtr srcDims, Rpp32f cutOffDB, Rpp32f multiplier,
+                                Rpp32f referenceMagnitude, rppHandle_t rppHandle);
+/* api/rppt_tensor_audio_augmentations.h:172; cpu/kernel/down_mixing.cpp:28-122 */
+RppStatus rppt_down_mixing_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                Rpp32s *srcDimsTensor, bool normalizeWeights, rppHandle_t rppHandle);
+/* api/rppt_tensor_audio_augmentations.h:65; cpu/kernel/non_silent_region_detection.cpp:93-191 */
+RppStatus rppt_non_silent_region_detection_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, Rpp32s *srcLengthTensor,
+                                                Rpp32s *detectedIndexTensor, Rpp32s *detectionLengthTensor,
+                                                Rpp32f cutOffDB, Rpp32s windowLength, Rpp32f referencePower,
+                                                Rpp32s resetInterval, rppHandle_t rppHandle);
+#ifdef __cplusplus
+/* api/rppt_tensor_audio_augmentations.h:292; cpu/kernel/resample.cpp:27-146 (reads window.lookup,
+ * window.lookupSize, scale, center, lobes) */
+RppStatus rppt_resample_host(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                             Rpp32f *inRateTensor, Rpp32f *outRateTensor, Rpp32s *srcDimsTensor,
+                             RpptResamplingWindow &window, rppHandle_t rppHandle);
 #endif
 
 /* ======================= extensions (no counterpart in the reference) ======================= */

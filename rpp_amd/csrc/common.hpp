@@ -78,6 +78,8 @@ struct MelTables {
 
 // The opaque handle of api/rpp.h.
 struct rppHandle {
+    RppBackend backend = RPP_HIP_BACKEND;
+    int numThreads = 1;               // host threads of the rppt_*_host entry points
     int device = 0;
     hipStream_t stream = nullptr;
     size_t batchSize = 1;

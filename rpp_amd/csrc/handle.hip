@@ -6,6 +6,9 @@
 // allocated until an entry point needs it.
 #include "common.hpp"
 
+#include <algorithm>
+#include <thread>
+
 namespace rppa {
 
 void *workspace(rppHandle *h, size_t bytes)
@@ -31,25 +34,34 @@ void *workspace(rppHandle *h, size_t bytes)
 
 }  // namespace rppa
 
+// RPP_HOST_BACKEND handles serve the rppt_*_host entry points and touch no GPU; their thread count
+// follows src/modules/handle_host.cpp:54-62 (at most the hardware threads; 0 means one per clip of
+// the batch).  RPP_HIP_BACKEND handles serve rppt_*_gpu on the current device.
 RPPA_EXPORT rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u numThreads, void *stream,
                                   RppBackend backend)
 {
-    (void)numThreads;
     if (!handle) return rppStatusBadParm;
-    if (backend != RPP_HIP_BACKEND) {
-        fprintf(stderr, "rpp_audio: only RPP_HIP_BACKEND handles are served by this library\n");
+    if (backend != RPP_HIP_BACKEND && backend != RPP_HOST_BACKEND) {
+        fprintf(stderr, "rpp_audio: only RPP_HOST_BACKEND and RPP_HIP_BACKEND handles are served by this library\n");
         return rppStatusNotImplemented;
     }
     rppHandle *h = new (std::nothrow) rppHandle();
     if (!h) return rppStatusAllocFailed;
-    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return rppStatusInternalError; }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
-        h->numCU = prop.multiProcessorCount;
-        h->maxLdsPerBlock = static_cast<int>(prop.sharedMemPerBlock);
-    }
-    h->stream = static_cast<hipStream_t>(stream);
+    h->backend = backend;
     h->batchSize = nBatchSize ? nBatchSize : 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    unsigned nt = hw ? std::min<unsigned>(numThreads, hw) : numThreads;
+    if (nt == 0) nt = static_cast<unsigned>(std::min<size_t>(h->batchSize, 1024));
+    h->numThreads = static_cast<int>(nt);
+    if (backend == RPP_HIP_BACKEND) {
+        if (hipGetDevice(&h->device) != hipSuccess) { delete h; return rppStatusInternalError; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
+            h->numCU = prop.multiProcessorCount;
+            h->maxLdsPerBlock = static_cast<int>(prop.sharedMemPerBlock);
+        }
+        h->stream = static_cast<hipStream_t>(stream);
+    }
     *handle = h;
     return rppStatusSuccess;
 }
@@ -57,7 +69,11 @@ RPPA_EXPORT rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u
 RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
 {
     if (!handle) return rppStatusBadParm;
-    if (backend != RPP_HIP_BACKEND) return rppStatusNotImplemented;
+    if (backend != handle->backend) return rppStatusBadParm;
+    if (backend == RPP_HOST_BACKEND) {
+        delete handle;
+        return rppStatusSuccess;
+    }
     (void)hipStreamSynchronize(handle->stream);
     if (handle->ws) (void)hipFree(handle->ws);
     if (handle->pinned) (void)hipHostFree(handle->pinned);

@@ -2,7 +2,9 @@
 // example of INTEGRATION.md, compiled against include/rpp_audio.h and linked against the library by
 // tests/test_boundary.py.  The static_asserts pin the descriptor layouts to the reference's
 // (api/rppdefs.h:570-589, :645-649, :751-790 on x86-64) and the enum values a binary caller passes.
-// main() runs the calls only when given a device argument; the CPU test only builds and links it.
+// main() resolves every entry point and runs the HOST-backend chain on the CPU (run_host_chain);
+// the GPU sequence (run_chain) is only linked, not called.
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -59,6 +61,36 @@ int run_chain(void *stream, int batch, float *d_src, RpptDesc &srcDesc, float *d
     return bad != 0;
 }
 
+// The HOST backend on host buffers: the same chain through rppt_*_host, run on the CPU.  One clip
+// of a 1 kHz tone at 16 kHz; returns 0 when every call succeeds and the dB peak (reference = the
+// clip's maximum) is 0 dB in the filter holding 1 kHz.
+int run_host_chain()
+{
+    const int L = 16000, F = L / 160 + 1, K = 257, M = 80;
+    static float x[L], pre[L], spec[K * F], mel[M * F], db[M * F];
+    for (int t = 0; t < L; t++) x[t] = 0.5f * std::sin(2.0f * 3.14159265f * 1000.0f * t / 16000.0f);
+    int len = L, melDims[2] = {K, F};
+    float coeff = 0.97f;
+    RpptImagePatch dbDims = {static_cast<Rpp32u>(F), static_cast<Rpp32u>(M)};
+    RpptDesc sd = {2, 0, F32, 1, 1, 1, static_cast<Rpp32u>(L), {static_cast<Rpp32u>(L), static_cast<Rpp32u>(L), 1, 1}, NHW};
+    RpptDesc specD = {3, 0, F32, 1, 1, K, static_cast<Rpp32u>(F), {static_cast<Rpp32u>(K * F), static_cast<Rpp32u>(K * F), static_cast<Rpp32u>(F), 1}, NFT};
+    RpptDesc melD = {3, 0, F32, 1, 1, M, static_cast<Rpp32u>(F), {static_cast<Rpp32u>(M * F), static_cast<Rpp32u>(M * F), static_cast<Rpp32u>(F), 1}, NFT};
+    rppHandle_t h;
+    if (rppCreate(&h, 1, 2, nullptr, RPP_HOST_BACKEND) != rppStatusSuccess) return 1;
+    int bad = 0;
+    bad |= rppt_pre_emphasis_filter_host(x, &sd, pre, &sd, &len, &coeff, CLAMP, h);
+    bad |= rppt_spectrogram_host(pre, &sd, spec, &specD, &len, true, true, nullptr, 512, 2, 400, 160, h);
+    bad |= rppt_mel_filter_bank_host(spec, &specD, mel, &melD, melDims, 8000.f, 0.f, SLANEY, M, 16000.f, true, h);
+    bad |= rppt_to_decibels_host(mel, &melD, db, &melD, &dbDims, -80.f, 10.f, 0.f, h);
+    rppDestroy(h, RPP_HOST_BACKEND);
+    if (bad) return 1;
+    int best = 0;
+    for (int m = 0; m < M; m++)
+        if (db[m * F + F / 2] > db[best * F + F / 2]) best = m;
+    std::printf("host chain: loudest filter %d at %.3f dB\n", best, db[best * F + F / 2]);
+    return db[best * F + F / 2] > -1.0f && best > 10 && best < 40 ? 0 : 1;
+}
+
 int main(int argc, char **argv)
 {
     (void)argv;
@@ -75,9 +107,14 @@ int main(int argc, char **argv)
         (const void *)&rppt_logmel_set_tiles, (const void *)&rppt_logmel_get_tiles,
         (const void *)&rppt_logmel_autotune, (const void *)&rppa_server_create, (const void *)&rppa_server_destroy,
         (const void *)&rppa_server_submit, (const void *)&rppa_server_wait, (const void *)&rppa_server_drain,
-        (const void *)&rppa_server_stats, (const void *)&run_chain};
+        (const void *)&rppa_server_stats, (const void *)&run_chain,
+        (const void *)&rppt_pre_emphasis_filter_host, (const void *)&rppt_spectrogram_host,
+        (const void *)&rppt_mel_filter_bank_host, (const void *)&rppt_to_decibels_host,
+        (const void *)&rppt_down_mixing_host, (const void *)&rppt_resample_host,
+        (const void *)&rppt_non_silent_region_detection_host, (const void *)&run_host_chain};
     int n = 0;
     for (const void *s : syms) n += s != nullptr;
     std::printf("%d entry points resolved\n", n);
+    if (run_host_chain() != 0) return 3;
     return argc > 1 ? 2 : 0;
 }

@@ -54,21 +54,28 @@ def test_struct_layouts():
     assert api.BORDER == {"zero": 0, "clamp": 1, "reflect": 2} and api.MEL == {"slaney": 0, "htk": 1}
 
 
-def test_create_rejects_host_backend():
-    """Only RPP_HIP_BACKEND handles are served; HOST returns rppStatusNotImplemented without
-    touching the device."""
+def test_create_host_and_hip_backends_only():
+    """RPP_HOST_BACKEND handles are made without touching a device (their thread count follows
+    handle_host.cpp:54-62) and destroyed with the same backend; RPP_OCL_BACKEND is refused with
+    rppStatusNotImplemented; destroying with the wrong backend is rppStatusBadParm."""
     import torch  # noqa: F401
     lib = C.CDLL(LIB)
-    h = C.c_void_p()
     lib.rppCreate.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint32, C.c_void_p, C.c_int]
-    assert lib.rppCreate(C.byref(h), 4, 0, None, 0) == -7
+    lib.rppDestroy.argtypes = [C.c_void_p, C.c_int]
+    h = C.c_void_p()
+    assert lib.rppCreate(C.byref(h), 4, 0, None, 0) == 0 and h.value
+    assert lib.rppDestroy(h, 1) == -1
+    assert lib.rppDestroy(h, 0) == 0
+    assert lib.rppCreate(C.byref(h), 4, 0, None, 2) == -7
 
 
 def test_cxx_caller_builds_and_links(tmp_path):
     """A C++ caller (tests/cxx/integration_example.cpp: INTEGRATION.md's call sequence) compiles
     against include/rpp_audio.h -- whose static_asserts pin sizeof/offsetof of RpptDesc,
     RpptImagePatch and RpptResamplingWindow and the enum values to the reference's -- and links
-    against librpp_audio.so, resolving every entry point.  The binary runs without touching a GPU."""
+        against librpp_audio.so, resolving every entry point.  The binary runs the log-mel chain
+    through the HOST entry points on the CPU (a 1 kHz tone peaks at 0 dB in its filter) and
+    touches no GPU."""
     src = os.path.join(ROOT, "tests", "cxx", "integration_example.cpp")
     exe = str(tmp_path / "integration_example")
     cmd = ["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
@@ -78,7 +85,7 @@ def test_cxx_caller_builds_and_links(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert "28 entry points resolved" in r.stdout
+    assert "36 entry points resolved" in r.stdout and "host chain" in r.stdout
 
 
 def test_cxx_header_is_plain_c_up_to_the_cpp_only_parts(tmp_path):
