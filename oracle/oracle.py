@@ -341,6 +341,54 @@ def logmel_f32(x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000
     return db, frames
 
 
+class LogMelF32Timer:
+    """The FP32 chain of logmel_f32 with every buffer allocated (and touched) up front, so that
+    run() makes only the four C calls -- pre-emphasis, spectrogram, mel, to_decibels -- as a
+    caller of RPP HOST's four rppt_*_host functions would time them (inputs resident, outputs
+    preallocated).  run() returns the dB array [B, nmel, maxF]."""
+
+    def __init__(self, x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000.0, fmin=0.0, fmax=0.0,
+                 coeff=0.97, cutoff_db=-80.0, multiplier=10.0, ref=1.0, formula="slaney", normalize=True,
+                 nthreads=1):
+        self.x = np.ascontiguousarray(x, np.float32)
+        B, S = self.x.shape
+        self.B, self.S = B, S
+        self.lens = np.ascontiguousarray(lengths, np.int32)
+        self.coeffs = np.full(B, coeff, np.float32)
+        self.frames = np.array([num_windows(int(n), W, H, True) for n in self.lens], np.int32)
+        self.maxF = int(self.frames.max()) if B else 0
+        self.K = (nfft or W) // 2 + 1
+        self.nfft, self.W, self.H, self.nmel = nfft, W, H, nmel
+        self.sample_rate, self.fmin, self.fmax = sample_rate, fmin, fmax
+        self.formula, self.normalize = MEL[formula], int(normalize)
+        self.cutoff_db, self.multiplier, self.ref = cutoff_db, multiplier, ref
+        self.nthreads = nthreads
+        # np.ones, not np.zeros: every page is written now, so no first-touch fault lands in run()
+        self.pe = np.ones((B, S), np.float32)
+        self.spec = np.ones((B, self.K, self.maxF), np.float32)
+        self.mel = np.ones((B, nmel, self.maxF), np.float32)
+        self.db = np.ones((B, nmel, self.maxF), np.float32)
+        self.mel_dims = np.zeros(2 * B, np.int32)
+        self.mel_dims[0::2] = self.K
+        self.mel_dims[1::2] = self.frames
+        self.db_dims = np.zeros(2 * B, np.uint32)
+        self.db_dims[0::2] = self.frames
+        self.db_dims[1::2] = nmel
+
+    def run(self) -> np.ndarray:
+        l = lib()
+        F = self.maxF
+        l.orc_pre_emphasis_f32(self.x, self.S, self.pe, self.S, self.lens, self.coeffs, self.B, BORDER["clamp"],
+                               self.nthreads)
+        l.orc_spectrogram_f32(self.pe, self.S, self.lens, self.B, self.spec, self.K * F, F, 1, 1, 1, None, self.nfft, 2,
+                              self.W, self.H, self.nthreads)
+        l.orc_mel_f32(self.spec, self.K * F, F, self.mel_dims, self.B, self.mel, self.nmel * F, F, self.fmax,
+                      self.fmin, self.formula, self.nmel, self.sample_rate, self.normalize, self.nthreads)
+        l.orc_to_db_f32(self.mel, self.nmel * F, F, self.db, self.nmel * F, F, self.db_dims, self.B, self.cutoff_db,
+                        self.multiplier, self.ref, self.nthreads, 1)
+        return self.db
+
+
 def logmel_f64(x, lengths, *, nfft=512, W=400, H=160, nmel=80, sample_rate=16000.0, fmin=0.0, fmax=0.0,
                coeff=0.97, cutoff_db=-80.0, multiplier=10.0, ref=1.0, formula="slaney", normalize=True,
                nthreads=1):

@@ -462,3 +462,18 @@ def test_logmel_f64_independent_composition():
         ref = 10 * np.log10(np.maximum(P @ Wm.T, 1e-8)).T                     # [M, F]
         assert frames[b] == ref.shape[1]
         assert np.allclose(db[b, :, :frames[b]], ref, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("workload,n", [(0, 8), (2, 3)])
+def test_logmel_f32_timer_equals_chain(workload, n):
+    """The CPU baseline's preallocated chain (LogMelF32Timer: the four C calls only) gives the
+    FP32 chain's output bit for bit, on repeated runs over the same buffers."""
+    from rpp_amd import synth
+    bt = synth.make(workload)
+    x, L = bt.x[:n], bt.lengths[:n]
+    ref, frames = O.logmel_f32(x, L, nthreads=4, **_features(bt))
+    t = O.LogMelF32Timer(x, L, nthreads=4, **_features(bt))
+    for _ in range(2):
+        db = t.run()
+        for b, F in enumerate(frames):
+            assert np.array_equal(db[b, :, :F], ref[b, :, :F]), b
