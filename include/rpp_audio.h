@@ -292,6 +292,20 @@ RppStatus rppt_logmel_fused_region_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, 
                                        const RpptLogMelParams *params, RpptLogMelPrecision precision,
                                        rppHandle_t rppHandle);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * output as rppt_logmel_fused_gpu, but the call never allocates: the DFT and mel tables, the tile
+ * plan, tile records and per-clip maxima live in one 16 MiB device array of the library (one copy
+ * per device, held until the process ends).  The tables of a parameter set are built there on the
+ * first call with it (that call must not be under stream capture); a call with other parameters
+ * rebuilds them in place.  RPP_ERROR_OUT_OF_BOUND_SCRATCH_MEMORY_SIZE when a batch's plan does not
+ * fit what the tables leave.  Every call on a device shares that array: calls that overlap on one
+ * device (other streams, other threads, other handles) race. */
+RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                        RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                        const RpptLogMelParams *params, RpptLogMelPrecision precision,
+                                        rppHandle_t rppHandle);
+
 /* Packed variant for serving: clip b's samples at srcPtr + srcOffsets[b] (srcLengthTensor[b] of
  * them), its dB output as a dense [numFilter][frames_b] block at dstPtr + dstOffsets[b], with
  * frames_b = L/H + 1 (centered) or (L-W)/H + 1.  Offsets and lengths in device or pinned memory. */

@@ -107,6 +107,7 @@ def lib():
         l.rppt_down_mix_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.c_bool, C.POINTER(RpptResamplingWindow), P]
         l.rppt_logmel_chain_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
+        l.rppt_logmel_fused_noalloc_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_region_gpu.argtypes = [P, DP, P, DP, P, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
         l.rppt_logmel_get_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
@@ -321,6 +322,20 @@ def logmel_fused(h: Handle, src, dst, lengths, params: RpptLogMelParams, precisi
                                      PRECISION[precision], h.h)
     if check_status:
         check("rppt_logmel_fused_gpu", st)
+    return st
+
+
+def logmel_fused_noalloc(h: Handle, src, dst, lengths, params: RpptLogMelParams, precision="fp16x3",
+                         check_status=True, layout=NFT, batch=None) -> int:
+    """rppt_logmel_fused_noalloc_gpu: rppt_logmel_fused_gpu with the library's own device array for
+    every buffer beyond src/dst (no allocation).  `batch` overrides the descriptor's clip count."""
+    B, S = src.shape
+    sd = desc_1d(B if batch is None else batch, S)
+    dd = desc_2d(B if batch is None else batch, dst.shape[1], dst.shape[2], layout=layout)
+    st = lib().rppt_logmel_fused_noalloc_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(lengths),
+                                             C.byref(params), PRECISION[precision], h.h)
+    if check_status:
+        check("rppt_logmel_fused_noalloc_gpu", st)
     return st
 
 

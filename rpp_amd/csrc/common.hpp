@@ -115,6 +115,19 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
 const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, float maxFreq, float minFreq,
                                 int formula, int M, int normalize);
 
+// The same tables built into caller-provided device memory, enqueued on `st` with no allocation and
+// no synchronisation.  DFT: `win` holds W floats of scratch (the window); t.basis ([W][cols] FP32)
+// is filled when wantBasis, t.foldHi/foldLo (dft_fold_elems halves each) and t.wHalf (W/2 floats)
+// when splitKind >= 0.  Mel: t.weightsDown/intervals [numBins], normFactors [M], filterBins [2M],
+// dense [M][numBins], denseHi/denseLo (mel_dense_elems halves each), `range` 2 ints; binStart and
+// binEnd stay unset (they are read back only by the cached builder).
+size_t dft_fold_elems(int nfft, int W);
+size_t mel_dense_elems(int numBins, int M);
+hipError_t build_dft_tables_at(hipStream_t st, int nfft, int W, const float *window, int splitKind, bool wantBasis,
+                               float *win, DftTables &t);
+hipError_t build_mel_tables_at(hipStream_t st, int numBins, float sampleRate, float maxFreq, float minFreq, int formula,
+                               int M, int normalize, int *range, MelTables &t);
+
 inline unsigned ceil_div(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 
 // RPP's frame count (cpu/kernel/spectrogram.cpp:47-52).

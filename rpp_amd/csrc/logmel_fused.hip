@@ -1389,19 +1389,32 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
     return RPP_SUCCESS;
 }
 
-// Validated parameters + buffers -> planner and fused kernel on h->stream.
+// Device memory of one launch supplied by the caller instead of the handle: the tables, and a
+// workspace of `wsBytes` bytes (256-byte aligned) for the tile plan, tile records and clip maxima.
+struct LaunchMem {
+    const DftTables *dt;
+    const MelTables *mt;
+    char *ws;
+    size_t wsBytes;
+};
+
+// Validated parameters + buffers -> planner and fused kernel on h->stream.  Tables and workspace
+// come from the handle (cached tables, grow-only workspace) unless `mem` supplies them; with `mem`
+// the launch allocates nothing and fails with RPP_ERROR_OUT_OF_BOUND_SCRATCH_MEMORY_SIZE when its
+// workspace does not fit.
 RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPrecision precision, const float *src,
                         uint32_t srcStride, const long long *srcOff, const int *lengths, int B, float *dst,
                         uint32_t dstNStride, uint32_t hStride, const long long *dstOff, int ntf, uint32_t frameCap,
-                        const int *begin = nullptr)
+                        const int *begin = nullptr, const LaunchMem *mem = nullptr)
 {
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     const int K = nfft / 2 + 1;
     const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
-    const DftTables *dt = get_dft_tables(h, nfft, p->windowLength, nullptr, splitKind);
-    const MelTables *mt = get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
-                                         p->numFilter, p->normalize ? 1 : 0);
+    const DftTables *dt = mem ? mem->dt : get_dft_tables(h, nfft, p->windowLength, nullptr, splitKind);
+    const MelTables *mt = mem ? mem->mt
+                              : get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq,
+                                               static_cast<int>(p->melFormula), p->numFilter, p->normalize ? 1 : 0);
     if (!dt || !mt) return RPP_ERROR;
     const bool clipRef = p->referenceMagnitude == 0.0f;
     // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
@@ -1410,6 +1423,10 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const int FT = mfma ? (CT <= 2 ? 64 : 32) : kFT32;
     // per-tile records for the MFMA kernel on the strided layout, when their count is bounded by
     // the row stride and the frame cap (every clip has at most ceil(maxFrames / FT) tiles)
+    // (records are only kept while they fit: 256 MB of the handle's workspace, or what a caller's
+    // workspace leaves after the tile plan; otherwise the kernel searches the plan per tile)
+    const size_t recOff = (sizeof(int) * (2 * B + 1) + 255) / 256 * 256;
+    const size_t recCap = mem ? (mem->wsBytes > recOff ? mem->wsBytes - recOff : 0) : (static_cast<size_t>(256) << 20);
     size_t recBytes = 0;
     if (mfma && !srcOff) {
         const long long maxF = std::min<long long>(
@@ -1417,25 +1434,26 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
                                  p->windowStep, p->centerWindows), 0),
             frameCap);
         const size_t n = static_cast<size_t>(B) * static_cast<size_t>((maxF + FT - 1) / FT);
-        if (n * sizeof(int4) <= (static_cast<size_t>(256) << 20)) recBytes = n * sizeof(int4);
+        if (n * sizeof(int4) <= recCap) recBytes = n * sizeof(int4);
     }
     // the MFMA kernel plans in each block (no planner launch, one dependent dispatch less) when the
     // per-tile records are bounded, the clip count fits the segment region and no per-clip maximum
     // needs initialising; each block's records take kPad slots, 256-byte aligned
     const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
-    const bool selfPlan = recBytes > 0 && !clipRef && B <= 4096;
+    bool selfPlan = recBytes > 0 && !clipRef && B <= 4096;
     int kPad = 0;
     size_t blockBytes = 0;
     if (selfPlan) {
         const size_t tiles = recBytes / sizeof(int4);
         kPad = static_cast<int>(((tiles + grid - 1) / grid + 15) / 16 * 16);
         blockBytes = static_cast<size_t>(grid) * kPad * sizeof(int4);
-        recBytes = 0;
+        if (blockBytes <= recCap) recBytes = 0;
+        else { selfPlan = false; kPad = 0; blockBytes = 0; }
     }
-    const size_t recOff = (sizeof(int) * (2 * B + 1) + 255) / 256 * 256;
-    char *ws = static_cast<char *>(workspace(h, recOff + recBytes + blockBytes));
-    if (!ws) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+    const size_t wsNeed = recOff + recBytes + blockBytes;
+    char *ws = mem ? (wsNeed <= mem->wsBytes ? mem->ws : nullptr) : static_cast<char *>(workspace(h, wsNeed));
+    if (!ws) return mem ? RPP_ERROR_OUT_OF_BOUND_SCRATCH_MEMORY_SIZE : RPP_ERROR_NOT_ENOUGH_MEMORY;
     int *tileStart = reinterpret_cast<int *>(ws);   // [B + 1], then the per-clip maxima [B]
     int4 *tileRec = recBytes ? reinterpret_cast<int4 *>(ws + recOff) : nullptr;
     int4 *blockRec = blockBytes ? reinterpret_cast<int4 *>(ws + recOff) : nullptr;
@@ -1674,4 +1692,142 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_packed_gpu(const Rpp32f *srcPtr, const l
     if (batchSize == 0) return RPP_SUCCESS;
     return logmel_launch(h, p, precision, srcPtr, 0, srcOffsets, srcLengthTensor, batchSize, dstPtr, 0, 0,
                          dstOffsets, 0, 0);
+}
+
+// ------------------------------------------------------------------------- no-allocation entry
+// rppt_logmel_fused_noalloc_gpu: the launch of rppt_logmel_fused_gpu with every byte of device
+// memory beyond the caller's inputs and outputs -- the DFT and mel tables, the tile plan, the tile
+// records and the per-clip maxima -- in one file-scope device array, so a call never allocates.
+// Each device holds its own copy of the array until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+namespace rppa {
+namespace {
+
+constexpr size_t kScratchBytes = static_cast<size_t>(16) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchDevice {
+    bool resolved = false;
+    char *ptr = nullptr;          // device address of `scratch` on this device
+    std::string tableKey;         // parameter set whose tables the array holds ("" none)
+    DftTables dt;
+    MelTables mt;
+    size_t tableBytes = 0;        // bytes at the start of the array taken by those tables
+};
+std::mutex gScratchMu;
+std::map<int, ScratchDevice> gScratch;
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// Carves the tables of one parameter set out of the array's head (256-byte aligned pieces);
+// returns the bytes used, or 0 when they do not fit.
+size_t layout_tables(char *base, int nfft, int W, int splitKind, bool wantBasis, int numBins, int M, float **win,
+                     int **range, DftTables &dt, MelTables &mt)
+{
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *q = base + o;
+        o += align256(bytes);
+        return q;
+    };
+    *win = reinterpret_cast<float *>(take(sizeof(float) * W));
+    if (wantBasis) dt.basis = reinterpret_cast<float *>(take(sizeof(float) * static_cast<size_t>(W) * (nfft + (nfft & 1))));
+    if (splitKind >= 0) {
+        const size_t n = dft_fold_elems(nfft, W);
+        dt.foldHi = reinterpret_cast<uint16_t *>(take(n * 2));
+        dt.foldLo = reinterpret_cast<uint16_t *>(take(n * 2));
+        dt.wHalf = reinterpret_cast<float *>(take(sizeof(float) * (W / 2)));
+    }
+    mt.weightsDown = reinterpret_cast<float *>(take(sizeof(float) * numBins));
+    mt.intervals = reinterpret_cast<int *>(take(sizeof(int) * numBins));
+    mt.normFactors = reinterpret_cast<float *>(take(sizeof(float) * M));
+    mt.filterBins = reinterpret_cast<int *>(take(sizeof(int) * 2 * M));
+    mt.dense = reinterpret_cast<float *>(take(sizeof(float) * static_cast<size_t>(M) * numBins));
+    const size_t ns = mel_dense_elems(numBins, M);
+    mt.denseHi = reinterpret_cast<uint16_t *>(take(ns * 2));
+    mt.denseLo = reinterpret_cast<uint16_t *>(take(ns * 2));
+    *range = reinterpret_cast<int *>(take(sizeof(int) * 2));
+    return o <= kScratchBytes ? o : 0;
+}
+
+}  // namespace
+}  // namespace rppa
+
+RPPA_EXPORT RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                                    RpptDescPtr dstDescPtr, Rpp32s *srcLengthTensor,
+                                                    const RpptLogMelParams *p, RpptLogMelPrecision precision,
+                                                    rppHandle_t h)
+{
+    using namespace rppa;
+    if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (h->backend != RPP_HIP_BACKEND) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (srcDescPtr->numDims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
+    if (dstDescPtr->layout != NFT && dstDescPtr->layout != NTF) return RPP_ERROR_INVALID_DST_LAYOUT;
+    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (!srcPtr || !dstPtr || !srcLengthTensor) return RPP_ERROR_INVALID_ARGUMENTS;
+    RppStatus st = check_params(p, precision);
+    if (st != RPP_SUCCESS) return st;
+    const unsigned B = srcDescPtr->n;
+    if (B == 0) return RPP_SUCCESS;
+    const int ntf = dstDescPtr->layout == NTF ? 1 : 0;
+    if (ntf ? dstDescPtr->strides.hStride < static_cast<unsigned>(p->numFilter)
+            : dstDescPtr->h < static_cast<unsigned>(p->numFilter))
+        return RPP_ERROR_INVALID_DST_DIMS;
+    const uint32_t frameCap = ntf ? dstDescPtr->h : dstDescPtr->strides.hStride;
+
+    int dev = 0;
+    RPPA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    ScratchDevice &sd = gScratch[dev];
+    if (!sd.resolved) {
+        // once per device, with that device current: the array's device address (the host-side
+        // name `scratch` is not one)
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "rpp_audio: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return RPP_ERROR;
+        }
+        sd.ptr = static_cast<char *>(ptr);
+        sd.resolved = true;
+    }
+
+    // the tables of this parameter set, rebuilt in place when the array holds another set's
+    const bool mfma = precision != RPPT_LOGMEL_FP32;
+    const int nfft = p->nfft ? p->nfft : p->windowLength;
+    const int K = nfft / 2 + 1;
+    const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
+    if (splitKind >= 0 && ((nfft & 1) || (p->windowLength & 1))) return RPP_ERROR_NOT_IMPLEMENTED;
+    char keyb[200];
+    snprintf(keyb, sizeof(keyb), "%d/%d/%d/%d/%a/%a/%a/%d/%d/%d", nfft, p->windowLength, splitKind, K, p->sampleRate,
+             p->maxFreq, p->minFreq, static_cast<int>(p->melFormula), p->numFilter, p->normalize ? 1 : 0);
+    if (sd.tableKey != keyb) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            fprintf(stderr, "rpp_audio: rppt_logmel_fused_noalloc_gpu must run once with these parameters before "
+                            "stream capture (its tables are built then)\n");
+            return RPP_ERROR;
+        }
+        DftTables dt;
+        MelTables mt;
+        float *win = nullptr;
+        int *range = nullptr;
+        const size_t used = layout_tables(sd.ptr, nfft, p->windowLength, splitKind, !mfma, K, p->numFilter, &win,
+                                          &range, dt, mt);
+        if (!used) return RPP_ERROR_OUT_OF_BOUND_SCRATCH_MEMORY_SIZE;
+        sd.tableKey.clear();
+        if (build_dft_tables_at(h->stream, nfft, p->windowLength, nullptr, splitKind, !mfma, win, dt) != hipSuccess ||
+            build_mel_tables_at(h->stream, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
+                                p->numFilter, p->normalize ? 1 : 0, range, mt) != hipSuccess)
+            return RPP_ERROR;
+        sd.dt = dt;
+        sd.mt = mt;
+        sd.tableBytes = used;
+        sd.tableKey = keyb;
+    }
+    LaunchMem mem{&sd.dt, &sd.mt, sd.ptr + sd.tableBytes, kScratchBytes - sd.tableBytes};
+    return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
+                         srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
+                         dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap, nullptr,
+                         &mem);
 }

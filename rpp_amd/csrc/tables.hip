@@ -228,6 +228,52 @@ uint64_t fnv1a(const void *p, size_t n)
 
 }  // namespace
 
+size_t dft_fold_elems(int nfft, int W) { return static_cast<size_t>((W / 2 + 15) / 16 * 16) * (nfft / 2); }
+
+size_t mel_dense_elems(int numBins, int M)
+{
+    return static_cast<size_t>((numBins + 31) / 32 * 32) * ((M + 15) / 16 * 16);
+}
+
+hipError_t build_dft_tables_at(hipStream_t st, int nfft, int W, const float *window, int splitKind, bool wantBasis,
+                               float *win, DftTables &t)
+{
+    t.nfft = nfft;
+    t.W = W;
+    t.cols = nfft + (nfft & 1);
+    if (window) {
+        const hipError_t e = hipMemcpyAsync(win, window, sizeof(float) * W, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    } else {
+        hipLaunchKernelGGL(hann_kernel, dim3(ceil_div(W, 256)), dim3(256), 0, st, W, win);
+    }
+    if (wantBasis)
+        hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(t.cols, 256), W), dim3(256), 0, st, nfft, W, t.cols, win,
+                           t.basis);
+    if (splitKind >= 0) {
+        t.Jp = (W / 2 + 15) / 16 * 16;
+        t.splitKind = splitKind;
+        const float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
+        hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft / 2, 256), t.Jp), dim3(256), 0, st, nfft, W, t.Jp,
+                           win, t.foldHi, t.foldLo, t.wHalf, splitKind, splitScale);
+    }
+    return hipGetLastError();
+}
+
+hipError_t build_mel_tables_at(hipStream_t st, int numBins, float sampleRate, float maxFreq, float minFreq, int formula,
+                               int M, int normalize, int *range, MelTables &t)
+{
+    t.numBins = numBins;
+    t.M = M;
+    t.KP = (numBins + 31) / 32 * 32;
+    t.Mpad = (M + 15) / 16 * 16;
+    hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(256), 0, st, numBins, sampleRate, maxFreq, minFreq, formula, M,
+                       normalize, t.weightsDown, t.intervals, t.normFactors, t.filterBins, t.dense, range);
+    hipLaunchKernelGGL(mel_split_kernel, dim3(ceil_div(t.Mpad, 64), t.KP), dim3(64), 0, st, t.dense, M, numBins,
+                       t.Mpad, t.KP, t.denseHi, t.denseLo);
+    return hipGetLastError();
+}
+
 const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *window, int splitKind)
 {
     uint64_t key = window ? fnv1a(window, sizeof(float) * W) : 0;
@@ -240,37 +286,19 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
 
     DftTables t;
     const bool fresh = it == h->dft.end();
-    if (!fresh) {
-        t = it->second;
-    } else {
-        t.nfft = nfft;
-        t.W = W;
-        t.cols = nfft + (nfft & 1);
-    }
+    if (!fresh) t = it->second;
     float *win = nullptr;
     if (hipMalloc(&win, sizeof(float) * W) != hipSuccess) return nullptr;
-    if (window) {
-        if (hipMemcpyAsync(win, window, sizeof(float) * W, hipMemcpyHostToDevice, h->stream) != hipSuccess) return nullptr;
-    } else {
-        hipLaunchKernelGGL(hann_kernel, dim3(ceil_div(W, 256)), dim3(256), 0, h->stream, W, win);
-    }
-    if (fresh) {
-        if (hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * t.cols) != hipSuccess) return nullptr;
-        hipLaunchKernelGGL(dft_basis_kernel, dim3(ceil_div(t.cols, 256), W), dim3(256), 0, h->stream, nfft, W, t.cols,
-                           win, t.basis);
-    }
+    if (fresh && hipMalloc(&t.basis, sizeof(float) * static_cast<size_t>(W) * (nfft + (nfft & 1))) != hipSuccess)
+        return nullptr;
     if (splitKind >= 0) {
         if (t.foldHi) { (void)hipFree(t.foldHi); (void)hipFree(t.foldLo); (void)hipFree(t.wHalf); }
-        t.Jp = (W / 2 + 15) / 16 * 16;
-        const size_t n = static_cast<size_t>(t.Jp) * (nfft / 2);
+        const size_t n = dft_fold_elems(nfft, W);
         if (hipMalloc(&t.foldHi, n * 2) != hipSuccess || hipMalloc(&t.foldLo, n * 2) != hipSuccess ||
             hipMalloc(&t.wHalf, sizeof(float) * (W / 2)) != hipSuccess)
             return nullptr;
-        t.splitKind = splitKind;
-        const float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
-        hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft / 2, 256), t.Jp), dim3(256), 0, h->stream, nfft, W,
-                           t.Jp, win, t.foldHi, t.foldLo, t.wHalf, splitKind, splitScale);
     }
+    if (build_dft_tables_at(h->stream, nfft, W, window, splitKind, fresh, win, t) != hipSuccess) return nullptr;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
     (void)hipFree(win);
     h->dft[k] = t;
@@ -289,24 +317,18 @@ const MelTables *get_mel_tables(rppHandle *h, int numBins, float sampleRate, flo
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
 
     MelTables t;
-    t.numBins = numBins;
-    t.M = M;
     int *range = nullptr;
+    const size_t ns = mel_dense_elems(numBins, M);
     if (hipMalloc(&t.weightsDown, sizeof(float) * numBins) != hipSuccess ||
         hipMalloc(&t.intervals, sizeof(int) * numBins) != hipSuccess ||
         hipMalloc(&t.normFactors, sizeof(float) * M) != hipSuccess ||
         hipMalloc(&t.filterBins, sizeof(int) * 2 * M) != hipSuccess ||
         hipMalloc(&t.dense, sizeof(float) * static_cast<size_t>(M) * numBins) != hipSuccess ||
+        hipMalloc(&t.denseHi, ns * 2) != hipSuccess || hipMalloc(&t.denseLo, ns * 2) != hipSuccess ||
         hipMalloc(&range, sizeof(int) * 2) != hipSuccess)
         return nullptr;
-    hipLaunchKernelGGL(mel_tables_kernel, dim3(1), dim3(256), 0, h->stream, numBins, sampleRate, maxFreq, minFreq,
-                       formula, M, normalize, t.weightsDown, t.intervals, t.normFactors, t.filterBins, t.dense, range);
-    t.KP = (numBins + 31) / 32 * 32;
-    t.Mpad = (M + 15) / 16 * 16;
-    size_t ns = static_cast<size_t>(t.KP) * t.Mpad;
-    if (hipMalloc(&t.denseHi, ns * 2) != hipSuccess || hipMalloc(&t.denseLo, ns * 2) != hipSuccess) return nullptr;
-    hipLaunchKernelGGL(mel_split_kernel, dim3(ceil_div(t.Mpad, 64), t.KP), dim3(64), 0, h->stream, t.dense, M,
-                       numBins, t.Mpad, t.KP, t.denseHi, t.denseLo);
+    if (build_mel_tables_at(h->stream, numBins, sampleRate, maxFreq, minFreq, formula, M, normalize, range, t) != hipSuccess)
+        return nullptr;
     int hr[2] = {0, 0};
     if (hipMemcpyAsync(hr, range, sizeof(hr), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return nullptr;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
