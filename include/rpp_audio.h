@@ -323,9 +323,10 @@ RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
 
 /* Launch shape of the fused kernel; 0 fields mean "default". */
 typedef struct {
-    Rpp32s framesPerTile;   /* reported only: frames per work item of the kernel the launch uses (FP32 32;
-                               MFMA 64 for nfft <= 512, 32 above) */
-    Rpp32s blocksPerCU;     /* persistent blocks per compute unit (default 1) */
+    Rpp32s framesPerTile;   /* frames per work item: 16 selects the factored-DFT kernel (fp16x3, nfft 512,
+                               hop 160; one wave per item), 64 (nfft <= 512) or 32 (above) the dense MFMA
+                               kernel, 32 the FP32 kernel; 0 = default (factored where it applies) */
+    Rpp32s blocksPerCU;     /* persistent workgroups per compute unit (dense kernels; default 2) */
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);
 /* Autotuner: times every supported tile shape on this batch (`iters` launches each, events on the

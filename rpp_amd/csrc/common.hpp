@@ -74,6 +74,25 @@ struct MelTables {
     int Mpad = 0;                   // M padded to a multiple of 16
 };
 
+// Tables of the factored-DFT fused kernel (logmel_fft.hip) for one (nfft, W), built on the GPU in
+// FP64: stage-1 and stage-2 MFMA operand fragments (fp16 hi/lo, scaled by 2^12), the window in
+// polyphase order and the twiddles.  One allocation (`mem`).
+struct FftTables {
+    int nfft = 0, W = 0;
+    uint16_t *a1 = nullptr, *a2 = nullptr;
+    float *winP = nullptr, *tw = nullptr;
+    void *mem = nullptr;
+};
+
+// Mel weight fragments of the (16-filter tile, 32-bin chunk) pairs a filter bank touches, for the
+// factored kernel: bf16 hi/lo in MFMA operand order, pairOff[tile * 16 + chunk] = fragment index or
+// -1, pairOff[256] = nPairs.
+struct MelFrag {
+    uint16_t *frag = nullptr;
+    int *pairOff = nullptr;
+    int nPairs = 0;
+};
+
 }  // namespace rppa
 
 // The opaque handle of api/rpp.h.
@@ -85,6 +104,7 @@ struct rppHandle {
     size_t batchSize = 1;
     int numCU = 256;
     int maxLdsPerBlock = 64 * 1024;   // bytes of LDS one workgroup may allocate
+    int maxLdsOptin = 64 * 1024;      // the same with the launch's explicit opt-in (dynamic LDS attribute)
     int numXCD = 8;
 
     // grow-only device workspace
@@ -97,6 +117,8 @@ struct rppHandle {
 
     std::map<std::tuple<int, int, uint64_t>, rppa::DftTables> dft;
     std::map<std::string, rppa::MelTables> mel;
+    std::map<std::pair<int, int>, rppa::FftTables> fft;
+    std::map<std::string, rppa::MelFrag> melFrag;
 
     RpptLogMelTiles tiles{0, 0};
 };
@@ -127,6 +149,39 @@ hipError_t build_dft_tables_at(hipStream_t st, int nfft, int W, const float *win
                                float *win, DftTables &t);
 hipError_t build_mel_tables_at(hipStream_t st, int numBins, float sampleRate, float maxFreq, float minFreq, int formula,
                                int M, int normalize, int *range, MelTables &t);
+
+// Factored-DFT fused kernel (logmel_fft.hip).
+struct FftLaunch {
+    const RpptLogMelParams *p;
+    const float *src;
+    uint32_t srcStride;
+    const long long *srcOff;
+    const int *lengths;
+    const int *begin;
+    int B;
+    float *dst;
+    uint32_t dstNStride, hStride;
+    const long long *dstOff;
+    int ntf;
+    uint32_t frameCap;
+    int *clipMax;
+    float minRatio, mult, invRef;
+    const FftTables *ft;
+    const MelFrag *mf;
+};
+// the launch shape it supports (parameters only; the LDS check needs the mel pair count)
+bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int B);
+size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut);
+const FftTables *get_fft_tables(rppHandle *h, int nfft, int W);
+const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string &key);
+// the same tables in caller-provided memory (no allocation): fft_table_bytes() bytes at `mem`
+// (256-byte aligned) for W samples of window; mel_frag_bytes(M) + 257 ints for the fragments
+size_t fft_table_bytes(int W);
+void fft_tables_carve(char *mem, int W, FftTables &t);
+hipError_t build_fft_tables_at(hipStream_t st, int W, FftTables &t);
+size_t mel_frag_bytes(int M);
+hipError_t build_mel_frag_at(hipStream_t st, const MelTables &mt, MelFrag &f);
+RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L);
 
 inline unsigned ceil_div(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 

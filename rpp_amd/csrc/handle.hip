@@ -59,6 +59,7 @@ RPPA_EXPORT rppStatus_t rppCreate(rppHandle_t *handle, size_t nBatchSize, Rpp32u
         if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
             h->numCU = prop.multiProcessorCount;
             h->maxLdsPerBlock = static_cast<int>(prop.sharedMemPerBlock);
+            h->maxLdsOptin = static_cast<int>(std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin));
         }
         h->stream = static_cast<hipStream_t>(stream);
     }
@@ -82,6 +83,11 @@ RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
         if (kv.second.foldHi) (void)hipFree(kv.second.foldHi);
         if (kv.second.foldLo) (void)hipFree(kv.second.foldLo);
         if (kv.second.wHalf) (void)hipFree(kv.second.wHalf);
+    }
+    for (auto &kv : handle->fft) (void)hipFree(kv.second.mem);
+    for (auto &kv : handle->melFrag) {
+        (void)hipFree(kv.second.frag);
+        (void)hipFree(kv.second.pairOff);
     }
     for (auto &kv : handle->mel) {
         (void)hipFree(kv.second.weightsDown);

@@ -1396,7 +1396,17 @@ struct LaunchMem {
     const MelTables *mt;
     char *ws;
     size_t wsBytes;
+    const FftTables *ft;   // tables of the factored kernel (null: that kernel is not used)
+    const MelFrag *mf;
 };
+
+std::string mel_key(const RpptLogMelParams *p, int K)
+{
+    char keyb[160];
+    snprintf(keyb, sizeof(keyb), "%d/%a/%a/%a/%d/%d/%d", K, p->sampleRate, p->maxFreq, p->minFreq,
+             static_cast<int>(p->melFormula), p->numFilter, p->normalize ? 1 : 0);
+    return std::string(keyb);
+}
 
 // Validated parameters + buffers -> planner and fused kernel on h->stream.  Tables and workspace
 // come from the handle (cached tables, grow-only workspace) unless `mem` supplies them; with `mem`
@@ -1417,6 +1427,20 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
                                                static_cast<int>(p->melFormula), p->numFilter, p->normalize ? 1 : 0);
     if (!dt || !mt) return RPP_ERROR;
     const bool clipRef = p->referenceMagnitude == 0.0f;
+    // the factored-DFT kernel (logmel_fft.hip) where its launch shape and LDS budget allow
+    const FftTables *ft = nullptr;
+    const MelFrag *mf = nullptr;
+    if (fft_eligible(p, precision, B) && !(h->tiles.blocksPerCU > 0 && h->tiles.framesPerTile > 0 &&
+                                            h->tiles.framesPerTile != 16)) {
+        ft = mem ? mem->ft : get_fft_tables(h, nfft, p->windowLength);
+        mf = mem ? mem->mf : get_mel_frag(h, mt, mel_key(p, K));
+        if (!ft || !mf || mf->nPairs <= 0 ||
+            fft_lds_bytes(p->windowStep, B, mf->nPairs, begin != nullptr, srcOff != nullptr, nullptr) >
+                static_cast<size_t>(h->maxLdsOptin)) {
+            ft = nullptr;
+            mf = nullptr;
+        }
+    }
     // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
     // tiles each, 64 frames per tile up to nfft 512 and 32 above
     const int CT = nfft / 256;
@@ -1509,11 +1533,34 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.KP = mt->KP;
     a.Mpad = mt->Mpad;
 
-    if (!selfPlan)
+    if (!selfPlan && (!ft || clipRef))   // the factored kernel plans in LDS; the planner then only sets the clip maxima
         hipLaunchKernelGGL(plan_tiles_kernel, dim3(1), dim3(1024), 0, h->stream, lengths, begin, a.B, a.srcStride,
                        a.frameCap, a.W, a.H, a.center, FT, srcOff ? 1 : 0, tileStart, a.clipMax, tileRec);
     const int pws = a.KP + kSkew;
-    if (!mfma) {
+    if (ft) {
+        FftLaunch L{};
+        L.p = p;
+        L.src = src;
+        L.srcStride = srcStride;
+        L.srcOff = srcOff;
+        L.lengths = lengths;
+        L.begin = begin;
+        L.B = B;
+        L.dst = dst;
+        L.dstNStride = dstNStride;
+        L.hStride = hStride;
+        L.dstOff = dstOff;
+        L.ntf = ntf;
+        L.frameCap = frameCap;
+        L.clipMax = a.clipMax;
+        L.minRatio = a.minRatio;
+        L.mult = a.mult;
+        L.invRef = a.invRef;
+        L.ft = ft;
+        L.mf = mf;
+        const RppStatus st = logmel_fft_launch(h, L);
+        if (st != RPP_SUCCESS) return st;
+    } else if (!mfma) {
         const int span = (kFT32 - 1) * a.H + a.W;
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 3) / 4 * 4;
         const int bFloats = kKC * kCC;
@@ -1650,6 +1697,9 @@ RPPA_EXPORT RppStatus rppt_logmel_autotune(RppPtr_t srcPtr, RpptDescPtr srcDescP
     if (!h || !p || iters <= 0) return RPP_ERROR_INVALID_ARGUMENTS;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     std::vector<RpptLogMelTiles> cands;
+    // the factored-DFT kernel (16 frames per wave item, one 8-wave workgroup per CU) where it
+    // applies, and the dense kernel at 1-4 workgroups per CU
+    if (srcDescPtr && fft_eligible(p, precision, static_cast<int>(srcDescPtr->n))) cands.push_back({16, 1});
     const int ft = precision == RPPT_LOGMEL_FP32 ? kFT32 : (nfft >= 768 ? 32 : 64);
     for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
     hipEvent_t e0, e1;
@@ -1712,6 +1762,9 @@ struct ScratchDevice {
     std::string tableKey;         // parameter set whose tables the array holds ("" none)
     DftTables dt;
     MelTables mt;
+    FftTables ft;
+    MelFrag mf;
+    bool hasFft = false;          // the factored kernel's tables are there too
     size_t tableBytes = 0;        // bytes at the start of the array taken by those tables
 };
 std::mutex gScratchMu;
@@ -1747,6 +1800,18 @@ size_t layout_tables(char *base, int nfft, int W, int splitKind, bool wantBasis,
     mt.denseHi = reinterpret_cast<uint16_t *>(take(ns * 2));
     mt.denseLo = reinterpret_cast<uint16_t *>(take(ns * 2));
     *range = reinterpret_cast<int *>(take(sizeof(int) * 2));
+    return o <= kScratchBytes ? o : 0;
+}
+
+// The factored kernel's tables after the others (from byte `o`); returns the new end, 0 if past the array.
+size_t layout_fft_tables(char *base, size_t o, int W, int M, FftTables &ft, MelFrag &mf)
+{
+    fft_tables_carve(base + o, W, ft);
+    o += align256(fft_table_bytes(W));
+    mf.frag = reinterpret_cast<uint16_t *>(base + o);
+    o += align256(mel_frag_bytes(M));
+    mf.pairOff = reinterpret_cast<int *>(base + o);
+    o += align256(sizeof(int) * 257);
     return o <= kScratchBytes ? o : 0;
 }
 
@@ -1820,12 +1885,36 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr
             build_mel_tables_at(h->stream, K, p->sampleRate, p->maxFreq, p->minFreq, static_cast<int>(p->melFormula),
                                 p->numFilter, p->normalize ? 1 : 0, range, mt) != hipSuccess)
             return RPP_ERROR;
+        // the factored kernel's tables too, when its launch shape fits these parameters (its mel pair
+        // count is read back once, here, outside any capture)
+        FftTables ft;
+        MelFrag mf;
+        bool hasFft = false;
+        size_t end = used;
+        if (fft_eligible(p, precision, 1)) {
+            const size_t e2 = layout_fft_tables(sd.ptr, used, p->windowLength, p->numFilter, ft, mf);
+            if (e2) {
+                int n[2] = {0, 0};   // [255] tiles of a chunk not contiguous, [256] pair count
+                if (build_fft_tables_at(h->stream, p->windowLength, ft) != hipSuccess ||
+                    build_mel_frag_at(h->stream, mt, mf) != hipSuccess ||
+                    hipMemcpyAsync(n, mf.pairOff + 255, sizeof(n), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                    hipStreamSynchronize(h->stream) != hipSuccess)
+                    return RPP_ERROR;
+                mf.nPairs = n[0] ? 0 : n[1];
+                hasFft = true;
+                end = e2;
+            }
+        }
         sd.dt = dt;
         sd.mt = mt;
-        sd.tableBytes = used;
+        sd.ft = ft;
+        sd.mf = mf;
+        sd.hasFft = hasFft;
+        sd.tableBytes = end;
         sd.tableKey = keyb;
     }
-    LaunchMem mem{&sd.dt, &sd.mt, sd.ptr + sd.tableBytes, kScratchBytes - sd.tableBytes};
+    LaunchMem mem{&sd.dt, &sd.mt, sd.ptr + sd.tableBytes, kScratchBytes - sd.tableBytes,
+                  sd.hasFft ? &sd.ft : nullptr, sd.hasFft ? &sd.mf : nullptr};
     return logmel_launch(h, p, precision, static_cast<const float *>(srcPtr), srcDescPtr->strides.nStride, nullptr,
                          srcLengthTensor, static_cast<int>(B), static_cast<float *>(dstPtr),
                          dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap, nullptr,

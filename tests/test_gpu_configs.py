@@ -79,8 +79,9 @@ def full_batch_check(gpu, orc, bt, name):
     h.close()
     full = eager.cpu().numpy()
     del g, out, eager
-    # the same clips in 64-clip batches, each its own launch on a fresh handle
+    # the same clips in 64-clip batches, each its own launch on a fresh handle with the tuned shape
     h = gpu.Handle(64)
+    gpu.set_tiles(h, ft, bpc)
     for b0 in range(0, B, 64):
         sl = slice(b0, min(B, b0 + 64))
         Fb = int(frames[sl].max())
