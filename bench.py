@@ -30,9 +30,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# revision of the fused MFMA kernel; recorded traffic (profiles/*.md `traffic:` lines) applies only to
-# the revision it was measured on
-KERNEL_REVISION = "mirror-il2-selfplan"
+# revisions of the fused kernels (the factored-DFT kernel runs at 16 frames per tile, the dense one at
+# 32/64); recorded traffic (profiles/*.md `traffic:` lines) applies only to the revision it was measured on
+KERNEL_REVISION = {16: "fft32x16-wave"}
+DENSE_REVISION = "mirror-il2-selfplan"
 MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 MI355X_HBM_GBS = 8000.0
 
@@ -46,7 +47,7 @@ def recorded_traffic(workload, precision, tiles):
     FETCH_SIZE/WRITE_SIZE runs summarised under profiles/ (a `traffic:` line per configuration,
     counters already scaled by the copy-kernel calibration), or None when none matches."""
     import glob
-    key = {"kernel": KERNEL_REVISION, "workload": str(workload), "precision": precision,
+    key = {"kernel": KERNEL_REVISION.get(tiles[0], DENSE_REVISION), "workload": str(workload), "precision": precision,
            "frames_per_tile": str(tiles[0]), "blocks_per_cu": str(tiles[1])}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
@@ -339,22 +340,47 @@ def main() -> int:
     value = audio_s * a.steps / wall
     head = out.clone()
     log(f"fused {a.precision}: {ms_per_step:.3f} ms/step, {value:,.0f} audio-s/s (events {gpu_s / a.steps * 1e3:.3f} ms)")
+    # the kernel's own launch duration: one graph of CHAIN back-to-back launches of the same step,
+    # so the few microseconds between graph replays fall out of the per-launch figure (the launches
+    # between two kernel nodes of one graph still count, so this stays an upper bound)
+    CHAIN = 10
+    with torch.cuda.stream(stream):
+        gc = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gc, stream=stream):
+            for _ in range(CHAIN):
+                api.logmel_fused(h, x, out, L, params, a.precision)
+    reps = max(1, a.steps // CHAIN)
+    _, chain_s = time_graph(gc, reps, 2)
+    kernel_s = min(chain_s / (reps * CHAIN), gpu_s / a.steps)
+    del gc
+    log(f"fused {a.precision}: {kernel_s * 1e6:.1f} us per launch in a graph of {CHAIN} launches")
 
     # roofline of the dominant kernel (the fused kernel; the planner is ~us): algorithmic FLOPs
     # (dense DFT 2*W*2K + dense mel 2*K*M per frame; split-product passes are overhead, not work)
     K = feat["nfft"] // 2 + 1
     flop_per_frame = 2 * feat["W"] * 2 * K + 2 * K * feat["nmel"]
     flops = float(frames.sum()) * flop_per_frame
-    kernel_s = gpu_s / a.steps
     achieved_tf = flops / kernel_s / 1e12
     rec = recorded_traffic(a.workload, a.precision, tuned[a.precision][0])
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MI355X_PEAK_TFLOPS[a.precision],
                 "unit": "TFLOP/s", "frac": round(achieved_tf / MI355X_PEAK_TFLOPS[a.precision], 4),
                 "traffic": rec[0] if rec else None, "flop_per_launch": flops, "kernel_ms": round(kernel_s * 1e3, 4),
+                "step_ms_events": round(gpu_s / a.steps * 1e3, 4),
                 "hbm_bytes_algorithmic": int(4 * bt.lengths.sum() + 4 * feat["nmel"] * frames.sum())}
     if rec:
         roofline["traffic_source"] = "profiles/" + rec[1]
         roofline["hbm_gbs_at_traffic"] = round(rec[0] / kernel_s / 1e9, 1)
+    ft_chosen = tuned[a.precision][0][0]
+    roofline["kernel"] = KERNEL_REVISION.get(ft_chosen, DENSE_REVISION)
+    if ft_chosen == 16 and a.precision == "fp16x3":
+        # the factored DFT executes fewer matrix FLOPs than the dense count above: per 16-frame item
+        # 96 + 96 fp16 products (stage 1, stage 2) and one bf16 triple for each of the 12 (16-mel tile,
+        # 32-bin chunk) pairs the 80-mel Slaney bank at nfft 512 occupies, each v_mfma_*_16x16x32 =
+        # 16,384 FLOP; split passes included
+        items = int(((frames + 15) // 16).sum())
+        mfma = items * (192 + 3 * 12) * 16384.0
+        roofline["mfma_flop_executed_per_launch"] = mfma
+        roofline["mfma_pipe_frac"] = round(mfma / kernel_s / 1e12 / MI355X_PEAK_TFLOPS[a.precision], 4)
 
     side = {}
     cpu = None
@@ -454,7 +480,8 @@ def main() -> int:
                                 2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],
                    "batch": B, "audio_seconds": round(audio_s, 2), "frames": int(frames.sum()),
                    "nfft": feat["nfft"], "window": feat["W"], "hop": H, "mels": feat["nmel"],
-                   "precision": a.precision, "launch": "persistent fused kernel (self-planned tiles), hipGraph replay",
+                   "precision": a.precision, "launch": ("persistent fused kernel, factored DFT (32 x 16), hipGraph replay" if tuned[a.precision][0][0] == 16
+                              else "persistent fused kernel, folded dense DFT, hipGraph replay"),
                    "tiles": {"frames_per_tile": tuned[a.precision][0][0], "blocks_per_cu": tuned[a.precision][0][1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
