@@ -10,8 +10,9 @@ Two libraries sit behind it:
   (``rpp_oracle.c``): FP32 functions that follow
   ``src/modules/tensor/cpu/kernel/{pre_emphasis_filter,spectrogram,mel_filter_bank,
   to_decibels,down_mixing,resample}.cpp`` and an FP64 oracle of the same maths.
-* ``_ref/libref_audio.so`` (optional, build container only) -- RPP's own HOST
-  kernels compiled from the reference sources; used to check the restatement.
+* ``_ref/libref_audio.so`` (optional; buildable only where the reference sources are,
+  and present wherever the built tree is copied) -- RPP's own HOST kernels compiled from
+  the reference sources; used only to check the restatement.
 """
 from __future__ import annotations
 
