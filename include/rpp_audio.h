@@ -324,8 +324,9 @@ RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
 /* Launch shape of the fused kernel; 0 fields mean "default". */
 typedef struct {
     Rpp32s framesPerTile;   /* frames per work item: 16 selects the factored-DFT kernel (fp16x3, nfft 512,
-                               hop 160; one wave per item), 64 (nfft <= 512) or 32 (above) the dense MFMA
-                               kernel, 32 the FP32 kernel; 0 = default (factored where it applies) */
+                               hop 160; one wave per item); the dense MFMA kernel runs 64 frames (nfft
+                               <= 512; at nfft 512, 32 on request) or 32 (above); the FP32 kernel 32;
+                               0 = default (factored where it applies) */
     Rpp32s blocksPerCU;     /* persistent workgroups per compute unit (dense kernels; default 2) */
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);

@@ -1443,8 +1443,9 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     }
     // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
     // tiles each, 64 frames per tile up to nfft 512 and 32 above
+    // (nfft 512 also runs 32-frame tiles, one unit per wave, when the handle's tiles ask for them)
     const int CT = nfft / 256;
-    const int FT = mfma ? (CT <= 2 ? 64 : 32) : kFT32;
+    const int FT = mfma ? (CT == 2 && h->tiles.framesPerTile == 32 ? 32 : (CT <= 2 ? 64 : 32)) : kFT32;
     // per-tile records for the MFMA kernel on the strided layout, when their count is bounded by
     // the row stride and the frame cap (every clip has at most ceil(maxFrames / FT) tiles)
     // (records are only kept while they fit: 256 MB of the handle's workspace, or what a caller's
@@ -1588,10 +1589,12 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
             hipLaunchKernelGGL((logmel_mfma_kernel<PR, F, C, R>), dim3(grid), dim3(kBlockMfma), bytes, h->stream, a, segFloats, wOff); \
         } while (0)
         // units (32-bin column tile of direct bins, 32-frame row tile) per wave: nfft 256: 2 x 2
-        // over 4 waves; 512: 4 x 2; 768: 6 x 1 (two waves repeat a tile); 1024: 8 x 1
+        // over 4 waves; 512: 4 x 2 (or 4 x 1 at 32 frames); 768: 6 x 1 (two waves repeat a tile);
+        // 1024: 8 x 1
 #define RPPA_LAUNCH(PR)                                                                                         \
         do {                                                                                                    \
             if (CT == 1) RPPA_LAUNCH_ONE(PR, 64, 1, 1);                                                         \
+            else if (CT == 2 && FT == 32) RPPA_LAUNCH_ONE(PR, 32, 1, 1);                                        \
             else if (CT == 2) RPPA_LAUNCH_ONE(PR, 64, 1, 2);                                                    \
             else RPPA_LAUNCH_ONE(PR, 32, 2, 1);                                                                 \
         } while (0)
@@ -1697,11 +1700,18 @@ RPPA_EXPORT RppStatus rppt_logmel_autotune(RppPtr_t srcPtr, RpptDescPtr srcDescP
     if (!h || !p || iters <= 0) return RPP_ERROR_INVALID_ARGUMENTS;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     std::vector<RpptLogMelTiles> cands;
-    // the factored-DFT kernel (16 frames per wave item, one 8-wave workgroup per CU) where it
-    // applies, and the dense kernel at 1-4 workgroups per CU
+    // three axes: the kernel (the factored DFT, 16 frames per wave item and one 8-wave workgroup
+    // per CU, where it applies; else the dense kernel), the dense kernel's frames per tile (64 or
+    // 32 at nfft 512, which also sets how many (column tile, row tile) units each wave holds) and
+    // its persistent workgroups per CU (1-4)
     if (srcDescPtr && fft_eligible(p, precision, static_cast<int>(srcDescPtr->n))) cands.push_back({16, 1});
-    const int ft = precision == RPPT_LOGMEL_FP32 ? kFT32 : (nfft >= 768 ? 32 : 64);
-    for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
+    std::vector<int> fts;
+    if (precision == RPPT_LOGMEL_FP32) fts = {kFT32};
+    else if (nfft >= 768) fts = {32};
+    else if (nfft / 256 == 2) fts = {64, 32};
+    else fts = {64};
+    for (int ft : fts)
+        for (int bpc = 1; bpc <= 4; bpc++) cands.push_back({ft, bpc});
     hipEvent_t e0, e1;
     RPPA_HIP(hipEventCreate(&e0));
     RPPA_HIP(hipEventCreate(&e1));

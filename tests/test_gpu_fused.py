@@ -99,6 +99,26 @@ def test_fused_workload2_subset(gpu, handle, orc, precision):
     check_against_oracles(orc, out, bt.x, bt.lengths, precision, "workload2[:6]", **kw)
 
 
+@pytest.mark.parametrize("ft", [64, 32])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_fused_dense_tile_shapes_workload1(gpu, orc, precision, ft):
+    """The dense MFMA kernel at both of its nfft-512 tile shapes (64 frames, each wave two 32-frame
+    row tiles of one column tile; 32 frames, one unit per wave), chosen through the handle's tiles
+    as the autotuner does: against the oracles as above, and the same bits at 1 and 3 persistent
+    blocks per CU."""
+    bt = synth.workload1(batch=24)
+    outs = []
+    for bpc in (1, 3):
+        h = gpu.Handle(len(bt.lengths))
+        gpu.set_tiles(h, ft, bpc)
+        out, frames = run_fused(gpu, h, bt.x, bt.lengths, bt.params, precision)
+        h.close()
+        outs.append(out)
+    for b, F in enumerate(frames):
+        assert np.array_equal(outs[0][b, :, :F], outs[1][b, :, :F]), ("blocks per CU", b)
+    check_against_oracles(orc, outs[0], bt.x, bt.lengths, precision, "workload1[:24]")
+
+
 def test_fused_bf16_single_reports(gpu, handle, orc):
     bt = synth.workload0(batch=4)
     ref, _, frames = orc.logmel_f64(bt.x, bt.lengths)
