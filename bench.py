@@ -62,35 +62,46 @@ def recorded_traffic(workload, precision, tiles):
 def front_end_measurement(api, synth, torch, precision, batch=64, seconds=60.0, steps=5, settle_seconds=0.2):
     """Workload 4 (64 clips): stereo int16-range 48 kHz clips, 60 s each ->
     down-mix -> Kaiser-windowed sinc resample to 16 kHz (16 zero-crossings, 4096-entry table) ->
-    fused log-mel, the three launches replayed from a hipGraph."""
+    fused log-mel through rppt_logmel_front_end_gpu replayed from a hipGraph: as one launch (the
+    log-mel work items compute the resampled samples they read) and, for comparison, as the two
+    launches (down-mix + resample into a buffer, then the fused log-mel)."""
     bt = synth.workload4(batch=batch, seconds=seconds)
     lut, sc, ce = synth.kaiser_sinc_lut()
     x = torch.from_numpy(bt.x).cuda()
-    L = torch.from_numpy(bt.lengths).cuda()
     lut_d = torch.from_numpy(lut).cuda()
+    window = api.make_window(lut_d, sc, ce, 16, 4091)
+    dims = torch.stack([torch.from_numpy(bt.lengths), torch.full((batch,), 2, dtype=torch.int32)], 1).reshape(-1)
+    dims = dims.contiguous().cuda()
     p = api.logmel_params()
-    stream = torch.cuda.Stream()
-    h = api.Handle(batch, stream=stream)
-    ws = {}
-    with torch.cuda.stream(stream):
-        api.front_end_logmel(h, x, L, 2, lut_d, sc, ce, 16, 4091, 48000.0, 16000.0, p, precision, ws)
-        stream.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=stream):
-            api.front_end_logmel(h, x, L, 2, lut_d, sc, ce, 16, 4091, 48000.0, 16000.0, p, precision, ws)
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < settle_seconds:   # untimed, to the clocks of sustained load
-        g.replay()
-        torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.replay()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    n_out = int(np.ceil(np.float32(bt.lengths.max()) * np.float32(16000.0) / np.float32(48000.0)))
+    out = torch.empty((batch, p.numFilter, n_out // p.windowStep + 1), dtype=torch.float32, device="cuda")
     audio = float(bt.lengths.sum()) / 48000.0
-    h.close()
-    return {"batch": batch, "seconds_per_clip": seconds, "ms_per_step": round(wall / steps * 1e3, 3),
-            "audio_s_per_s": round(audio * steps / wall, 1)}
+    res = {"batch": batch, "seconds_per_clip": seconds}
+    stream = torch.cuda.Stream()
+    for one in (True, False):
+        h = api.Handle(batch, stream=stream)
+        with torch.cuda.stream(stream):
+            api.logmel_front_end(h, x, out, dims, 48000.0, 16000.0, window, p, precision, channels=2, one_launch=one)
+            stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=stream):
+                api.logmel_front_end(h, x, out, dims, 48000.0, 16000.0, window, p, precision, channels=2,
+                                     one_launch=one)
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < settle_seconds:   # untimed, to the clocks of sustained load
+            g.replay()
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.replay()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        del g
+        h.close()
+        key = "" if one else "two_launch_"
+        res[key + "ms_per_step"] = round(wall / steps * 1e3, 3)
+        res[key + "audio_s_per_s"] = round(audio * steps / wall, 1)
+    return res
 
 
 def workload_measurement(api, synth, torch, workload, precision, steps=20, settle_seconds=0.1):
@@ -258,6 +269,14 @@ def main() -> int:
     from rpp_amd import api, synth
 
     torch.cuda.set_device(0)
+    if a.copy_calibration_mb:
+        n = a.copy_calibration_mb * (1 << 20) // 4
+        src_c = torch.rand(n, device="cuda")
+        dst_c = torch.empty_like(src_c)
+        for _ in range(3):
+            dst_c.copy_(src_c)
+        torch.cuda.synchronize()
+        del src_c, dst_c
     if a.front_end_only:
         print(json.dumps({"front_end": front_end_measurement(api, synth, torch, a.precision, steps=a.steps)}))
         return 0
@@ -267,14 +286,6 @@ def main() -> int:
                               min_freq=f["fmin"], max_freq=f["fmax"], num_filter=f["nmel"])
         print(json.dumps({"trim_workload1": trim_measurement(api, synth, torch, p, a.precision, steps=a.steps)}))
         return 0
-    if a.copy_calibration_mb:
-        n = a.copy_calibration_mb * (1 << 20) // 4
-        src_c = torch.rand(n, device="cuda")
-        dst_c = torch.empty_like(src_c)
-        for _ in range(3):
-            dst_c.copy_(src_c)
-        torch.cuda.synchronize()
-        del src_c, dst_c
     bt = synth.make(a.workload)
     feat = bt.params
     B = len(bt.lengths)

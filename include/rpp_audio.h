@@ -292,6 +292,21 @@ RppStatus rppt_logmel_fused_region_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, 
                                        const RpptLogMelParams *params, RpptLogMelPrecision precision,
                                        rppHandle_t rppHandle);
 
+#ifdef __cplusplus
+/* Down-mix, resample and the fused log-mel chain: src interleaved [n][samples * channels] f32 at
+ * inRate (srcDimsTensor[2b] = samples per channel, [2b+1] = channels; device or pinned), dst as
+ * rppt_logmel_fused_gpu at outRate, each clip's length ceil(samples * outRate / inRate) -- the output
+ * of rppt_down_mix_resample_gpu (one rate pair for the batch) fed to rppt_logmel_fused_gpu, bit for
+ * bit.  oneLaunch != 0 runs it as one launch whose work items compute the resampled samples they
+ * read (the resampled clips never reach memory) where that launch applies: FP16X3, nfft 512, hop
+ * 160, power 2, NFT output, referenceMagnitude != 0, inRate / outRate = 2 or 3 and window.lobes = 16;
+ * otherwise, and with oneLaunch = 0, the two launches through a buffer of the handle. */
+RppStatus rppt_logmel_front_end_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr, RpptDescPtr dstDescPtr,
+                                    Rpp32s *srcDimsTensor, Rpp32f inRate, Rpp32f outRate, bool normalizeWeights,
+                                    RpptResamplingWindow &window, const RpptLogMelParams *params,
+                                    RpptLogMelPrecision precision, Rpp32s oneLaunch, rppHandle_t rppHandle);
+#endif
+
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * output as rppt_logmel_fused_gpu, but the call never allocates: the DFT and mel tables, the tile

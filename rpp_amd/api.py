@@ -105,6 +105,9 @@ def lib():
         l.rppt_down_mixing_gpu.argtypes = [P, DP, P, DP, P, C.c_bool, P]
         l.rppt_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.POINTER(RpptResamplingWindow), P]
         l.rppt_down_mix_resample_gpu.argtypes = [P, DP, P, DP, P, P, P, C.c_bool, C.POINTER(RpptResamplingWindow), P]
+        l.rppt_logmel_front_end_gpu.argtypes = [P, DP, P, DP, P, C.c_float, C.c_float, C.c_bool,
+                                                C.POINTER(RpptResamplingWindow), C.POINTER(RpptLogMelParams), C.c_int,
+                                                C.c_int32, P]
         l.rppt_logmel_chain_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_noalloc_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
@@ -322,6 +325,22 @@ def logmel_fused(h: Handle, src, dst, lengths, params: RpptLogMelParams, precisi
                                      PRECISION[precision], h.h)
     if check_status:
         check("rppt_logmel_fused_gpu", st)
+    return st
+
+
+def logmel_front_end(h: Handle, src, dst, src_dims, in_rate: float, out_rate: float, window: RpptResamplingWindow,
+                     params: RpptLogMelParams, precision="fp16x3", channels=2, normalize=False, one_launch=True,
+                     check_status=True, layout=NFT) -> int:
+    """rppt_logmel_front_end_gpu: src [B, L*C] interleaved at in_rate, src_dims int32 [B*2] = (L, C);
+    dst as logmel_fused at out_rate.  one_launch=False forces the two-launch form."""
+    B, S = src.shape
+    sd = desc_1d(B, S, num_dims=3, channels=channels)
+    dd = desc_2d(B, dst.shape[1], dst.shape[2], layout=layout)
+    st = lib().rppt_logmel_front_end_gpu(ptr(src), C.byref(sd), ptr(dst), C.byref(dd), ptr(src_dims), in_rate,
+                                         out_rate, normalize, C.byref(window), C.byref(params), PRECISION[precision],
+                                         1 if one_launch else 0, h.h)
+    if check_status:
+        check("rppt_logmel_front_end_gpu", st)
     return st
 
 

@@ -110,6 +110,9 @@ struct rppHandle {
     // grow-only device workspace
     void *ws = nullptr;
     size_t wsBytes = 0;
+    // grow-only buffer of the front end's two-launch form (the resampled clips and their lengths)
+    void *feWs = nullptr;
+    size_t feWsBytes = 0;
 
     // small pinned staging buffer for host-built metadata
     void *pinned = nullptr;
@@ -128,6 +131,9 @@ namespace rppa {
 // Returns a device pointer of at least `bytes` (grow-only); nullptr on failure or when the
 // stream is being captured and the pool would have to grow.
 void *workspace(rppHandle *h, size_t bytes);
+// The same for the front end's resampled intermediate (a buffer of its own, so the log-mel launch
+// that reads it can use the workspace).
+void *front_workspace(rppHandle *h, size_t bytes);
 
 // Window-folded DFT basis for (nfft, W, window); window == nullptr means RPP's Hann.
 // splitKind: -1 FP32 only, 0 bf16 hi/lo, 1 fp16 hi/lo (scaled).
@@ -151,6 +157,24 @@ hipError_t build_mel_tables_at(hipStream_t st, int numBins, float sampleRate, fl
                                int M, int normalize, int *range, MelTables &t);
 
 // Factored-DFT fused kernel (logmel_fft.hip).
+// The fused front end of the factored-DFT launch: interleaved clips at an integer multiple k of the
+// output rate, down-mixed and resampled inside the staging (down_mix_resample_launch's arithmetic).
+struct FrontEnd {
+    const float *src;    // [B][stride] interleaved
+    uint32_t stride;
+    const int *dims;     // [2 b] samples per channel, [2 b + 1] channels
+    float inRate, outRate;
+    int k;               // inRate / outRate, 2 or 3
+    int normalize;       // down-mix weights normalised
+    const float *lut;    // the resampling window table
+    float wScale, wCenter;
+    int lookupSize;
+};
+// down-mix + resample of a batch at one rate pair (resample.hip's kernel, rates given once) into
+// dst [B][dstStride], and each clip's resampled length into lengths[B]
+hipError_t down_mix_resample_launch(hipStream_t st, const float *src, uint32_t srcStride, float *dst,
+                                    uint32_t dstStride, unsigned B, const int *dims, float inRate, float outRate,
+                                    const RpptResamplingWindow &w, int normalize, int *lengths);
 struct FftLaunch {
     const RpptLogMelParams *p;
     const float *src;
@@ -168,10 +192,13 @@ struct FftLaunch {
     float minRatio, mult, invRef;
     const FftTables *ft;
     const MelFrag *mf;
+    const FrontEnd *fe;  // null: the clips are given at the output rate
 };
 // the launch shape it supports (parameters only; the LDS check needs the mel pair count)
 bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int B);
-size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut);
+// the fused front end's resampling shapes: integer ratio k and window lobes
+bool fft_front_eligible(int k, int lobes);
+size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false);
 const FftTables *get_fft_tables(rppHandle *h, int nfft, int W);
 const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string &key);
 // the same tables in caller-provided memory (no allocation): fft_table_bytes() bytes at `mem`

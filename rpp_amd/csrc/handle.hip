@@ -11,26 +11,32 @@
 
 namespace rppa {
 
-void *workspace(rppHandle *h, size_t bytes)
+namespace {
+void *grow(rppHandle *h, void *&buf, size_t &cap, size_t bytes)
 {
-    if (bytes <= h->wsBytes) return h->ws;
+    if (bytes <= cap) return buf;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
         fprintf(stderr, "rpp_audio: workspace must grow to %zu bytes while the stream is being captured; "
                         "run the call once before capture\n", bytes);
         return nullptr;
     }
-    if (h->ws) {
+    if (buf) {
         if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
-        (void)hipFree(h->ws);
-        h->ws = nullptr;
-        h->wsBytes = 0;
+        (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
     }
     size_t want = bytes + bytes / 4;
-    if (hipMalloc(&h->ws, want) != hipSuccess) return nullptr;
-    h->wsBytes = want;
-    return h->ws;
+    if (hipMalloc(&buf, want) != hipSuccess) return nullptr;
+    cap = want;
+    return buf;
 }
+}  // namespace
+
+void *workspace(rppHandle *h, size_t bytes) { return grow(h, h->ws, h->wsBytes, bytes); }
+
+void *front_workspace(rppHandle *h, size_t bytes) { return grow(h, h->feWs, h->feWsBytes, bytes); }
 
 }  // namespace rppa
 
@@ -77,6 +83,7 @@ RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
     }
     (void)hipStreamSynchronize(handle->stream);
     if (handle->ws) (void)hipFree(handle->ws);
+    if (handle->feWs) (void)hipFree(handle->feWs);
     if (handle->pinned) (void)hipHostFree(handle->pinned);
     for (auto &kv : handle->dft) {
         (void)hipFree(kv.second.basis);

@@ -48,6 +48,8 @@
 //             the filters touch, the weight fragments in LDS;
 //   dB        in registers, stored as 64-byte runs of a mel row (NFT) or four filters (NTF).
 #include "common.hpp"
+#include "down_mix_math.hpp"
+#include "resample_math.hpp"
 
 #include <cmath>
 
@@ -68,6 +70,13 @@ constexpr int kWaves = 8;             // waves per workgroup (two per SIMD)
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxMT = 8;             // filter tiles of 16 (numFilter <= 128)
 constexpr int kChunks = kN / 2 / 32;  // 32-bin chunks of bins 0 .. N/2 - 1
+// fused front end: 2 x 16 FIR taps (16 zero crossings), resampled outputs per chunk (two per lane)
+// and the down-mixed inputs a chunk reads at up to 3 inputs per output
+constexpr int kFeTaps = 32;
+constexpr int kFeOut = 128;
+constexpr int kFeMaxK = 3;
+constexpr int kFeSpan = 448;
+static_assert(kFeSpan >= kFeMaxK * (kFeOut - 1) + kFeTaps && kFeSpan % 64 == 0, "front-end span");
 
 // ------------------------------------------------------------------------- tables (FP64)
 __device__ inline uint16_t f16_bits(double v)
@@ -271,6 +280,18 @@ struct FftArgs {
     const int *pairOff;
     int nPairs;
     int pitch;       // floats per phase row of a wave's segment
+    // fused front end (FRONT kernels): interleaved multi-channel clips fsrc[b * fStride + t * C + c],
+    // fDims[2 b] samples per channel and fDims[2 b + 1] channels, down-mixed and resampled by the
+    // integer ratio fK = inRate / outRate with the window table fLut; the resampled clip is what the
+    // rest of the kernel sees
+    const float *fsrc;
+    uint32_t fStride;
+    const int *fDims;
+    int fK, fNormalize;
+    float fIr, fOr;
+    const float *fLut;
+    float fWScale, fWCenter;
+    int fLookup;
 };
 
 struct Clip {
@@ -457,7 +478,10 @@ constexpr int pitch_for(int positions)
 // GENERAL = false: the common launch (power 2, NFT output, a fixed dB reference, a cut-off whose
 // ratio is a normal float); true: any power, NTF, the clip-maximum reference, subnormal ratios.
 // HOP: the hop in samples (a multiple of 32), which fixes the segment geometry.
-template <int NMT, int HOP, bool GENERAL>
+// FRONT: 0, or the clips arrive interleaved at FRONT (2 or 3) times the rate and each item computes
+// the down-mixed, resampled samples it reads (bit for bit what the down-mix + resample launch
+// writes) instead of loading them.
+template <int NMT, int HOP, bool GENERAL, int FRONT>
 __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
 {
     constexpr int HQ = HOP / 16;                     // positions per hop
@@ -465,6 +489,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     constexpr int PITCH = pitch_for(SEGN / 16);
     constexpr int ROUNDS = (SEGN + 1023) / 1024;
     static_assert(HOP % 32 == 0 && ROUNDS <= 3, "segment geometry");
+    static_assert(PITCH - SEGN / 16 >= 2, "two spare columns for the front end's predecessors");
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // LDS: stage fragments a1 (4 KB) and a2 (8 KB), window [16][32], twiddles [16][16] float2, mel
@@ -479,10 +504,13 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     const int B4 = (a.B + 1 + 3) / 4 * 4;
     int *ts = reinterpret_cast<int *>(melF + a.nPairs * 2 * 64);
     int *lenL = ts + B4;
-    int *startL = lenL + B4;                                       // used with a begin array
-    long long *offL = reinterpret_cast<long long *>(startL + (a.begin ? B4 : 0));   // packed: [B][2]
+    int *startL = lenL + B4;                                       // region start; front end: input samples
+    int *chanL = startL + ((a.begin || FRONT) ? B4 : 0);           // front end: channels
+    long long *offL = reinterpret_cast<long long *>(chanL + (FRONT ? B4 : 0));   // packed: [B][2]
     int *chunkW = reinterpret_cast<int *>(offL + (a.srcOff ? 2 * B4 : 0));
-    const int segOff = static_cast<int>(reinterpret_cast<float *>(chunkW + 16) - lds);
+    float *firW = reinterpret_cast<float *>(chunkW + 16);          // front end: the 2 lobes FIR taps
+    float *dspan = firW + (FRONT ? kFeTaps : 0);                   // front end: [wave][kFeSpan] inputs
+    const int segOff = static_cast<int>(dspan + (FRONT ? kWaves * kFeSpan : 0) - lds);
 
     {
         const u32x4 *s1 = reinterpret_cast<const u32x4 *>(a.a1), *s2 = reinterpret_cast<const u32x4 *>(a.a2);
@@ -496,7 +524,15 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     // per-clip metadata into LDS (the item loop then reads no scalar memory: scalar loads and LDS
     // reads share one completion counter, so a scalar load's latency would stall every LDS wait)
     for (int b = tid; b < a.B; b += kThreads) {
-        if (a.srcOff) {
+        if (FRONT) {
+            // the resample launch's output length: ceil(samples * outRate / inRate) in FP32
+            const int C = max(a.fDims[2 * b + 1], 1);
+            const int Lin = max(min(a.fDims[2 * b], static_cast<int>(a.fStride / C)), 0);
+            const float lf = static_cast<float>(Lin) * a.fOr;
+            lenL[b] = static_cast<int>(ceilf(lf / a.fIr));
+            startL[b] = Lin;
+            chanL[b] = C;
+        } else if (a.srcOff) {
             lenL[b] = max(a.lengths[b], 0);
             offL[2 * b] = a.srcOff[b];
             offL[2 * b + 1] = a.dstOff[b];
@@ -507,6 +543,12 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         }
     }
     if (tid < 16) chunkW[tid] = a.pairOff[tid];
+    if (FRONT && tid < kFeTaps) {
+        // an unclipped output's window values at offsets -lobes .. lobes - 1 (all four-tap groups:
+        // the truncating lookup), as the resample launch computes them
+        const Win win{a.fLut, a.fWScale, a.fWCenter, a.fLookup};
+        firW[tid] = win.trunc_lookup(static_cast<float>(tid - kFeTaps / 2));
+    }
     __shared__ int workCtrS;
     int *workCtr = &workCtrS;
     if (tid == 0) workCtrS = 0;
@@ -578,6 +620,98 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
             }
         }
     };
+    // Front end: segment slot of item position i (i = sample - s0): the polyphase position
+    // (i % 16) PITCH + i / 16; the same formula with an arithmetic shift puts the predecessors
+    // i = -1, -2 in the last (spare) column of phase rows 14 and 13.  Clamped, so a lane whose value
+    // is discarded never reads outside the segment; branch-free.
+    auto slot = [&](int i) {
+        i = min(max(i, -2), 16 * (SEGN / 16) - 1);
+        return (i & 15) * PITCH + (i >> 4);
+    };
+    // Front end: the resampled samples [lo, hi) item t reads -- from two before its segment start to
+    // its last valid sample (and at least to the reflection of its start), or the whole clip when it
+    // is at most half a window long (every index the padding rules reach lies in that range, checked
+    // over all clip lengths to 2000 and a sample beyond) -- 128 outputs per round, two per lane, from the
+    // down-mixed inputs of the round staged in the wave's span (the next round's loads in flight
+    // during this round's products).  Each output is the resample launch's arithmetic: the plain FIR
+    // in the HOST order where all 32 taps lie in the clip, the clipped per-tap form at its ends.  The
+    // outputs go raw into the segment slots (a short clip: linearly, then copied to the span).
+    auto produce = [&](const Item &t, float *seg, int lane) {
+        const int b = t.v.b;
+        const int Lin = __builtin_amdgcn_readfirstlane(startL[b]);
+        const int C = __builtin_amdgcn_readfirstlane(chanL[b]);
+        const float *xin = a.fsrc + static_cast<size_t>(b) * a.fStride;
+        const float dw = down_mix_weight(C, a.fNormalize);
+        constexpr int k = FRONT > 0 ? FRONT : 1;
+        const int Lr = t.v.L;
+        const bool tiny = !t.interior && Lr <= a.W / 2 + 1;
+        const int lo = tiny ? 0 : max(t.s0 - 2, 0);
+        const int hi = tiny ? Lr : min(max(t.s0 + t.valid, 1 - t.s0), Lr);   // reflections reach -s0
+        float *dsp = dspan + (lane >= 0 ? wave : 0) * kFeSpan;
+        float w[kFeTaps];
+#pragma unroll
+        for (int q = 0; q < kFeTaps; q++) w[q] = firW[q];
+        const Win win{a.fLut, a.fWScale, a.fWCenter, a.fLookup};
+        const bool pairs = C == 2 && (reinterpret_cast<uintptr_t>(xin) & 7) == 0;
+        constexpr int U = kFeSpan / 64;
+        float2 v[U];
+        auto load = [&](int P0) {
+            const int d0 = k * P0 - kFeTaps / 2;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int ti = min(max(d0 + lane + 64 * u, 0), Lin - 1);
+                v[u] = *reinterpret_cast<const float2 *>(xin + 2 * static_cast<size_t>(ti));
+            }
+        };
+        if (pairs && lo < hi) load(lo);
+        for (int P0 = lo; P0 < hi; P0 += kFeOut) {
+            const int d0 = k * P0 - kFeTaps / 2;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int ti = d0 + lane + 64 * u;
+                float d;
+                if (pairs) {
+                    const float pr[2] = {v[u].x, v[u].y};
+                    d = down_mix_sample(pr, 2, dw);
+                } else {
+                    const int tc = min(max(ti, 0), Lin - 1);
+                    d = down_mix_sample(xin + static_cast<size_t>(tc) * C, C, dw);
+                }
+                dsp[lane + 64 * u] = (ti >= 0 && ti < Lin) ? d : 0.0f;   // outside the clip: never read
+            }
+            if (pairs && P0 + kFeOut < hi) load(P0 + kFeOut);
+            // the lane's two outputs read the k + 32 inputs from 2 k lane on: 8-byte reads into
+            // registers, shared by both FIRs
+            constexpr int NX = (kFeTaps + k + 1) / 2 * 2;
+            float xs[NX];
+            {
+                const float2 *xp = reinterpret_cast<const float2 *>(dsp + 2 * k * lane);
+#pragma unroll
+                for (int q = 0; q < NX / 2; q++) {
+                    const float2 u = xp[q];
+                    xs[2 * q] = u.x;
+                    xs[2 * q + 1] = u.y;
+                }
+            }
+#pragma unroll
+            for (int uo = 0; uo < 2; uo++) {
+                const int P = P0 + 2 * lane + uo;
+                const int c = k * P;
+                float r;
+                if (c - kFeTaps / 2 >= 0 && c + kFeTaps / 2 <= Lin)
+                    r = fir_one<kFeTaps>(xs + k * uo, w, 0);
+                else
+                    r = resample_one([&](int li) { return dsp[li - d0]; }, win, static_cast<float>(c),
+                                     max(c - kFeTaps / 2, 0), min(c + kFeTaps / 2, Lin));
+                if (P < hi) {
+                    if (tiny) seg[P] = r;
+                    else seg[slot(P - t.s0)] = r;
+                }
+            }
+        }
+        if (tiny)
+            for (int i = lane; i < Lr; i += 64) dsp[i] = seg[i];
+    };
     // Stages item t into this wave's segment (pre-emphasised, scaled, polyphase) and returns the scale.
     // Interior items from px/pp; items at a clip edge gather each sample with the padding rule (the
     // index and its predecessor's resolved first, both clamped into the clip, so every lane's loads
@@ -585,11 +719,27 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     auto stage = [&](const Item &t) {
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
         float *seg = lds + segOff + static_cast<size_t>(lane >= 0 ? wave : 0) * 16 * PITCH;
+        if (FRONT) produce(t, seg, lane);
         float y[ROUNDS][4][4];
         float mx = 0.0f;
         const float c = a.coeff;
         float scale;
         if (t.interior) {
+            if (FRONT) {
+                // the values prefetch() would have loaded, from the slots
+#pragma unroll
+                for (int r = 0; r < ROUNDS; r++)
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const int i = 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
+                        const bool in = i < t.valid;
+                        const int ic = min(i, t.valid - 4);
+                        const float4 q = make_float4(seg[slot(ic)], seg[slot(ic + 1)], seg[slot(ic + 2)], seg[slot(ic + 3)]);
+                        const float pv = seg[slot(ic - 1)];
+                        px[r][m] = in ? q : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        pp[r][m] = in ? pv : 0.0f;
+                    }
+            }
             // the scale from the raw samples: |x[n] - c x[n-1]| <= (1 + |c|) max |x|, so the
             // pre-emphasised samples are formed in the scaled domain at once, one FMA each
 #pragma unroll
@@ -639,8 +789,8 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                         const int pj = j == 0 ? p0 : j - 1;
                         const bool pz = j == 0 && p0zero;
                         // both loads unconditional (in-bounds indices); the rules applied by selects
-                        const float xv = cv.x[j];
-                        const float pl = cv.x[min(pj, L - 1)];
+                        const float xv = FRONT ? seg[slot(j - t.s0)] : cv.x[j];
+                        const float pl = FRONT ? seg[slot(min(pj, L - 1) - t.s0)] : cv.x[min(pj, L - 1)];
                         const float u = c * (pz ? 0.0f : pl);
                         const float yv = xv - u;
                         y[r][m][e] = zero ? 0.0f : yv;
@@ -660,8 +810,9 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         } else {
             // a clip no longer than half a window (or empty): sample by sample with RPP's reflect map
             // (any number of bounces), straight into LDS, rescaled below
+            const float *xs = FRONT ? dspan + wave * kFeSpan : t.v.x;
             for (int i = lane; i < 16 * PITCH; i += 64) {
-                const float yv = (i < t.valid && t.v.L > 0) ? pe_at(t.v.x, t.v.L, t.s0 + i, c, a.border, a.reflect) : 0.0f;
+                const float yv = (i < t.valid && t.v.L > 0) ? pe_at(xs, t.v.L, t.s0 + i, c, a.border, a.reflect) : 0.0f;
                 seg[(i & 15) * PITCH + (i >> 4)] = yv;
                 mx = fmaxf(mx, fabsf(yv));
             }
@@ -707,7 +858,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     Item cur{};
     if (k < nMine) {
         cur = describe(item, owner(item, 0));
-        if (cur.interior) prefetch(cur);
+        if (!FRONT && cur.interior) prefetch(cur);
     }
 
     while (k < nMine) {
@@ -857,7 +1008,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         Item nxt{};
         if (nextItem < T) {
             nxt = describe(nextItem, clipNext);
-            if (nxt.interior) prefetch(nxt);
+            if (!FRONT && nxt.interior) prefetch(nxt);
         }
 
         // ---- banded mel on MFMA, pairs in their table order (chunk by chunk, tiles ascending): chunk
@@ -1026,14 +1177,16 @@ const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string
 }
 
 // LDS bytes of the factored kernel for hop H, B clips and nPairs mel pairs.
-size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut)
+size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front)
 {
     const int pitch = pitch_for(((kFI - 1) * H + kN) / 16);
     if (pitchOut) *pitchOut = pitch;
     const size_t B4 = static_cast<size_t>((B + 1 + 3) / 4 * 4);
-    const size_t perClip = sizeof(int) * B4 * (2 + (begin ? 1 : 0)) + (packed ? sizeof(long long) * 2 * B4 : 0);
+    const size_t perClip = sizeof(int) * B4 * (2 + ((begin || front) ? 1 : 0) + (front ? 1 : 0)) +
+                           (packed ? sizeof(long long) * 2 * B4 : 0);
+    const size_t frontBytes = front ? sizeof(float) * (kFeTaps + kWaves * kFeSpan) : 0;
     return 4096 + 8192 + sizeof(float) * (kN2 * kN1 + 512) + static_cast<size_t>(nPairs) * 2048 + perClip +
-           sizeof(int) * 16 + sizeof(float) * kWaves * 16 * static_cast<size_t>(pitch);
+           sizeof(int) * 16 + frontBytes + sizeof(float) * kWaves * 16 * static_cast<size_t>(pitch);
 }
 
 // nfft 512 in fp16x3 with W <= 512, a hop that is a multiple of 32 with the 16-frame segment within
@@ -1046,11 +1199,16 @@ bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int 
     return p->numFilter <= 16 * kMaxMT && B <= 4096;
 }
 
+bool fft_front_eligible(int k, int lobes) { return k >= 2 && k <= kFeMaxK && 2 * lobes == kFeTaps; }
+
 RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
 {
     const RpptLogMelParams *p = L.p;
     int pitch = 0;
-    const size_t bytes = fft_lds_bytes(p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr, &pitch);
+    const bool front = L.fe != nullptr;
+    if (front && (L.begin || L.srcOff)) return RPP_ERROR_NOT_IMPLEMENTED;
+    const size_t bytes = fft_lds_bytes(p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr, &pitch,
+                                       front);
     if (bytes > static_cast<size_t>(h->maxLdsOptin)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
     FftArgs a{};
     a.src = L.src;
@@ -1085,22 +1243,44 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     a.pairOff = L.mf->pairOff;
     a.nPairs = L.mf->nPairs;
     a.pitch = pitch;
+    if (front) {
+        a.src = L.fe->src;   // only offset, never read: the kernel reads fsrc
+        a.fsrc = L.fe->src;
+        a.fStride = L.fe->stride;
+        a.fDims = L.fe->dims;
+        a.fK = L.fe->k;
+        a.fNormalize = L.fe->normalize;
+        a.fIr = L.fe->inRate;
+        a.fOr = L.fe->outRate;
+        a.fLut = L.fe->lut;
+        a.fWScale = L.fe->wScale;
+        a.fWCenter = L.fe->wCenter;
+        a.fLookup = L.fe->lookupSize;
+    }
     if (L.mf->nPairs <= 0) return RPP_ERROR_NOT_IMPLEMENTED;
     const int nMT = (p->numFilter + 15) / 16;
     const unsigned grid = static_cast<unsigned>(h->numCU);
     const bool general = a.power != 2 || a.ntf || a.clipMax || !(a.minRatio >= 1.17549435e-38f);
-#define RPPA_FFT_LAUNCH(NMT, GEN)                                                                            \
+    // the front end runs with the common launch only (the caller falls back to two launches)
+    if (front && general) return RPP_ERROR_NOT_IMPLEMENTED;
+#define RPPA_FFT_LAUNCH(NMT, GEN, FE)                                                                        \
     do {                                                                                                     \
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_fft_kernel<NMT, 160, GEN>),         \
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_fft_kernel<NMT, 160, GEN, FE>),     \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));      \
-        hipLaunchKernelGGL((logmel_fft_kernel<NMT, 160, GEN>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
+        hipLaunchKernelGGL((logmel_fft_kernel<NMT, 160, GEN, FE>), dim3(grid), dim3(kThreads), bytes, h->stream, a); \
     } while (0)
+    const int fk = front ? L.fe->k : 0;
+    if (front && fk != 2 && fk != 3) return RPP_ERROR_NOT_IMPLEMENTED;
     if (nMT <= 5) {
-        if (general) RPPA_FFT_LAUNCH(5, true);
-        else RPPA_FFT_LAUNCH(5, false);
+        if (fk == 3) RPPA_FFT_LAUNCH(5, false, 3);
+        else if (fk == 2) RPPA_FFT_LAUNCH(5, false, 2);
+        else if (general) RPPA_FFT_LAUNCH(5, true, 0);
+        else RPPA_FFT_LAUNCH(5, false, 0);
     } else {
-        if (general) RPPA_FFT_LAUNCH(8, true);
-        else RPPA_FFT_LAUNCH(8, false);
+        if (fk == 3) RPPA_FFT_LAUNCH(8, false, 3);
+        else if (fk == 2) RPPA_FFT_LAUNCH(8, false, 2);
+        else if (general) RPPA_FFT_LAUNCH(8, true, 0);
+        else RPPA_FFT_LAUNCH(8, false, 0);
     }
 #undef RPPA_FFT_LAUNCH
     RPPA_HIP(hipGetLastError());

@@ -1415,7 +1415,7 @@ std::string mel_key(const RpptLogMelParams *p, int K)
 RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPrecision precision, const float *src,
                         uint32_t srcStride, const long long *srcOff, const int *lengths, int B, float *dst,
                         uint32_t dstNStride, uint32_t hStride, const long long *dstOff, int ntf, uint32_t frameCap,
-                        const int *begin = nullptr, const LaunchMem *mem = nullptr)
+                        const int *begin = nullptr, const LaunchMem *mem = nullptr, const FrontEnd *fe = nullptr)
 {
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
@@ -1435,12 +1435,14 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         ft = mem ? mem->ft : get_fft_tables(h, nfft, p->windowLength);
         mf = mem ? mem->mf : get_mel_frag(h, mt, mel_key(p, K));
         if (!ft || !mf || mf->nPairs <= 0 ||
-            fft_lds_bytes(p->windowStep, B, mf->nPairs, begin != nullptr, srcOff != nullptr, nullptr) >
+            fft_lds_bytes(p->windowStep, B, mf->nPairs, begin != nullptr, srcOff != nullptr, nullptr, fe != nullptr) >
                 static_cast<size_t>(h->maxLdsOptin)) {
             ft = nullptr;
             mf = nullptr;
         }
     }
+    // the fused front end needs the factored kernel's common launch; the caller falls back otherwise
+    if (fe && (!ft || clipRef || ntf || p->power != 2 || begin || srcOff)) return RPP_ERROR_NOT_IMPLEMENTED;
     // tile shape: FP32 kernel 32 frames per tile; MFMA kernel 4 waves of CT = nfft/256 32-bin column
     // tiles each, 64 frames per tile up to nfft 512 and 32 above
     // (nfft 512 also runs 32-frame tiles, one unit per wave, when the handle's tiles ask for them)
@@ -1559,6 +1561,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         L.invRef = a.invRef;
         L.ft = ft;
         L.mf = mf;
+        L.fe = fe;
         const RppStatus st = logmel_fft_launch(h, L);
         if (st != RPP_SUCCESS) return st;
     } else if (!mfma) {
@@ -1688,6 +1691,62 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_region_gpu(RppPtr_t srcPtr, RpptDescPtr 
 {
     if (!srcBeginTensor) return RPP_ERROR_INVALID_ARGUMENTS;
     return rppa::fused_entry(srcPtr, srcDescPtr, dstPtr, dstDescPtr, srcBeginTensor, srcLengthTensor, p, precision, h);
+}
+
+// Down-mix, resample and the fused log-mel: interleaved clips at inRate in, dB at outRate out -- the
+// chain rppt_down_mix_resample_gpu -> rppt_logmel_fused_gpu over the resampled clips.  With oneLaunch
+// and where the factored-DFT kernel's common launch applies (fp16x3, nfft 512, hop 160, power 2,
+// NFT, a fixed reference), inRate / outRate is 2 or 3 and the window has 16 lobes, it is one launch
+// whose work items compute the down-mixed, resampled samples they read; otherwise the two launches
+// through a buffer of the handle.  Both forms give the same bits.
+RPPA_EXPORT RppStatus rppt_logmel_front_end_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_t dstPtr,
+                                                RpptDescPtr dstDescPtr, Rpp32s *srcDimsTensor, Rpp32f inRate,
+                                                Rpp32f outRate, bool normalizeWeights, RpptResamplingWindow &window,
+                                                const RpptLogMelParams *p, RpptLogMelPrecision precision,
+                                                Rpp32s oneLaunch, rppHandle_t h)
+{
+    if (!h || !srcDescPtr || !dstDescPtr || !p) return RPP_ERROR_INVALID_ARGUMENTS;
+    const size_t dims = srcDescPtr->numDims - 1;
+    if (dims != 1 && dims != 2) return RPP_ERROR_INVALID_SRC_DIMS;
+    if (dstDescPtr->layout != NFT && dstDescPtr->layout != NTF) return RPP_ERROR_INVALID_DST_LAYOUT;
+    if (srcDescPtr->dataType != F32 || dstDescPtr->dataType != F32) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (!srcPtr || !dstPtr || !srcDimsTensor || !window.lookupPinned) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (window.lookupSize < 2 || window.lookupSize > 16384 || window.lobes < 0) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (!(inRate > 0.0f) || !(outRate > 0.0f)) return RPP_ERROR_INVALID_ARGUMENTS;
+    RppStatus st = check_params(p, precision);
+    if (st != RPP_SUCCESS) return st;
+    const unsigned B = srcDescPtr->n;
+    if (B == 0) return RPP_SUCCESS;
+    const int ntf = dstDescPtr->layout == NTF ? 1 : 0;
+    if (ntf ? dstDescPtr->strides.hStride < static_cast<unsigned>(p->numFilter)
+            : dstDescPtr->h < static_cast<unsigned>(p->numFilter))
+        return RPP_ERROR_INVALID_DST_DIMS;
+    const uint32_t frameCap = ntf ? dstDescPtr->h : dstDescPtr->strides.hStride;
+    const float *src = static_cast<const float *>(srcPtr);
+    const uint32_t srcStride = srcDescPtr->strides.nStride;
+    float *dst = static_cast<float *>(dstPtr);
+    const double ratio = static_cast<double>(inRate) / outRate;
+    const int k = static_cast<int>(ratio);
+    if (oneLaunch && static_cast<double>(k) == ratio && fft_front_eligible(k, window.lobes)) {
+        const FrontEnd fe{src, srcStride, srcDimsTensor, inRate, outRate, k, normalizeWeights ? 1 : 0,
+                          window.lookupPinned, window.scale, window.center, window.lookupSize};
+        st = logmel_launch(h, p, precision, src, srcStride, nullptr, nullptr, static_cast<int>(B), dst,
+                           dstDescPtr->strides.nStride, dstDescPtr->strides.hStride, nullptr, ntf, frameCap, nullptr,
+                           nullptr, &fe);
+        if (st != RPP_ERROR_NOT_IMPLEMENTED) return st;
+    }
+    // two launches: every clip has at most srcStride samples per channel, so its resampled length
+    // ceil(L * outRate / inRate) fits a row of rs
+    const uint32_t rs = static_cast<uint32_t>(std::ceil(static_cast<double>(srcStride) * outRate / inRate)) + 8;
+    const size_t lenOff = (static_cast<size_t>(B) * rs * sizeof(float) + 255) / 256 * 256;
+    char *buf = static_cast<char *>(front_workspace(h, lenOff + sizeof(int) * B));
+    if (!buf) return RPP_ERROR_NOT_ENOUGH_MEMORY;
+    float *res = reinterpret_cast<float *>(buf);
+    int *lens = reinterpret_cast<int *>(buf + lenOff);
+    RPPA_HIP(down_mix_resample_launch(h->stream, src, srcStride, res, rs, B, srcDimsTensor, inRate, outRate, window,
+                                      normalizeWeights ? 1 : 0, lens));
+    return logmel_launch(h, p, precision, res, rs, nullptr, lens, static_cast<int>(B), dst, dstDescPtr->strides.nStride,
+                         dstDescPtr->strides.hStride, nullptr, ntf, frameCap);
 }
 
 // Autotuner: times every tile shape the launch supports on the caller's batch (events on the

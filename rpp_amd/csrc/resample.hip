@@ -21,6 +21,7 @@
 // lookups give.
 #include "common.hpp"
 #include "down_mix_math.hpp"
+#include "resample_math.hpp"
 
 namespace rppa {
 namespace {
@@ -31,80 +32,6 @@ constexpr int kMaxScale = 3;   // inputs per output up to which a block's input 
                                // blocks share a CU)
 constexpr int kMaxTaps = 64;   // 2 * lobes up to which the span is staged
 constexpr int kSpan = kBlockOut * kMaxScale + kMaxTaps + 8;   // staged floats per wave
-
-struct Win {
-    const float *lut;
-    float scale, center;
-    int lookupSize;
-    __device__ float floor_lookup(float x) const
-    {
-        float locRaw = x * scale + center;
-        int i = static_cast<int>(floorf(locRaw));
-        float weight = locRaw - static_cast<float>(i);
-        i = max(min(i, lookupSize - 2), 0);
-        float cur = lut[i], nxt = lut[i + 1];
-        return cur + weight * (nxt - cur);
-    }
-    __device__ float trunc_lookup(float x) const
-    {
-        float locRaw = x * scale + center;
-        int i = static_cast<int>(locRaw);
-        float weight = locRaw - static_cast<float>(i);
-        i = max(min(i, lookupSize - 2), 0);  // in range for every tap the HOST path visits
-        float cur = lut[i], nxt = lut[i + 1];
-        return cur + weight * (nxt - cur);
-    }
-};
-
-// One mono output in the HOST order: taps [loc0, loc1) of the block's input `in` (indexable by
-// the tap index relative to the block start), window at li - inPos.
-template <typename In>
-__device__ __forceinline__ float resample_one(const In &in, const Win &win, float inPos, int loc0, int loc1)
-{
-    int li = loc0;
-    const float locBegin = static_cast<float>(li) - inPos;
-    float p0 = locBegin + 0.f, p1 = locBegin + 1.f, p2 = locBegin + 2.f, p3 = locBegin + 3.f;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    for (; li + 3 < loc1; li += 4) {
-        float q;
-        q = in(li) * win.trunc_lookup(p0);     a0 = a0 + q;
-        q = in(li + 1) * win.trunc_lookup(p1); a1 = a1 + q;
-        q = in(li + 2) * win.trunc_lookup(p2); a2 = a2 + q;
-        q = in(li + 3) * win.trunc_lookup(p3); a3 = a3 + q;
-        p0 = p0 + 4.0f; p1 = p1 + 4.0f; p2 = p2 + 4.0f; p3 = p3 + 4.0f;
-    }
-    float acc = (a0 + a2) + (a1 + a3);
-    float xx = static_cast<float>(li) - inPos;
-    for (; li < loc1; li++, xx += 1.0f) {
-        float q = in(li) * win.floor_lookup(xx);
-        acc = acc + q;
-    }
-    return acc;
-}
-
-// Unclipped output of an integer-ratio block: taps xt[0 .. nt) against the block's window values
-// w[0 .. nt) in the HOST order (fours, (s0+s2)+(s1+s3), then the leftover taps).  NT > 0 fixes
-// the tap count at compile time so the window values sit in registers (W is then a register
-// array), NT == 0 reads them from LDS.
-template <int NT, typename Wv>
-__device__ __forceinline__ float fir_one(const float *xt, const Wv &w, int nt)
-{
-    const int n = NT > 0 ? NT : nt;
-    const int ng = n / 4;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-#pragma unroll
-    for (int g = 0; g < (NT > 0 ? NT / 4 : ng); g++) {
-        float q;
-        q = xt[4 * g] * w[4 * g];         a0 = a0 + q;
-        q = xt[4 * g + 1] * w[4 * g + 1]; a1 = a1 + q;
-        q = xt[4 * g + 2] * w[4 * g + 2]; a2 = a2 + q;
-        q = xt[4 * g + 3] * w[4 * g + 3]; a3 = a3 + q;
-    }
-    float acc = (a0 + a2) + (a1 + a3);
-#pragma unroll
-    for (int t = 4 * ng; t < n; t++) { float q = xt[t] * w[t]; acc = acc + q; }
-    return acc;
-}
 
 // An integer-ratio block (every input position j*k exact): lane l takes outputs j = l + 64 i, so
 // consecutive lanes read taps k floats apart (no LDS bank conflicts for odd k) and store
@@ -142,7 +69,8 @@ template <bool DMIX>
 __global__ void __launch_bounds__(kThreads)
 resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__restrict__ dst, uint32_t dstStride,
                 const float *__restrict__ inRates, const float *__restrict__ outRates, const int *__restrict__ dims,
-                float wScale, float wCenter, int lobes, int lookupSize, const float *__restrict__ lutG, int normalize)
+                float wScale, float wCenter, int lobes, int lookupSize, const float *__restrict__ lutG, int normalize,
+                float inRateAll = 0.0f, float outRateAll = 0.0f)
 {
     extern __shared__ float lds[];
     float *lut = lds;
@@ -153,7 +81,7 @@ resample_kernel(const float *__restrict__ src, uint32_t srcStride, float *__rest
     const int C = max(dims[2 * b + 1], 1);
     const int L = min(dims[2 * b], static_cast<int>(srcStride / C));
     const int CO = DMIX ? 1 : C;                           // output channels
-    const float ir = inRates[b], orr = outRates[b];
+    const float ir = inRates ? inRates[b] : inRateAll, orr = outRates ? outRates[b] : outRateAll;
     const float *x = src + static_cast<size_t>(b) * srcStride;
     float *y = dst + static_cast<size_t>(b) * dstStride;
     const float dw = DMIX ? down_mix_weight(C, normalize) : 1.0f;
@@ -272,7 +200,39 @@ size_t resample_lds_bytes(int lookupSize)
     return sizeof(float) * ((lookupSize + 3) / 4 * 4 + kMaxTaps + (kThreads / kWave) * kSpan);
 }
 
+// Each clip's resampled length, ceil(samples * outRate / inRate) in FP32 as the resample kernel
+// computes its output count.
+__global__ void resampled_length_kernel(const int *__restrict__ dims, uint32_t srcStride, float inRate, float outRate,
+                                        int B, int *__restrict__ lengths)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int C = max(dims[2 * b + 1], 1);
+    const int L = max(min(dims[2 * b], static_cast<int>(srcStride / C)), 0);
+    const float lf = static_cast<float>(L) * outRate;
+    lengths[b] = static_cast<int>(ceilf(lf / inRate));
+}
+
 }  // namespace
+
+hipError_t down_mix_resample_launch(hipStream_t st, const float *src, uint32_t srcStride, float *dst,
+                                    uint32_t dstStride, unsigned B, const int *dims, float inRate, float outRate,
+                                    const RpptResamplingWindow &w, int normalize, int *lengths)
+{
+    dim3 grid(ceil_div(dstStride, kBlockOut * (kThreads / kWave)), B);
+    const size_t bytes = resample_lds_bytes(w.lookupSize);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_kernel<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+    hipLaunchKernelGGL(resample_kernel<true>, grid, dim3(kThreads), bytes, st, src, srcStride, dst, dstStride, nullptr,
+                       nullptr, dims, w.scale, w.center, w.lobes, w.lookupSize, w.lookupPinned, normalize, inRate,
+                       outRate);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(resampled_length_kernel, dim3(ceil_div(B, 256u)), dim3(256), 0, st, dims, srcStride, inRate,
+                       outRate, static_cast<int>(B), lengths);
+    return hipGetLastError();
+}
+
 }  // namespace rppa
 
 // Validation follows rppt_tensor_audio_augmentations.cpp:539-575.
