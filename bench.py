@@ -248,7 +248,7 @@ def main() -> int:
                     help="untimed replays before the warmup steps, so the timed steps run at the clocks of "
                          "sustained load (an idle GPU needs tens of milliseconds of work to reach them)")
     ap.add_argument("--workload", type=int, default=1, choices=[0, 1, 2])
-    ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "bf16", "fp32"])
+    ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x6", "bf16x3", "bf16", "fp32"])
     ap.add_argument("--no-side", action="store_true", help="skip side measurements (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -435,6 +435,17 @@ def main() -> int:
                                          "p99_9": float(f"{np.percentile(rel, 99.9):.3e}"),
                                          "outputs": int(rel.size)}
         side["accuracy_sample_clips"] = int(len(idx))
+        # the three-term bf16 split (six products) on the same batch: throughput of its fused
+        # graph, and its max |dB error| on the same clips
+        g6 = fused_graph("bf16x6")
+        n6 = max(3, a.steps // 4)
+        w6, _ = time_graph(g6, n6, 1)
+        o6 = out[torch.from_numpy(idx).cuda()].cpu().numpy()
+        side["bf16x6_fused"] = {"audio_s_per_s": round(audio_s * n6 / w6, 1),
+                                "max_db_err_vs_fp64": round(max_db_err(o6, ref64, fr64), 6),
+                                "tiles": {"frames_per_tile": tuned["bf16x6"][0][0],
+                                          "blocks_per_cu": tuned["bf16x6"][0][1]}}
+        del g6
         # CPU baseline: the oracle's RPP-HOST-equivalent FP32 chain on the host cores, over the
         # whole batch: inputs resident and every output preallocated, only the four C calls timed
         # (pre-emphasis, spectrogram, mel, to_decibels), 2 warm-up runs, then the median of at
@@ -485,7 +496,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp16x3": "fp16 hi+lo split, fp32 accumulate", "bf16x3": "bf16 hi+lo split, fp32 accumulate",
-                  "bf16": "bf16, fp32 accumulate", "fp32": "fp32"}[a.precision],
+                  "bf16x6": "bf16 hi+mid+lo split, fp32 accumulate", "bf16": "bf16, fp32 accumulate",
+                  "fp32": "fp32"}[a.precision],
         "data": "synthetic (seeded SplitMix64)",
         "config": {"workload": {0: "batch 8 mono 16 kHz 1-3 s", 1: "batch 512 ragged mono 16 kHz 0.5-20 s",
                                 2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],

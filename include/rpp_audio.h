@@ -256,7 +256,9 @@ typedef enum {
     RPPT_LOGMEL_FP32 = 0,      /* FP32 vector FMA throughout */
     RPPT_LOGMEL_BF16X3 = 1,    /* MFMA bf16, split hi+lo: hi*hi + hi*lo + lo*hi, FP32 accumulate */
     RPPT_LOGMEL_FP16X3 = 2,    /* MFMA fp16, split hi+lo with power-of-two scaling, FP32 accumulate */
-    RPPT_LOGMEL_BF16 = 3       /* MFMA bf16, single product (fast, reported error only) */
+    RPPT_LOGMEL_BF16 = 3,      /* MFMA bf16, single product (fast, reported error only) */
+    RPPT_LOGMEL_BF16X6 = 4     /* MFMA bf16, three-term split hi+mid+lo of both DFT operands: the six
+                                  products down to 2^-16 relative, FP32 accumulate */
 } RpptLogMelPrecision;
 
 /* Parameters of the whole log-mel chain, the union of the four reference calls' arguments. */

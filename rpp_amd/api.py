@@ -30,7 +30,7 @@ U8, F32, F16, I8, I16 = range(5)
 NCHW, NHWC, NCDHW, NDHWC, NHW, NFT, NTF = range(7)
 BORDER = {"zero": 0, "clamp": 1, "reflect": 2}
 MEL = {"slaney": 0, "htk": 1}
-PRECISION = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "bf16": 3}
+PRECISION = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "bf16": 3, "bf16x6": 4}
 
 
 class RppError(RuntimeError):

@@ -50,9 +50,11 @@ struct DftTables {
     // 0..N/4-1, [N/4, N/2) -w sin of the same bins.  Element (j, c) sits at
     // ((j / 8) * (N/2) + c) * 8 + j % 8 -- the operand order of a 32x32x16 MFMA fragment.
     uint16_t *foldHi = nullptr, *foldLo = nullptr;
+    uint16_t *foldLo2 = nullptr;   // third term of the bf16 three-term split (splitKind 2), else null
     float *wHalf = nullptr;        // [W/2] symmetrised window half, w_j = w[W/2 + j]
     int Jp = 0;                    // W/2 padded to a multiple of 16 (zero rows)
-    int splitKind = -1;            // format of foldHi/foldLo (-1 none, 0 bf16, 1 fp16 scaled by 4096)
+    int splitKind = -1;            // format of foldHi/foldLo (-1 none, 0 bf16, 1 fp16 scaled by 4096,
+                                   // 2 bf16 hi/mid/lo in foldHi/foldLo/foldLo2)
 };
 
 // Mel tables for one parameter set, built on the GPU (tables.hip).

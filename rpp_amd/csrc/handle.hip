@@ -89,6 +89,7 @@ RPPA_EXPORT rppStatus_t rppDestroy(rppHandle_t handle, RppBackend backend)
         (void)hipFree(kv.second.basis);
         if (kv.second.foldHi) (void)hipFree(kv.second.foldHi);
         if (kv.second.foldLo) (void)hipFree(kv.second.foldLo);
+        if (kv.second.foldLo2) (void)hipFree(kv.second.foldLo2);
         if (kv.second.wHalf) (void)hipFree(kv.second.wHalf);
     }
     for (auto &kv : handle->fft) (void)hipFree(kv.second.mem);

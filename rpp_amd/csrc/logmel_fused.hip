@@ -67,6 +67,7 @@ struct LogMelArgs {
     int cols;
     // split basis
     const uint16_t *bHi, *bLo;      // folded split basis (DftTables::foldHi/foldLo)
+    const uint16_t *bLo2;           // its third term (bf16 three-term split)
     const float *wHalf;
     int Wp;                         // folded contraction length, padded to 16
     float basisInv;                 // 1 / the scale of the fp16 basis (an exact power of two)
@@ -437,7 +438,9 @@ logmel_fp32_kernel(LogMelArgs a, int segFloats, int bFloats, int pwFloats, int a
 }
 
 // ------------------------------------------------------------------------- MFMA kernel
-enum { kPrecBF16x3 = 1, kPrecFP16x3 = 2, kPrecBF16 = 3 };
+enum { kPrecBF16x3 = 1, kPrecFP16x3 = 2, kPrecBF16 = 3, kPrecBF16x6 = 4 };
+// terms of each split DFT operand: hi, lo (and mid between them for the three-term bf16 split)
+constexpr int split_terms(int prec) { return prec == kPrecBF16x6 ? 3 : 2; }
 
 __device__ __forceinline__ void split_bf16(const float *v, bf16x8 &hi, bf16x8 &lo)
 {
@@ -495,6 +498,30 @@ __device__ __forceinline__ void mfma3(f32x16 &acc, const u16x8 &ah, const u16x8 
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
         }
     }
+}
+
+// acc += A * B over three-term bf16 operands (hi, mid, lo: each term about 2^-8 of the one before):
+// the six products down to 2^-16 relative -- hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid.
+__device__ __forceinline__ void mfma6(f32x16 &acc, const u16x8 (&a)[3], const u16x8 (&b)[3])
+{
+    const bf16x8 a0 = __builtin_bit_cast(bf16x8, a[0]), a1 = __builtin_bit_cast(bf16x8, a[1]),
+                 a2 = __builtin_bit_cast(bf16x8, a[2]);
+    const bf16x8 b0 = __builtin_bit_cast(bf16x8, b[0]), b1 = __builtin_bit_cast(bf16x8, b[1]),
+                 b2 = __builtin_bit_cast(bf16x8, b[2]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
+}
+
+// The products of one split operand pair for any precision (NT terms per operand).
+template <int PREC, int NT>
+__device__ __forceinline__ void mfma_split(f32x16 &acc, const u16x8 (&a)[NT], const u16x8 (&b)[NT])
+{
+    if constexpr (PREC == kPrecBF16x6) mfma6(acc, a, b);
+    else mfma3<PREC>(acc, a[0], a[1], b[0], b[1]);
 }
 
 // Largest b with tileStart[b] <= tile, by a 64-ary search over the wave (two rounds of loads for a
@@ -730,10 +757,36 @@ __device__ __forceinline__ void split2(float v0, float v1, float s, uint32_t &hi
     }
 }
 
+// Two-element three-term bf16 split: hi = bf16(v), mid = bf16(v - hi), lo = bf16(v - hi - mid)
+// (each difference exact in FP32), packed as split2 packs them.
+__device__ __forceinline__ void split3_bf16(float v0, float v1, uint32_t &hi, uint32_t &mid, uint32_t &lo)
+{
+    const __bf16 h0 = static_cast<__bf16>(v0), h1 = static_cast<__bf16>(v1);
+    const float r0 = v0 - static_cast<float>(h0), r1 = v1 - static_cast<float>(h1);
+    const __bf16 m0 = static_cast<__bf16>(r0), m1 = static_cast<__bf16>(r1);
+    const __bf16 l0 = static_cast<__bf16>(r0 - static_cast<float>(m0));
+    const __bf16 l1 = static_cast<__bf16>(r1 - static_cast<float>(m1));
+    auto pack = [](__bf16 x, __bf16 y) {
+        return __builtin_bit_cast(uint16_t, x) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, y)) << 16);
+    };
+    hi = pack(h0, h1);
+    mid = pack(m0, m1);
+    lo = pack(l0, l1);
+}
+
+// The split of two elements into the NT terms of precision PREC, term t of the pair in w[t].
+template <int PREC>
+__device__ __forceinline__ void split_terms2(float v0, float v1, float s, uint32_t (&w)[split_terms(PREC)])
+{
+    if constexpr (PREC == kPrecBF16x6) split3_bf16(v0, v1, w[0], w[1], w[2]);
+    else split2<PREC>(v0, v1, s, w[0], w[1]);
+}
+
 // Folded, split DFT operand of SPB 16-row contraction steps starting at step s, built into buffer
 // `buf` of the slice ring while the waves run the previous steps' products: slot
-// ((buf * 4 + arr) * 2SPB + g) * FT + f holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7,
-// for arr = 0 e-hi, 1 e-lo, 2 o-hi, 3 o-lo -- the lane order of a 32x32x16 MFMA fragment.  A work
+// ((buf * 2NT + arr) * 2SPB + g) * FT + f holds 8 halves (16 bytes) of frame f, rows 16s + 8g .. +7,
+// for arr = t the e operand's term t, NT + t the o operand's (NT = 2: hi, lo; 3: hi, mid, lo) --
+// the lane order of a 32x32x16 MFMA fragment.  A work
 // unit forms four rows of one frame (one 8-byte half of a slot).  q4 gathers the thread's share of
 // bin N/4: re += e_j w_j c_j, im += o_j w_j s_j with (c_j, s_j) = sqrt(2) (cos, sin)(pi (2j+1)/4),
 // i.e. c = +1 -1 -1 +1 and s = +1 +1 -1 -1 over j mod 4 (the sqrt(2)/2 factors are applied to the
@@ -774,16 +827,18 @@ __device__ __forceinline__ void prep_step(u16x8 *ring, const float *seg, const f
             e3 = up.w + dn.x; o3 = up.w - dn.x;
             q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, *reinterpret_cast<const float4 *>(&wHalf[j]));
         }
-        uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
-        split2<PREC>(e0, e1, segScale, eh0, el0);
-        split2<PREC>(e2, e3, segScale, eh1, el1);
-        split2<PREC>(o0, o1, segScale, oh0, ol0);
-        split2<PREC>(o2, o3, segScale, oh1, ol1);
-        const int slot = (buf * 4 * GB + (qd >> 1)) * FT + f, hh = qd & 1;
-        dst[(slot + 0 * GB * FT) * 2 + hh] = make_uint2(eh0, eh1);
-        dst[(slot + 1 * GB * FT) * 2 + hh] = make_uint2(el0, el1);
-        dst[(slot + 2 * GB * FT) * 2 + hh] = make_uint2(oh0, oh1);
-        dst[(slot + 3 * GB * FT) * 2 + hh] = make_uint2(ol0, ol1);
+        constexpr int NT = split_terms(PREC);
+        uint32_t e01[NT], e23[NT], o01[NT], o23[NT];
+        split_terms2<PREC>(e0, e1, segScale, e01);
+        split_terms2<PREC>(e2, e3, segScale, e23);
+        split_terms2<PREC>(o0, o1, segScale, o01);
+        split_terms2<PREC>(o2, o3, segScale, o23);
+        const int slot = (buf * 2 * NT * GB + (qd >> 1)) * FT + f, hh = qd & 1;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            dst[(slot + t * GB * FT) * 2 + hh] = make_uint2(e01[t], e23[t]);
+            dst[(slot + (NT + t) * GB * FT) * 2 + hh] = make_uint2(o01[t], o23[t]);
+        }
     }
 }
 
@@ -821,17 +876,19 @@ __device__ __forceinline__ void prep_finish(u16x8 *ring, const PrepIn &p, int u,
     const float e2 = up.z + dn.y, o2 = up.z - dn.y;
     const float e3 = up.w + dn.x, o3 = up.w - dn.x;
     q4_accumulate(q4, e0, e1, e2, e3, o0, o1, o2, o3, w);   // zero w past W/2 leaves q4 unchanged
-    uint32_t eh0, el0, eh1, el1, oh0, ol0, oh1, ol1;
-    split2<PREC>(e0, e1, segScale, eh0, el0);
-    split2<PREC>(e2, e3, segScale, eh1, el1);
-    split2<PREC>(o0, o1, segScale, oh0, ol0);
-    split2<PREC>(o2, o3, segScale, oh1, ol1);
+    constexpr int NT = split_terms(PREC);
+    uint32_t e01[NT], e23[NT], o01[NT], o23[NT];
+    split_terms2<PREC>(e0, e1, segScale, e01);
+    split_terms2<PREC>(e2, e3, segScale, e23);
+    split_terms2<PREC>(o0, o1, segScale, o01);
+    split_terms2<PREC>(o2, o3, segScale, o23);
     uint2 *dst = reinterpret_cast<uint2 *>(ring);
-    const int slot = (buf * 4 * GB + (qd >> 1)) * FT + f, hh = qd & 1;
-    dst[(slot + 0 * GB * FT) * 2 + hh] = make_uint2(eh0, eh1);
-    dst[(slot + 1 * GB * FT) * 2 + hh] = make_uint2(el0, el1);
-    dst[(slot + 2 * GB * FT) * 2 + hh] = make_uint2(oh0, oh1);
-    dst[(slot + 3 * GB * FT) * 2 + hh] = make_uint2(ol0, ol1);
+    const int slot = (buf * 2 * NT * GB + (qd >> 1)) * FT + f, hh = qd & 1;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        dst[(slot + t * GB * FT) * 2 + hh] = make_uint2(e01[t], e23[t]);
+        dst[(slot + (NT + t) * GB * FT) * 2 + hh] = make_uint2(o01[t], o23[t]);
+    }
 }
 
 // Scheduling hint for the region before it: NMFMA products, each followed by up to three vector
@@ -979,8 +1036,10 @@ __device__ int plan_tiles_block(const LogMelArgs &a, int FT, int *ts, int *ws)
 // hi/lo planes; banded mel on MFMA, each weight fragment applied to every frame group of the
 // tile; dB out.
 template <int PREC, int FT, int CTM, int RTW>
-__global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a, int segFloats, int wOff)
+__global__ void __launch_bounds__(kBlockMfma, PREC == kPrecBF16x6 ? 1 : 2) logmel_mfma_kernel(LogMelArgs a, int segFloats,
+                                                                                             int wOff)
 {
+    constexpr int NT = split_terms(PREC);   // terms per split operand
     constexpr int NRT = FT / 32;
     constexpr int U = CTM * RTW;                         // units per wave
     constexpr int NW = 4;
@@ -993,7 +1052,7 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
     __shared__ int melSeqLen[NW];
     __shared__ uint16_t melBand[16];                     // filter tile's band: first step << 8 | end step
     float *seg = lds;                                    // the power planes alias segment + ring after the DFT
-    u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][4][2 kSPB][FT] operand slices
+    u16x8 *ring = reinterpret_cast<u16x8 *>(lds + segFloats);   // [2][2 NT][2 kSPB][FT] operand slices
     const int pws2 = a.KP + 8;                           // power plane row stride (halves; 16-byte skew)
     const int span = (FT - 1) * a.H + a.W;
     const int halfN = a.N / 2;
@@ -1125,11 +1184,12 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         // products cover the latency), which keeps the register budget.  A tile index past the
         // last (four waves over six tiles) repeats the last tile; its power is never stored.
         constexpr int NB = CTM == 1 ? 2 : 1;
-        u16x8 bc[NB][CTM][2], bs[NB][CTM][2];
+        u16x8 bc[NB][CTM][NT], bs[NB][CTM][NT];
         const int NC = a.N / 2;
         // 32-bit byte offsets from the table bases (the tables stay far below 4 GB), so the loads
         // take a scalar base and a vector offset
         const char *bHi = reinterpret_cast<const char *>(a.bHi), *bLo = reinterpret_cast<const char *>(a.bLo);
+        const char *bLo2 = reinterpret_cast<const char *>(a.bLo2);
         auto loadB = [&](int buf, int ct, int s) {
             const uint32_t c = min(ct0 + ct, nct - 1) * 32 + (lane & 31);
             const uint32_t oc = (static_cast<uint32_t>(2 * s + (lane >> 5)) * NC + c) * 16u;
@@ -1139,6 +1199,10 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
             if (PREC != kPrecBF16) {
                 bc[buf][ct][1] = *reinterpret_cast<const u16x8 *>(bLo + oc);
                 bs[buf][ct][1] = *reinterpret_cast<const u16x8 *>(bLo + os);
+            }
+            if constexpr (NT == 3) {
+                bc[buf][ct][NT - 1] = *reinterpret_cast<const u16x8 *>(bLo2 + oc);
+                bs[buf][ct][NT - 1] = *reinterpret_cast<const u16x8 *>(bLo2 + os);
             }
         };
         float2 q4 = make_float2(0.0f, 0.0f);
@@ -1151,14 +1215,15 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
         // refills of step sn follow each column tile's products
         auto step_products = [&](int rb, int h, int buf, int sn, bool refill) {
             __builtin_amdgcn_sched_barrier(0);
-            u16x8 ah[RTW][2], oh[RTW][2];
+            u16x8 ah[RTW][NT], oh[RTW][NT];
 #pragma unroll
             for (int r = 0; r < RTW; r++) {
-                const int slot = (rb * 4 * GB + 2 * h + (lane >> 5)) * FT + (rt0 + r) * 32 + (lane & 31);
-                ah[r][0] = ring[slot + 0 * GB * FT];
-                ah[r][1] = ring[slot + 1 * GB * FT];
-                oh[r][0] = ring[slot + 2 * GB * FT];
-                oh[r][1] = ring[slot + 3 * GB * FT];
+                const int slot = (rb * 2 * NT * GB + 2 * h + (lane >> 5)) * FT + (rt0 + r) * 32 + (lane & 31);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    ah[r][t] = ring[slot + t * GB * FT];
+                    oh[r][t] = ring[slot + (NT + t) * GB * FT];
+                }
             }
             // the other buffer's refill goes out here, a whole step ahead of its use
             if (NB == 2 && refill) loadB(buf ^ 1, 0, sn);
@@ -1170,22 +1235,24 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
                 // X^T = basis (A: rows = bins) x frames (B: columns = frames); bin k from the
                 // columns as stored, bin N/2 - k from the same columns with odd rows negated
 #pragma unroll
-                for (int r = 0; r < RTW; r++)
-                    mfma3<PREC>(acc[r][ct][0], bc[buf][ct][0], bc[buf][ct][1], ah[r][0], ah[r][1]);
+                for (int r = 0; r < RTW; r++) mfma_split<PREC, NT>(acc[r][ct][0], bc[buf][ct], ah[r]);
 #pragma unroll
-                for (int r = 0; r < RTW; r++)
-                    mfma3<PREC>(acc[r][ct][1], bs[buf][ct][0], bs[buf][ct][1], oh[r][0], oh[r][1]);
-                const u16x8 mc0 = flip_odd(bc[buf][ct][0]), mc1 = flip_odd(bc[buf][ct][1]);
-                const u16x8 ms0 = flip_odd(bs[buf][ct][0]), ms1 = flip_odd(bs[buf][ct][1]);
+                for (int r = 0; r < RTW; r++) mfma_split<PREC, NT>(acc[r][ct][1], bs[buf][ct], oh[r]);
+                u16x8 mc[NT], ms[NT];
 #pragma unroll
-                for (int r = 0; r < RTW; r++) mfma3<PREC>(acc[r][ct][2], ms0, ms1, ah[r][0], ah[r][1]);
+                for (int t = 0; t < NT; t++) {
+                    mc[t] = flip_odd(bc[buf][ct][t]);
+                    ms[t] = flip_odd(bs[buf][ct][t]);
+                }
 #pragma unroll
-                for (int r = 0; r < RTW; r++) mfma3<PREC>(acc[r][ct][3], mc0, mc1, oh[r][0], oh[r][1]);
+                for (int r = 0; r < RTW; r++) mfma_split<PREC, NT>(acc[r][ct][2], ms, ah[r]);
+#pragma unroll
+                for (int r = 0; r < RTW; r++) mfma_split<PREC, NT>(acc[r][ct][3], mc, oh[r]);
                 if (NB == 1 && refill) loadB(0, ct, sn);
             }
         };
         constexpr int UPT = 2 * GB * FT / 256;            // operand units per thread and step pair
-        constexpr int MPS = (PREC == kPrecBF16 ? 4 : 12) * U;   // products per step and wave
+        constexpr int MPS = (PREC == kPrecBF16 ? 4 : (PREC == kPrecBF16x6 ? 24 : 12)) * U;   // products per step and wave
         static_assert(kSPB == 2, "the loop below runs step pairs");
         // full pairs: straight-line, the next pair's operand build beside the products (unit i of
         // it loaded before step s0 + i and finished after it); the last pair's build and refills
@@ -1370,6 +1437,18 @@ __global__ void __launch_bounds__(kBlockMfma, 2) logmel_mfma_kernel(LogMelArgs a
 }
 
 // ------------------------------------------------------------------------- dispatch
+// Format of the folded basis a precision reads (DftTables::splitKind): none for FP32, fp16 scaled
+// hi/lo for FP16X3, bf16 hi/mid/lo for BF16X6, bf16 hi/lo otherwise.
+int split_kind(RpptLogMelPrecision precision)
+{
+    switch (precision) {
+    case RPPT_LOGMEL_FP32: return -1;
+    case RPPT_LOGMEL_FP16X3: return 1;
+    case RPPT_LOGMEL_BF16X6: return 2;
+    default: return 0;
+    }
+}
+
 RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
 {
     if (!p->multiplier) return RPP_ERROR_ZERO_DIVISION;
@@ -1379,7 +1458,7 @@ RppStatus check_params(const RpptLogMelParams *p, RpptLogMelPrecision precision)
     if (p->maxFreq < 0 || p->maxFreq > p->sampleRate / 2 || p->minFreq < 0 || p->minFreq > p->sampleRate / 2)
         return RPP_ERROR_INVALID_ARGUMENTS;
     if (p->referenceMagnitude < 0.0f) return RPP_ERROR_INVALID_ARGUMENTS;
-    if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16) return RPP_ERROR_INVALID_ARGUMENTS;
+    if (precision < RPPT_LOGMEL_FP32 || precision > RPPT_LOGMEL_BF16X6) return RPP_ERROR_INVALID_ARGUMENTS;
     // the folded MFMA path: nfft 256, 512, 768 or 1024, window length a multiple of 16 (its half a
     // multiple of 8), hop a multiple of 8, at most 256 mel filters
     if (precision != RPPT_LOGMEL_FP32 &&
@@ -1420,7 +1499,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     const int K = nfft / 2 + 1;
-    const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
+    const int splitKind = split_kind(precision);
     const DftTables *dt = mem ? mem->dt : get_dft_tables(h, nfft, p->windowLength, nullptr, splitKind);
     const MelTables *mt = mem ? mem->mt
                               : get_mel_tables(h, K, p->sampleRate, p->maxFreq, p->minFreq,
@@ -1466,7 +1545,9 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     // the MFMA kernel plans in each block (no planner launch, one dependent dispatch less) when the
     // per-tile records are bounded, the clip count fits the segment region and no per-clip maximum
     // needs initialising; each block's records take kPad slots, 256-byte aligned
-    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU : (mfma ? 2 : 1);
+    // (the three-term bf16 kernel keeps one block per CU: its operands take the whole register file)
+    const int bpc = h->tiles.blocksPerCU > 0 ? h->tiles.blocksPerCU
+                                             : (mfma && precision != RPPT_LOGMEL_BF16X6 ? 2 : 1);
     const unsigned grid = static_cast<unsigned>(h->numCU * bpc);
     bool selfPlan = recBytes > 0 && !clipRef && B <= 4096;
     int kPad = 0;
@@ -1523,6 +1604,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     a.cols = dt->cols;
     a.bHi = dt->foldHi;
     a.bLo = dt->foldLo;
+    a.bLo2 = dt->foldLo2;
     a.wHalf = dt->wHalf;
     a.Wp = dt->Jp;
     a.basisInv = splitKind == 1 ? 1.0f / 4096.0f : 1.0f;
@@ -1581,7 +1663,9 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         const int segFloats = (seg_pos(span - 1, a.H) + 1 + 8 + 3) / 4 * 4;
         const size_t pwBytes = static_cast<size_t>(4) * FT * (a.KP + 8);   // bf16 hi + lo planes
         const size_t staticBytes = sizeof(float) * (4 + 2 * 4 * 64 + 4) + sizeof(uint16_t) * (4 * kMelSeq + 16);
-        const size_t alias = (std::max(sizeof(float) * segFloats + static_cast<size_t>(256) * kSPB * FT, pwBytes) + 15) / 16 * 16;
+        const size_t ringBytes = static_cast<size_t>(128) * split_terms(precision == RPPT_LOGMEL_BF16X6 ? kPrecBF16x6 : 0) *
+                                 kSPB * FT;   // [2][2 NT][2 kSPB][FT] 16-byte slices
+        const size_t alias = (std::max(sizeof(float) * segFloats + ringBytes, pwBytes) + 15) / 16 * 16;
         const int wOff = static_cast<int>(alias / sizeof(float));
         const size_t bytes = alias + sizeof(float) * ((a.W / 2 + 4 + 3) / 4 * 4);   // window half + 4 zeros
         if (bytes + staticBytes > static_cast<size_t>(h->maxLdsPerBlock)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
@@ -1603,6 +1687,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
         } while (0)
         if (precision == RPPT_LOGMEL_BF16X3) RPPA_LAUNCH(kPrecBF16x3);
         else if (precision == RPPT_LOGMEL_FP16X3) RPPA_LAUNCH(kPrecFP16x3);
+        else if (precision == RPPT_LOGMEL_BF16X6) RPPA_LAUNCH(kPrecBF16x6);
         else RPPA_LAUNCH(kPrecBF16);
 #undef RPPA_LAUNCH
 #undef RPPA_LAUNCH_ONE
@@ -1858,6 +1943,7 @@ size_t layout_tables(char *base, int nfft, int W, int splitKind, bool wantBasis,
         const size_t n = dft_fold_elems(nfft, W);
         dt.foldHi = reinterpret_cast<uint16_t *>(take(n * 2));
         dt.foldLo = reinterpret_cast<uint16_t *>(take(n * 2));
+        if (splitKind == 2) dt.foldLo2 = reinterpret_cast<uint16_t *>(take(n * 2));
         dt.wHalf = reinterpret_cast<float *>(take(sizeof(float) * (W / 2)));
     }
     mt.weightsDown = reinterpret_cast<float *>(take(sizeof(float) * numBins));
@@ -1930,7 +2016,7 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr
     const bool mfma = precision != RPPT_LOGMEL_FP32;
     const int nfft = p->nfft ? p->nfft : p->windowLength;
     const int K = nfft / 2 + 1;
-    const int splitKind = !mfma ? -1 : (precision == RPPT_LOGMEL_FP16X3 ? 1 : 0);
+    const int splitKind = split_kind(precision);
     if (splitKind >= 0 && ((nfft & 1) || (p->windowLength & 1))) return RPP_ERROR_NOT_IMPLEMENTED;
     char keyb[200];
     snprintf(keyb, sizeof(keyb), "%d/%d/%d/%d/%a/%a/%a/%d/%d/%d", nfft, p->windowLength, splitKind, K, p->sampleRate,

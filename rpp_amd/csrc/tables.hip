@@ -72,8 +72,8 @@ __global__ void dft_basis_kernel(int N, int W, int cols, const float *__restrict
 // (mean of w[W/2+j] and w[W/2-1-j], equal for RPP's Hann up to FP32 rounding).  Angles are
 // reduced exactly: k (2j+1) mod 2N over 2N.
 __global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict__ win, uint16_t *__restrict__ hi,
-                                  uint16_t *__restrict__ lo, float *__restrict__ wHalf, int splitKind,
-                                  float splitScale)
+                                  uint16_t *__restrict__ lo, uint16_t *__restrict__ lo2, float *__restrict__ wHalf,
+                                  int splitKind, float splitScale)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -91,8 +91,14 @@ __global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict_
         v = isSin ? -w * sin(th) : w * cos(th);
     }
     const double s = v * splitScale;
-    uint16_t h, l;
-    if (splitKind == 0) {
+    uint16_t h, l, l2 = 0;
+    if (splitKind == 2) {
+        // three bf16 terms, each the rounded remainder of the ones before (remainders in FP64)
+        h = f32_to_bf16_rne(static_cast<float>(s));
+        const double r1 = s - bf16_to_f32(h);
+        l = f32_to_bf16_rne(static_cast<float>(r1));
+        l2 = f32_to_bf16_rne(static_cast<float>(r1 - bf16_to_f32(l)));
+    } else if (splitKind == 0) {
         h = f32_to_bf16_rne(static_cast<float>(s));
         l = f32_to_bf16_rne(static_cast<float>(s - bf16_to_f32(h)));
     } else {
@@ -102,6 +108,7 @@ __global__ void fold_basis_kernel(int N, int W, int Jp, const float *__restrict_
     const size_t o = (static_cast<size_t>(j / 8) * NC + c) * 8 + (j % 8);
     hi[o] = h;
     lo[o] = l;
+    if (splitKind == 2) lo2[o] = l2;
 }
 
 // bf16 hi/lo split of the dense mel matrix in MFMA operand order (see common.hpp).
@@ -255,7 +262,7 @@ hipError_t build_dft_tables_at(hipStream_t st, int nfft, int W, const float *win
         t.splitKind = splitKind;
         const float splitScale = (splitKind == 1) ? 4096.0f : 1.0f;
         hipLaunchKernelGGL(fold_basis_kernel, dim3(ceil_div(nfft / 2, 256), t.Jp), dim3(256), 0, st, nfft, W, t.Jp,
-                           win, t.foldHi, t.foldLo, t.wHalf, splitKind, splitScale);
+                           win, t.foldHi, t.foldLo, t.foldLo2, t.wHalf, splitKind, splitScale);
     }
     return hipGetLastError();
 }
@@ -293,9 +300,12 @@ const DftTables *get_dft_tables(rppHandle *h, int nfft, int W, const float *wind
         return nullptr;
     if (splitKind >= 0) {
         if (t.foldHi) { (void)hipFree(t.foldHi); (void)hipFree(t.foldLo); (void)hipFree(t.wHalf); }
+        if (t.foldLo2) (void)hipFree(t.foldLo2);
+        t.foldLo2 = nullptr;
         const size_t n = dft_fold_elems(nfft, W);
         if (hipMalloc(&t.foldHi, n * 2) != hipSuccess || hipMalloc(&t.foldLo, n * 2) != hipSuccess ||
-            hipMalloc(&t.wHalf, sizeof(float) * (W / 2)) != hipSuccess)
+            hipMalloc(&t.wHalf, sizeof(float) * (W / 2)) != hipSuccess ||
+            (splitKind == 2 && hipMalloc(&t.foldLo2, n * 2) != hipSuccess))
             return nullptr;
     }
     if (build_dft_tables_at(h->stream, nfft, W, window, splitKind, fresh, win, t) != hipSuccess) return nullptr;

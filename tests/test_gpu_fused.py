@@ -4,6 +4,7 @@
               oracle (the RPP HOST restatement) 1e-4 relative in mel power with a floor at 1e-6 of
               the frame's loudest band (helpers.fp32_bar)
   fp16x3      max |dB - FP64 oracle dB| <= 0.05 dB over every output (values clamp at -80 dB)
+  bf16x6      (three-term bf16 split, six products) max |dB - FP64 oracle dB| <= 0.05 dB, like fp16x3
   bf16x3      meets 0.05 dB on workload 0 only (a two-term bf16 split keeps ~16 mantissa bits); on
               workloads 1/2 it is held to its recorded error plus a margin (BF16X3_BOUND_DB), so a
               regression fails
@@ -53,7 +54,7 @@ def test_fused_fp32_equals_chain(gpu, handle):
         assert np.array_equal(out[b, :, :F], d[b, :, :F]), b
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "bf16x6"])
 def test_fused_mixed_workload0_vs_fp64(gpu, handle, orc, precision):
     bt = synth.workload0()
     ref, _, frames = orc.logmel_f64(bt.x, bt.lengths, nthreads=8)
@@ -64,8 +65,8 @@ def test_fused_mixed_workload0_vs_fp64(gpu, handle, orc, precision):
 
 
 def check_against_oracles(orc, out, x, L, precision, name, **kw):
-    """FP32 mode: fp32_bar against the FP32 oracle; fp16x3: <= 0.05 dB from the FP64 oracle;
-    bf16x3: reported.  All figures are printed."""
+    """FP32 mode: fp32_bar against the FP32 oracle; fp16x3 and bf16x6: <= 0.05 dB from the FP64
+    oracle; bf16x3: its recorded bound.  All figures are printed."""
     ref64, _, frames = orc.logmel_f64(x, L, nthreads=8, **kw)
     ref32, _ = orc.logmel_f32(x, L, nthreads=8, **kw)
     e64 = max_db_err(out, ref64, frames)
@@ -76,21 +77,21 @@ def check_against_oracles(orc, out, x, L, precision, name, **kw):
           f"vs FP32 oracle = {e32:.3e} (fp32 bar {bar:.3f})")
     if precision == "fp32":
         assert bar <= 1.0
-    elif precision == "fp16x3":
+    elif precision in ("fp16x3", "bf16x6"):
         assert e64 <= DB_TOL_MIXED
     elif precision == "bf16x3":
         assert e64 <= BF16X3_BOUND_DB[name]
     return e64
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3", "bf16x6"])
 def test_fused_workload1_subset(gpu, handle, orc, precision):
     bt = synth.workload1(batch=24)
     out, fr = run_fused(gpu, handle, bt.x, bt.lengths, bt.params, precision)
     check_against_oracles(orc, out, bt.x, bt.lengths, precision, "workload1[:24]")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3", "bf16x6"])
 def test_fused_workload2_subset(gpu, handle, orc, precision):
     bt = synth.workload2(batch=6)
     p = bt.params
@@ -100,7 +101,7 @@ def test_fused_workload2_subset(gpu, handle, orc, precision):
 
 
 @pytest.mark.parametrize("ft", [64, 32])
-@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "bf16x6"])
 def test_fused_dense_tile_shapes_workload1(gpu, orc, precision, ft):
     """The dense MFMA kernel at both of its nfft-512 tile shapes (64 frames, each wave two 32-frame
     row tiles of one column tile; 32 frames, one unit per wave), chosen through the handle's tiles
@@ -213,10 +214,11 @@ def test_fused_grid_and_alignment_invariance(gpu, orc, precision):
 
 @pytest.mark.parametrize("nfft,W,H,nmel,sr", [(256, 256, 128, 40, 8000.0), (768, 768, 192, 96, 24000.0),
                                               (1024, 800, 320, 128, 32000.0), (512, 512, 256, 64, 16000.0)])
-def test_fused_mfma_shapes(gpu, handle, orc, nfft, W, H, nmel, sr):
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x6"])
+def test_fused_mfma_shapes(gpu, handle, orc, nfft, W, H, nmel, sr, precision):
     """Every MFMA kernel instance (1-4 column tiles per wave, 64- and 32-frame tiles) against the
     FP64 oracle at 0.05 dB, on ragged clips including edge lengths, with HTK and unnormalised
-    filters on one of them."""
+    filters on one of them, for both split precisions that meet that bound."""
     rng = np.random.default_rng(nfft + W)
     L = np.array([1, W // 2, W, W + H, 7 * H + 3, 40 * H, 123 * H + 17, 300 * H + 5], np.int32)
     x = np.zeros((len(L), int(L.max()) + 8), np.float32)
@@ -229,12 +231,12 @@ def test_fused_mfma_shapes(gpu, handle, orc, nfft, W, H, nmel, sr):
     out = torch.full((len(L), nmel, int(frames.max())), float("nan"), device="cuda")
     p = gpu.logmel_params(nfft=nfft, window_length=W, window_step=H, sample_rate=sr, min_freq=0.0, max_freq=sr / 2,
                           num_filter=nmel, formula="htk" if htk else "slaney", normalize=not htk)
-    gpu.logmel_fused(handle, dev(x), out, dev(L), p, "fp16x3")
+    gpu.logmel_fused(handle, dev(x), out, dev(L), p, precision)
     torch.cuda.synchronize()
     ref, _, fr = orc.logmel_f64(x, L, nfft=nfft, W=W, H=H, nmel=nmel, sample_rate=sr, fmax=sr / 2,
                                 formula="htk" if htk else "slaney", normalize=not htk, nthreads=8)
     err = max_db_err(out.cpu().numpy(), ref, fr)
-    print(f"fp16x3 nfft {nfft} W {W} H {H} M {nmel}: max |dB err| vs FP64 = {err:.3e}")
+    print(f"{precision} nfft {nfft} W {W} H {H} M {nmel}: max |dB err| vs FP64 = {err:.3e}")
     assert err <= DB_TOL_MIXED
 
 
