@@ -222,20 +222,60 @@ def serve_measurement(api, synth, params, precision, seconds, rate=4000.0):
                 "p99_ms": round(st["p99Us"] / 1e3, 3), "max_ms": round(st["maxUs"] / 1e3, 3),
                 "audio_s_per_s": round(st["audioSamples"] / 16000.0 / wall, 1), "completed": st["completed"],
                 "refused": st["refused"], "batches": st["batches"], "mean_batch_clips": round(st["meanBatchClips"], 2),
-                "gpu_busy_fraction": round(st["gpuBusySeconds"] / wall, 4)})
-    # saturation: a burst of 4000 requests submitted at once, drained by continuous batching
-    n_burst = 4000
+                "gpu_busy_fraction": round(st["gpuBusySeconds"] / wall, 4),
+                # the slowest request: waiting for its batch vs its batch's copies and launch
+                "slowest": {"ms": round(st["maxUs"] / 1e3, 3), "queue_ms": round(st["maxQueueUs"] / 1e3, 3),
+                            "service_ms": round(st["maxServiceUs"] / 1e3, 3),
+                            "batch_gpu_ms": round(st["maxBatchGpuUs"] / 1e3, 3),
+                            "scheduler_longest_turn_ms": round(st["maxLoopUs"] / 1e3, 3),
+                            "longest_launch_call_ms": round(st["maxLaunchUs"] / 1e3, 3),
+                            "longest_call_parts_ms": {"copies_in": round(st["maxCopyInCallUs"] / 1e3, 3),
+                                                      "launch": round(st["maxKernelCallUs"] / 1e3, 3),
+                                                      "copy_out": round(st["maxCopyOutCallUs"] / 1e3, 3)},
+                            "audio_s": round(st["maxLatencySamples"] / 16000.0, 2)},
+                "one_copy_submits": st["arenaSubmits"], "staged_submits": st["stagedSubmits"]})
+    # saturation: a burst of 4000 requests submitted at once from 8 host threads (the library call
+    # releases the interpreter lock, so the copies into the server run in parallel), drained by
+    # continuous batching
+    import threading
+    n_burst, n_threads = 4000, 8
     with Server(params, precision, max_batch=128, deadline_us=2000, mem_budget=8 << 30, slots=4, eager=True,
                 keep_outputs=False) as srv:
+        def submitter(k):
+            for i in range(k, n_burst, n_threads):
+                srv.submit(pool[i % len(pool)][:int(lengths[i % len(lengths)])])
+        ths = [threading.Thread(target=submitter, args=(k,)) for k in range(n_threads)]
         t0 = time.perf_counter()
-        for i in range(n_burst):
-            srv.submit(pool[i % len(pool)][:int(lengths[i % len(lengths)])])
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        t_sub = time.perf_counter() - t0
         srv.drain(120000)
         wall = time.perf_counter() - t0
         st = srv.stats()
-    res["saturation"] = {"requests": n_burst, "audio_s_per_s": round(st["audioSamples"] / 16000.0 / wall, 1),
+    sat = st["audioSamples"] / 16000.0 / wall
+    res["saturation"] = {"requests": n_burst, "audio_s_per_s": round(sat, 1),
                          "batches": st["batches"], "mean_batch_clips": round(st["meanBatchClips"], 2),
-                         "p50_ms": round(st["p50Us"] / 1e3, 3), "p99_ms": round(st["p99Us"] / 1e3, 3)}
+                         "p50_ms": round(st["p50Us"] / 1e3, 3), "p99_ms": round(st["p99Us"] / 1e3, 3),
+                         "one_copy_submits": st["arenaSubmits"], "staged_submits": st["stagedSubmits"],
+                         "submit_s": round(t_sub, 4), "submit_threads": n_threads}
+    # the host link the served clips cross: pinned host -> device copies of 256 MiB, and the
+    # samples' share of it at saturation (64,000 B per audio-second in)
+    import torch
+    hsrc = torch.empty(64 << 20, dtype=torch.float32, pin_memory=True)
+    ddst = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    ddst.copy_(hsrc, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ddst.copy_(hsrc, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = 5 * hsrc.numel() * 4 / (time.perf_counter() - t0) / 1e9
+    del hsrc, ddst
+    res["saturation"]["h2d_pinned_gbs"] = round(h2d, 2)
+    res["saturation"]["input_gbs"] = round(sat * 64000.0 / 1e9, 3)
+    res["saturation"]["input_fraction_of_h2d"] = round(sat * 64000.0 / 1e9 / h2d, 4)
     return res
 
 
@@ -257,6 +297,8 @@ def main() -> int:
                     help="duration of the served-stream measurement (Poisson arrivals, 4000 clips/s); 0 skips it")
     ap.add_argument("--front-end-only", action="store_true",
                     help="only the workload-4 front end (down-mix + resample, then fused log-mel; profiling runs)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served-stream measurement (profiling runs)")
     ap.add_argument("--trim-only", action="store_true",
                     help="only the workload-1 detection + region log-mel side line (profiling runs)")
     a = ap.parse_args()
@@ -277,6 +319,10 @@ def main() -> int:
             dst_c.copy_(src_c)
         torch.cuda.synchronize()
         del src_c, dst_c
+    if a.serve_only:
+        print(json.dumps({"serving": serve_measurement(api, synth, api.logmel_params(), a.precision,
+                                                       a.serve_seconds)}))
+        return 0
     if a.front_end_only:
         print(json.dumps({"front_end": front_end_measurement(api, synth, torch, a.precision, steps=a.steps)}))
         return 0

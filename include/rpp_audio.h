@@ -358,7 +358,11 @@ RppStatus rppt_logmel_get_tiles(rppHandle_t rppHandle, RpptLogMelTiles *tiles);
 /* ======================= serving (no counterpart in the reference) ======================= */
 /* A continuous-batching request loop over rppt_logmel_fused_packed_gpu on the current device:
  * host threads submit clips, a scheduler thread batches them onto a pool of device slots (each a
- * stream with device-resident buffers) and stamps each request's completion time. */
+ * stream with device-resident buffers) and stamps each request's completion time.  A submitted clip
+ * is copied once, into a pinned arena the batch's host-to-device copy reads (a clip that finds the
+ * arena full is held in its own buffer and staged instead); a clip larger than the slots were sized
+ * for is admitted when the free device memory at that moment (hipMemGetInfo) can hold a slot for
+ * it, and an idle slot is regrown for it. */
 typedef struct rppaServer rppaServer;
 typedef struct {
     int maxBatch;             /* clips per batch (upper bound) */
@@ -367,17 +371,26 @@ typedef struct {
     int numSlots;             /* concurrent batches (streams), 1..16 */
     int eager;                /* close a batch as soon as a slot is idle */
     int keepOutputs;          /* keep dB rows for rppa_server_wait (else timings only) */
+    int growSlots;            /* admit a clip larger than a slot when 80% of the free device memory
+                                 at submit covers a slot grown for it (else refuse it) */
 } rppaServeConfig;
 typedef struct {
     long long submitted, completed, refused, batches;
     double p50Us, p90Us, p99Us, meanUs, maxUs;   /* arrival -> result in host memory */
     double meanBatchClips, audioSamples, gpuBusySeconds;
     long long slotCapacitySamples;
+    double maxQueueUs, maxServiceUs;   /* the slowest request: arrival -> its batch launched, launch -> done */
+    long long maxLatencySamples;       /* and its length */
+    double maxBatchGpuUs;              /* its batch's copies and launch on the device (events) */
+    double maxLoopUs, maxLaunchUs;     /* the scheduler's longest loop turn, and longest batch launch call */
+    double maxCopyInCallUs, maxKernelCallUs, maxCopyOutCallUs;   /* longest host time of each part of it */
+    long long arenaSubmits, stagedSubmits, slotGrowths;
 } rppaServeStats;
 RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLogMelPrecision precision,
                              const rppaServeConfig *config, rppaServer **server);
 RppStatus rppa_server_destroy(rppaServer *server);
-/* copies `length` host samples; RPP_ERROR_NOT_ENOUGH_MEMORY when the clip can never fit a slot */
+/* copies `length` host samples; RPP_ERROR_NOT_ENOUGH_MEMORY when no slot can hold the clip (nor be
+ * grown for it within the free device memory) */
 RppStatus rppa_server_submit(rppaServer *server, const float *samples, int length, long long *id);
 /* blocks until `id` is done; out receives [numFilter][frames] dB rows when outputs are kept.  Several
  * threads may wait on one id (the request is freed when the last returns).  Without kept outputs a

@@ -3,19 +3,26 @@
 // Callers submit clips from any host thread; a scheduler thread forms batches and runs them on a
 // pool of device slots, each slot a stream with its own handle and device-resident buffers:
 //
-//   submit()  -> pending queue (arrival order, arrival timestamp)
+//   submit()  -> the clip is copied once, into a pinned arena (a ring allocated in submission order
+//                and freed when the clip's batch has finished; a clip that finds it full keeps its
+//                own copy and is staged through the slot's pinned buffer), then queued (arrival
+//                order, arrival timestamp)
 //   scheduler -> when a slot is free: take the oldest pending clips while they fit the slot's
 //                memory (admission by memory and slots); close the batch immediately when the
 //                device has an idle slot (eager) or when the oldest clip has waited `deadlineUs`
-//                or `maxBatch` clips are waiting; stage into pinned memory, copy to the slot's
-//                device buffers, launch rppt_logmel_fused_packed_gpu, copy the dB rows back and
-//                record an event -- on the slot's own stream, so batches overlap on the device
+//                or `maxBatch` clips are waiting; copy each run of arena-contiguous clips to the
+//                slot's device buffer in one copy, launch rppt_logmel_fused_packed_gpu, copy the dB
+//                rows back and record an event -- on the slot's own stream, so batches overlap on
+//                the device
 //   scheduler -> polls the slot events; a finished batch stamps each request's completion time
 //                (latency = completion - arrival) and frees the slot
 //
-// The slot buffers are sized once, at creation, from min(memBudgetBytes, 80% of free HBM); a clip
-// that can never fit a slot is refused at submit with RPP_ERROR_NOT_ENOUGH_MEMORY.  A clip's result
-// depends on that clip only (work items never span clips), so it is identical alone and in any mix.
+// The slot buffers are sized at creation from min(memBudgetBytes, 80% of free HBM).  With growSlots,
+// a clip larger than a slot holds is admitted when 80% of the free device memory at that moment
+// (hipMemGetInfo, refreshed by the scheduler every millisecond) covers a slot grown for it, and the
+// next idle slot is regrown before the clip's batch; otherwise it is refused at submit with
+// RPP_ERROR_NOT_ENOUGH_MEMORY.  A clip's result depends on that clip only (work items never span
+// clips), so it is identical alone and in any mix.
 #include "common.hpp"
 
 #include <algorithm>
@@ -40,9 +47,12 @@ using Clock = std::chrono::steady_clock;
 
 struct Request {
     long long id;
-    std::vector<float> samples;
+    std::vector<float> samples;   // the clip's own copy when the arena was full
+    const float *src = nullptr;   // its samples: in the arena or in `samples`
+    long long span = -1;          // arena span id (-1: staged)
+    size_t arenaOff = 0;          // offset of the samples in the arena (floats)
     int length = 0, frames = 0;
-    Clock::time_point arrival, done;
+    Clock::time_point arrival, launched, done;
     bool finished = false;
     int waiters = 0;          // rppa_server_wait calls blocked on this request (they own its lifetime)
     std::vector<float> out;
@@ -76,6 +86,21 @@ struct rppaServer {
     std::unordered_map<long long, rppa::Request *> live;
     std::vector<double> latUs;
     long long nextId = 1, submitted = 0, completed = 0, refused = 0, batches = 0, batchClips = 0;
+    // pinned submission arena: a ring of spans in allocation order (span ids spanFront ..)
+    struct Span { size_t off, n; bool freed; };
+    float *arena = nullptr;
+    size_t arenaCap = 0, arenaHead = 0;
+    std::deque<Span> spans;
+    long long spanFront = 0, spanNext = 0;
+    long long arenaSubmits = 0, stagedSubmits = 0, slotGrowths = 0;
+    // growth: the slot capacities the queued clips need, the free device memory last read
+    size_t growIn = 0, growOut = 0, maxInCap = 0, maxOutCap = 0;
+    std::atomic<long long> freeHbm{0};
+    double maxLatUs = 0, maxQueueUs = 0, maxServiceUs = 0, maxGpuUs = 0;
+    // scheduler turn and launch_batch host times (written by the scheduler thread only)
+    std::atomic<double> maxLoopUs{0.0}, maxLaunchUs{0.0};
+    std::atomic<double> maxCopyInUs{0.0}, maxKernelUs{0.0}, maxCopyOutUs{0.0};
+    long long maxLatLen = 0;
     double audioSamples = 0, busy = 0;
     std::atomic<bool> stop{false};
     std::thread worker;
@@ -90,31 +115,121 @@ int frames_of(const rppaServer *s, int L)
     return std::max(0, num_windows(std::max(L, 0), s->W, s->H, s->center != 0));
 }
 
+// Reserves n floats of the arena (caller holds s->mu); returns the span id or -1 when it is full.
+// The used region is [front span, head) -- wrapped past the end when the head is below the front.
+long long arena_alloc(rppaServer *s, size_t n, size_t &off)
+{
+    if (!s->arena || n > s->arenaCap) return -1;
+    if (s->spans.empty()) s->arenaHead = 0;
+    const size_t tail = s->spans.empty() ? 0 : s->spans.front().off;
+    // wrapped: the head has passed the end and restarted below the front (it never reaches it)
+    const bool wrapped = !s->spans.empty() && s->arenaHead < tail;
+    if (!wrapped) {
+        if (s->arenaHead + n <= s->arenaCap) off = s->arenaHead;
+        else if (n < tail) off = 0;          // wrap: the end's gap is left unused until the ring passes it
+        else return -1;
+    } else {
+        if (s->arenaHead + n < tail) off = s->arenaHead;
+        else return -1;
+    }
+    s->arenaHead = off + n;
+    s->spans.push_back({off, n, false});
+    return s->spanNext++;
+}
+
+// Releases span `id` (caller holds s->mu); the ring's front advances over released spans.
+void arena_free(rppaServer *s, long long id)
+{
+    if (id < 0) return;
+    s->spans[static_cast<size_t>(id - s->spanFront)].freed = true;
+    while (!s->spans.empty() && s->spans.front().freed) {
+        s->spans.pop_front();
+        s->spanFront++;
+    }
+}
+
+// Writes a slot's pinned buffers and takes every byte of them through a copy once (host to device
+// for the input staging, device to host for the outputs): the first copy through a fresh pinned
+// range is slow (milliseconds), which would otherwise land on some request's batch.
+void warm_slot(Slot &sl)
+{
+    std::memset(sl.hIn, 0, sizeof(float) * sl.inCap);
+    std::memset(sl.hOut, 0, sizeof(float) * sl.outCap);
+    (void)hipMemcpyAsync(sl.dIn, sl.hIn, sizeof(float) * sl.inCap, hipMemcpyHostToDevice, sl.stream);
+    (void)hipMemcpyAsync(sl.hOut, sl.dOut, sizeof(float) * sl.outCap, hipMemcpyDeviceToHost, sl.stream);
+    (void)hipStreamSynchronize(sl.stream);
+}
+
+// (Re)allocates a slot's data buffers for inCap input and outCap output floats.
+bool slot_buffers(Slot &sl, size_t inCap, size_t outCap)
+{
+    if (sl.dIn) (void)hipFree(sl.dIn);
+    if (sl.dOut) (void)hipFree(sl.dOut);
+    if (sl.hIn) (void)hipHostFree(sl.hIn);
+    if (sl.hOut) (void)hipHostFree(sl.hOut);
+    sl.dIn = sl.dOut = sl.hIn = sl.hOut = nullptr;
+    sl.inCap = sl.outCap = 0;
+    if (hipMalloc(&sl.dIn, sizeof(float) * inCap) != hipSuccess || hipMalloc(&sl.dOut, sizeof(float) * outCap) != hipSuccess ||
+        hipHostMalloc(&sl.hIn, sizeof(float) * inCap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&sl.hOut, sizeof(float) * outCap, hipHostMallocDefault) != hipSuccess)
+        return false;
+    sl.inCap = inCap;
+    sl.outCap = outCap;
+    warm_slot(sl);
+    return true;
+}
+
 bool launch_batch(rppaServer *s, Slot &sl, std::vector<Request *> &batch)
 {
     const int B = static_cast<int>(batch.size());
     long long *srcOff = sl.hMeta, *dstOff = sl.hMeta + sl.capB;
     int *len = reinterpret_cast<int *>(sl.hMeta + 2 * sl.capB);
     long long so = 0, doff = 0;
+    const auto now = Clock::now();
     for (int b = 0; b < B; b++) {
         Request *r = batch[b];
+        r->launched = now;
         srcOff[b] = so;
         dstOff[b] = doff;
         len[b] = r->length;
-        std::memcpy(sl.hIn + so, r->samples.data(), sizeof(float) * r->length);
+        if (r->span < 0) std::memcpy(sl.hIn + so, r->src, sizeof(float) * r->length);   // staged clip
         so += (r->length + 3) / 4 * 4;           // 16-byte aligned clip starts
         doff += static_cast<long long>(r->frames) * s->M;
     }
     const size_t metaBytes = sizeof(long long) * 2 * sl.capB + sizeof(int) * sl.capB;
+    auto upd = [](std::atomic<double> &m, Clock::time_point t0) {
+        m.store(std::max(m.load(), std::chrono::duration<double, std::micro>(Clock::now() - t0).count()));
+    };
+    auto t0 = Clock::now();
     if (hipEventRecord(sl.start, sl.stream) != hipSuccess) return false;
-    if (hipMemcpyAsync(sl.dIn, sl.hIn, sizeof(float) * so, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
+    // host-to-device: one copy per run of clips that lie back to back in the arena (or in the
+    // slot's staging buffer), straight from where they are
+    for (int b = 0; b < B;) {
+        const bool arena = batch[b]->span >= 0;
+        const float *from = arena ? s->arena + batch[b]->arenaOff : sl.hIn + srcOff[b];
+        long long n = 0;
+        int e = b;
+        while (e < B && (batch[e]->span >= 0) == arena &&
+               (!arena || s->arena + batch[e]->arenaOff == from + n)) {
+            n += (batch[e]->length + 3) / 4 * 4;
+            e++;
+        }
+        if (hipMemcpyAsync(sl.dIn + srcOff[b], from, sizeof(float) * n, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+            return false;
+        b = e;
+    }
     if (hipMemcpyAsync(sl.dMeta, sl.hMeta, metaBytes, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
+    upd(s->maxCopyInUs, t0);
+    t0 = Clock::now();
     RppStatus st = rppt_logmel_fused_packed_gpu(sl.dIn, sl.dMeta, reinterpret_cast<int *>(sl.dMeta + 2 * sl.capB), B,
                                                 sl.dOut, sl.dMeta + sl.capB, &s->params, s->precision, sl.h);
     if (st != RPP_SUCCESS) return false;
+    upd(s->maxKernelUs, t0);
+    t0 = Clock::now();
     if (hipMemcpyAsync(sl.hOut, sl.dOut, sizeof(float) * doff, hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
         return false;
     if (hipEventRecord(sl.done, sl.stream) != hipSuccess) return false;
+    upd(s->maxCopyOutUs, t0);
     sl.reqs.swap(batch);
     sl.busy = true;
     return true;
@@ -133,8 +248,19 @@ void finish_batch(rppaServer *s, Slot &sl)
         const size_t n = static_cast<size_t>(r->frames) * s->M;
         if (s->cfg.keepOutputs) r->out.assign(sl.hOut + doff, sl.hOut + doff + n);
         doff += static_cast<long long>(n);
-        s->latUs.push_back(std::chrono::duration<double, std::micro>(r->done - r->arrival).count());
+        const double lat = std::chrono::duration<double, std::micro>(r->done - r->arrival).count();
+        s->latUs.push_back(lat);
+        if (lat > s->maxLatUs) {
+            s->maxLatUs = lat;
+            s->maxQueueUs = std::chrono::duration<double, std::micro>(r->launched - r->arrival).count();
+            s->maxServiceUs = std::chrono::duration<double, std::micro>(r->done - r->launched).count();
+            s->maxLatLen = r->length;
+            s->maxGpuUs = ms * 1e3;
+        }
         s->audioSamples += r->length;
+        arena_free(s, r->span);
+        r->span = -1;
+        r->src = nullptr;
         std::vector<float>().swap(r->samples);
         // without kept outputs nothing is left to collect: forget the request now, unless a waiter
         // holds it (the last waiter frees it)
@@ -159,6 +285,8 @@ void fail_requests(rppaServer *s, std::vector<Request *> &reqs)
     for (Request *r : reqs) {
         r->finished = true;
         r->done = Clock::now();
+        arena_free(s, r->span);
+        r->span = -1;
         if (!s->cfg.keepOutputs && r->waiters == 0) {
             s->live.erase(r->id);
             delete r;
@@ -173,7 +301,21 @@ void scheduler(rppaServer *s)
 {
     (void)hipSetDevice(s->device);
     const auto deadline = std::chrono::microseconds(s->cfg.deadlineUs);
+    auto lastMemInfo = Clock::now() - std::chrono::seconds(1);
+    auto turn = Clock::now();
     while (!s->stop.load()) {
+        {
+            // the longest turn of this loop that did not sleep on purpose (waits below restart it)
+            const auto now = Clock::now();
+            s->maxLoopUs.store(std::max(s->maxLoopUs.load(), std::chrono::duration<double, std::micro>(now - turn).count()));
+            turn = now;
+        }
+        if (Clock::now() - lastMemInfo >= std::chrono::milliseconds(20)) {
+            // the free device memory submit() admits oversized clips against
+            size_t freeB = 0, totalB = 0;
+            if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) s->freeHbm.store(static_cast<long long>(freeB));
+            lastMemInfo = Clock::now();
+        }
         bool anyBusy = false;
         for (Slot &sl : s->slots) {
             if (!sl.busy) continue;
@@ -188,6 +330,35 @@ void scheduler(rppaServer *s)
             const auto age = Clock::now() - s->pending.front()->arrival;
             const bool close = s->cfg.eager || static_cast<int>(s->pending.size()) >= s->cfg.maxBatch || age >= deadline;
             if (!close) break;
+            {
+                // a queued clip larger than this idle slot: regrow the slot to the capacity the
+                // admitted clips need (checked against the free device memory at submit)
+                Request *r = s->pending.front();
+                const size_t rin = (r->length + 3) / 4 * 4, rout = static_cast<size_t>(r->frames) * s->M;
+                if (rin > sl.inCap || rout > sl.outCap) {
+                    const size_t gi = std::max(s->growIn, sl.inCap), go = std::max(s->growOut, sl.outCap);
+                    lk.unlock();
+                    (void)hipStreamSynchronize(sl.stream);
+                    const bool ok = slot_buffers(sl, gi, go);
+                    lk.lock();
+                    if (!ok) {
+                        // the memory is gone: fail the clip rather than stall the queue behind it
+                        fprintf(stderr, "rpp_audio: serving slot could not grow to %zu samples\n", gi);
+                        (void)slot_buffers(sl, s->slots[0].inCap ? s->slots[0].inCap : 1024,
+                                           s->slots[0].outCap ? s->slots[0].outCap : 1024);
+                        std::vector<Request *> one{r};
+                        s->pending.pop_front();
+                        lk.unlock();
+                        fail_requests(s, one);
+                        lk.lock();
+                        continue;
+                    }
+                    s->slotGrowths++;
+                    s->maxInCap = std::max(s->maxInCap, sl.inCap);
+                    s->maxOutCap = std::max(s->maxOutCap, sl.outCap);
+                    if (s->pending.empty()) continue;
+                }
+            }
             std::vector<Request *> batch;
             size_t in = 0, out = 0;
             while (!s->pending.empty() && static_cast<int>(batch.size()) < std::min(s->cfg.maxBatch, sl.capB)) {
@@ -201,7 +372,10 @@ void scheduler(rppaServer *s)
             }
             if (batch.empty()) break;
             lk.unlock();
+            const auto l0 = Clock::now();
             const bool ok = launch_batch(s, sl, batch);
+            s->maxLaunchUs.store(
+                std::max(s->maxLaunchUs.load(), std::chrono::duration<double, std::micro>(Clock::now() - l0).count()));
             if (!ok) fail_requests(s, batch);
             lk.lock();
             anyBusy = true;
@@ -209,9 +383,11 @@ void scheduler(rppaServer *s)
         if (s->stop.load()) break;
         if (!anyBusy && s->pending.empty()) {
             s->cvWork.wait_for(lk, std::chrono::milliseconds(5));
+            turn = Clock::now();
         } else if (!anyBusy && !s->cfg.eager) {
             // a batch is forming: sleep until the oldest clip's deadline or a new arrival
             s->cvWork.wait_for(lk, std::chrono::microseconds(std::max(20, s->cfg.deadlineUs / 8)));
+            turn = Clock::now();
         } else {
             lk.unlock();
             std::this_thread::yield();
@@ -242,6 +418,7 @@ static void destroy_server(rppaServer *s)
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
+    if (s->arena) (void)hipHostFree(s->arena);
     for (auto &kv : s->live) delete kv.second;  // every pending request is also live
     delete s;
 }
@@ -297,6 +474,24 @@ RPPA_EXPORT RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLog
             return RPP_ERROR_NOT_ENOUGH_MEMORY;
         }
     }
+    s->maxInCap = inCap;
+    s->maxOutCap = outCap;
+    s->freeHbm.store(static_cast<long long>(freeB));
+    // the pinned submission arena: room for two rounds of full slots, at most 256 MiB; every page is
+    // written and taken through a host-to-device copy once here (the first copies from a fresh pinned
+    // range are slow), into the first slot's device buffer in pieces it holds
+    s->arenaCap = std::min<size_t>(2 * inCap * config->numSlots, static_cast<size_t>(64) << 20);
+    if (hipHostMalloc(&s->arena, sizeof(float) * s->arenaCap, hipHostMallocDefault) != hipSuccess) {
+        s->arena = nullptr;
+        s->arenaCap = 0;
+    } else {
+        std::memset(s->arena, 0, sizeof(float) * s->arenaCap);
+        Slot &s0 = s->slots[0];
+        for (size_t o = 0; o < s->arenaCap; o += s0.inCap)
+            (void)hipMemcpyAsync(s0.dIn, s->arena + o, sizeof(float) * std::min(s0.inCap, s->arenaCap - o),
+                                 hipMemcpyHostToDevice, s0.stream);
+        (void)hipStreamSynchronize(s0.stream);
+    }
     // build every table, size every workspace and take each slot's copy paths once before traffic
     // arrives: capB zero clips spread over the first 64 MB of the pinned input buffer, copied in,
     // run and copied back (a workspace that grew later would free its old buffer, which waits for
@@ -306,6 +501,7 @@ RPPA_EXPORT RppStatus rppa_server_create(const RpptLogMelParams *params, RpptLog
     while (warmLen > 4 && static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB > outCap) warmLen = (warmLen / 2 + 3) / 4 * 4;
     const bool canWarm = static_cast<size_t>(warmLen) * capB <= inCap &&
                          static_cast<size_t>(frames_of(s, warmLen)) * s->M * capB <= outCap;
+    for (Slot &sl : s->slots) warm_slot(sl);
     for (Slot &sl : s->slots) {
         if (!canWarm) break;
         long long *srcOff = sl.hMeta, *dstOff = sl.hMeta + capB;
@@ -350,20 +546,48 @@ RPPA_EXPORT RppStatus rppa_server_submit(rppaServer *s, const float *samples, in
     if (!s || !samples || length <= 0 || !id) return RPP_ERROR_INVALID_ARGUMENTS;
     const auto arrival = Clock::now();
     const int frames = frames_of(s, length);
-    const Slot &sl = s->slots[0];
-    if (static_cast<size_t>((length + 3) / 4 * 4) > sl.inCap || static_cast<size_t>(frames) * s->M > sl.outCap) {
+    const size_t n4 = static_cast<size_t>((length + 3) / 4 * 4), nOut = static_cast<size_t>(frames) * s->M;
+    {
         std::lock_guard<std::mutex> lk(s->mu);
-        s->refused++;
-        return RPP_ERROR_NOT_ENOUGH_MEMORY;
+        if (n4 > s->maxInCap || nOut > s->maxOutCap) {
+            // admission by the free device memory now: a slot regrown for this clip (a quarter of
+            // headroom) must fit 80% of it
+            if (!s->cfg.growSlots) {
+                s->refused++;
+                return RPP_ERROR_NOT_ENOUGH_MEMORY;
+            }
+            const size_t gi = std::max(s->growIn, n4 + n4 / 4), go = std::max(s->growOut, nOut + nOut / 4);
+            const double extra = 4.0 * (static_cast<double>(gi) - static_cast<double>(s->slots[0].inCap)) +
+                                 4.0 * (static_cast<double>(go) - static_cast<double>(s->slots[0].outCap));
+            if (extra > 0.8 * static_cast<double>(s->freeHbm.load())) {
+                s->refused++;
+                return RPP_ERROR_NOT_ENOUGH_MEMORY;
+            }
+            s->growIn = gi;
+            s->growOut = go;
+        }
     }
     Request *r = new (std::nothrow) Request();
     if (!r) return RPP_ERROR_NOT_ENOUGH_MEMORY;
-    r->samples.assign(samples, samples + length);
     r->length = length;
     r->frames = frames;
     r->arrival = arrival;
     {
         std::lock_guard<std::mutex> lk(s->mu);
+        r->span = arena_alloc(s, n4, r->arenaOff);
+    }
+    if (r->span >= 0) {
+        // the one host copy: into the arena the batch's host-to-device copy reads
+        r->src = s->arena + r->arenaOff;
+        std::memcpy(s->arena + r->arenaOff, samples, sizeof(float) * length);
+    } else {
+        r->samples.assign(samples, samples + length);
+        r->src = r->samples.data();
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->span >= 0) s->arenaSubmits++;
+        else s->stagedSubmits++;
         r->id = s->nextId++;
         s->live[r->id] = r;
         s->pending.push_back(r);
@@ -444,6 +668,18 @@ RPPA_EXPORT RppStatus rppa_server_stats(rppaServer *s, rppaServeStats *st)
     st->meanBatchClips = s->batches ? static_cast<double>(s->batchClips) / s->batches : 0.0;
     st->audioSamples = s->audioSamples;
     st->gpuBusySeconds = s->busy;
-    st->slotCapacitySamples = s->slots.empty() ? 0 : static_cast<long long>(s->slots[0].inCap);
+    st->slotCapacitySamples = static_cast<long long>(s->maxInCap);
+    st->maxQueueUs = s->maxQueueUs;
+    st->maxServiceUs = s->maxServiceUs;
+    st->maxLatencySamples = s->maxLatLen;
+    st->maxBatchGpuUs = s->maxGpuUs;
+    st->maxLoopUs = s->maxLoopUs.load();
+    st->maxLaunchUs = s->maxLaunchUs.load();
+    st->maxCopyInCallUs = s->maxCopyInUs.load();
+    st->maxKernelCallUs = s->maxKernelUs.load();
+    st->maxCopyOutCallUs = s->maxCopyOutUs.load();
+    st->arenaSubmits = s->arenaSubmits;
+    st->stagedSubmits = s->stagedSubmits;
+    st->slotGrowths = s->slotGrowths;
     return RPP_SUCCESS;
 }

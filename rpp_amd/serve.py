@@ -10,14 +10,18 @@ from . import api
 
 class ServeConfig(C.Structure):
     _fields_ = [("maxBatch", C.c_int), ("deadlineUs", C.c_int), ("memBudgetBytes", C.c_longlong),
-                ("numSlots", C.c_int), ("eager", C.c_int), ("keepOutputs", C.c_int)]
+                ("numSlots", C.c_int), ("eager", C.c_int), ("keepOutputs", C.c_int), ("growSlots", C.c_int)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("refused", C.c_longlong),
                 ("batches", C.c_longlong), ("p50Us", C.c_double), ("p90Us", C.c_double), ("p99Us", C.c_double),
                 ("meanUs", C.c_double), ("maxUs", C.c_double), ("meanBatchClips", C.c_double),
-                ("audioSamples", C.c_double), ("gpuBusySeconds", C.c_double), ("slotCapacitySamples", C.c_longlong)]
+                ("audioSamples", C.c_double), ("gpuBusySeconds", C.c_double), ("slotCapacitySamples", C.c_longlong),
+                ("maxQueueUs", C.c_double), ("maxServiceUs", C.c_double), ("maxLatencySamples", C.c_longlong),
+                ("maxBatchGpuUs", C.c_double), ("maxLoopUs", C.c_double), ("maxLaunchUs", C.c_double),
+                ("maxCopyInCallUs", C.c_double), ("maxKernelCallUs", C.c_double), ("maxCopyOutCallUs", C.c_double),
+                ("arenaSubmits", C.c_longlong), ("stagedSubmits", C.c_longlong), ("slotGrowths", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -44,10 +48,11 @@ def _bind():
 
 class Server:
     def __init__(self, params: api.RpptLogMelParams, precision="fp16x3", max_batch=64, deadline_us=2000,
-                 mem_budget=1 << 30, slots=4, eager=True, keep_outputs=True):
+                 mem_budget=1 << 30, slots=4, eager=True, keep_outputs=True, grow_slots=True):
         l = _bind()
         self.params = params
-        self.cfg = ServeConfig(max_batch, deadline_us, mem_budget, slots, int(eager), int(keep_outputs))
+        self.cfg = ServeConfig(max_batch, deadline_us, mem_budget, slots, int(eager), int(keep_outputs),
+                               int(grow_slots))
         self.s = C.c_void_p()
         api.check("rppa_server_create", l.rppa_server_create(C.byref(params), api.PRECISION[precision],
                                                             C.byref(self.cfg), C.byref(self.s)))

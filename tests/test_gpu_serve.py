@@ -1,6 +1,7 @@
 """The continuous-batching server through the C-ABI: every clip's result is the same alone and in
 the mix (bit for bit: work items never span clips), within 0.05 dB of the FP64 oracle (fp16x3),
-admission refuses a clip that cannot fit, and a non-eager batch closes at its deadline."""
+admission refuses a clip that cannot fit (or grows a slot for it against the free device memory),
+and a non-eager batch closes at its deadline."""
 import time
 
 import numpy as np
@@ -55,7 +56,7 @@ def test_same_alone_and_in_the_mix(gpu, orc):
 
 def test_admission_refuses_what_cannot_fit(gpu):
     from rpp_amd.serve import Server
-    with Server(gpu.logmel_params(), "fp16x3", max_batch=4, mem_budget=1 << 20, slots=1) as srv:
+    with Server(gpu.logmel_params(), "fp16x3", max_batch=4, mem_budget=1 << 20, slots=1, grow_slots=False) as srv:
         cap = srv.stats()["slotCapacitySamples"]
         assert 1000 < cap < 1 << 20
         big = np.zeros(cap + 1000, np.float32)
@@ -64,6 +65,26 @@ def test_admission_refuses_what_cannot_fit(gpu):
         o, lat = srv.wait(rid, 80, 2000 // 160 + 1)
         assert np.isfinite(o).all()
         assert srv.stats()["refused"] == 1
+
+
+def test_admission_grows_a_slot_for_a_long_clip(gpu):
+    """With growSlots a clip longer than the slots were sized for is admitted against the free device
+    memory at submit: an idle slot is regrown for it, its result equals the clip alone, and the
+    clips before and after it still run (one host copy each, through the pinned arena)."""
+    from rpp_amd.serve import Server
+    with Server(gpu.logmel_params(), "fp16x3", max_batch=4, deadline_us=2000, mem_budget=1 << 20, slots=2) as srv:
+        cap = srv.stats()["slotCapacitySamples"]
+        xs = clips(3, seed=21, lo=2000, hi=8000)
+        long_clip = synth.clip_for_request(99, int(cap) + 20000, 21)
+        seq = [xs[0], long_clip, xs[1], xs[2]]
+        ids = [srv.submit(x) for x in seq]
+        outs = [srv.wait(r, 80, len(x) // 160 + 1)[0] for r, x in zip(ids, seq)]
+        st = srv.stats()
+    assert st["refused"] == 0 and st["completed"] == 4
+    assert st["slotGrowths"] >= 1 and st["slotCapacitySamples"] >= len(long_clip)
+    assert st["arenaSubmits"] + st["stagedSubmits"] == 4
+    for x, o in zip(seq, outs):
+        assert np.array_equal(o, alone(gpu, x))
 
 
 def test_deadline_closes_a_partial_batch(gpu):
