@@ -1259,7 +1259,19 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     }
     if (L.mf->nPairs <= 0) return RPP_ERROR_NOT_IMPLEMENTED;
     const int nMT = (p->numFilter + 15) / 16;
-    const unsigned grid = static_cast<unsigned>(h->numCU);
+    // one workgroup per CU, or fewer when the batch cannot give every wave an item: the host bounds
+    // the items by the row stride (strided layout; the packed layout's lengths are on the device)
+    unsigned grid = static_cast<unsigned>(h->numCU);
+    if (!L.srcOff) {
+        long long maxLen = L.srcStride;
+        if (front) maxLen = static_cast<long long>(std::ceil(static_cast<double>(L.fe->stride) * L.fe->outRate / L.fe->inRate)) + 1;
+        const long long maxF = std::min<long long>(std::max(num_windows(static_cast<int>(std::min<long long>(maxLen, 0x7fffffffLL)),
+                                                                        p->windowLength, p->windowStep, p->centerWindows != 0),
+                                                            0),
+                                                   L.dstOff ? 0x7fffffffLL : static_cast<long long>(L.frameCap));
+        const long long items = static_cast<long long>(L.B) * ((maxF + kFI - 1) / kFI);
+        grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, (items + kWaves - 1) / kWaves)));
+    }
     const bool general = a.power != 2 || a.ntf || a.clipMax || !(a.minRatio >= 1.17549435e-38f);
     // the front end runs with the common launch only (the caller falls back to two launches)
     if (front && general) return RPP_ERROR_NOT_IMPLEMENTED;
