@@ -83,16 +83,20 @@ struct FftTables {
     int nfft = 0, W = 0;
     uint16_t *a1 = nullptr, *a2 = nullptr;
     float *winP = nullptr, *tw = nullptr;
+    float *tw2 = nullptr;   // nfft 1024: the radix-2 twiddles
     void *mem = nullptr;
 };
 
 // Mel weight fragments of the (16-filter tile, 32-bin chunk) pairs a filter bank touches, for the
 // factored kernel: bf16 hi/lo in MFMA operand order, pairOff[tile * 16 + chunk] = fragment index or
 // -1, pairOff[256] = nPairs.
+// nfft 1024: each chunk has a run of pairs for bins 0 .. 255 and one for bins 512 - u (words
+// pairOff[c] and pairOff[16 + c]), and w256 holds the weights of bin 256 (after pairOff's 257 ints).
 struct MelFrag {
     uint16_t *frag = nullptr;
     int *pairOff = nullptr;
     int nPairs = 0;
+    float *w256 = nullptr;
 };
 
 }  // namespace rppa
@@ -200,13 +204,16 @@ struct FftLaunch {
 bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int B);
 // the fused front end's resampling shapes: integer ratio k and window lobes
 bool fft_front_eligible(int k, int lobes);
-size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false);
+size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false);
+// nfft 1024 runs only the common launch (power 2, NFT output, a fixed dB reference, a normal cut-off
+// ratio) on tables the handle owns
+bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool callerMem, bool frontEnd);
 const FftTables *get_fft_tables(rppHandle *h, int nfft, int W);
 const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string &key);
 // the same tables in caller-provided memory (no allocation): fft_table_bytes() bytes at `mem`
 // (256-byte aligned) for W samples of window; mel_frag_bytes(M) + 257 ints for the fragments
-size_t fft_table_bytes(int W);
-void fft_tables_carve(char *mem, int W, FftTables &t);
+size_t fft_table_bytes(int W, int nfft = 512);
+void fft_tables_carve(char *mem, int W, FftTables &t, int nfft = 512);
 hipError_t build_fft_tables_at(hipStream_t st, int W, FftTables &t);
 size_t mel_frag_bytes(int M);
 hipError_t build_mel_frag_at(hipStream_t st, const MelTables &mt, MelFrag &f);

@@ -68,6 +68,8 @@ constexpr int kN = kN1 * kN2;
 constexpr int kFI = 16;               // frames per work item
 constexpr int kWaves = 8;             // waves per workgroup (two per SIMD)
 constexpr int kThreads = 64 * kWaves;
+constexpr int kWavesR2 = 8;           // nfft 1024: waves per workgroup
+constexpr int kPitchR2 = 88;          // nfft 1024: floats per phase row (odd rows skewed by 4 more)
 constexpr int kMaxMT = 8;             // filter tiles of 16 (numFilter <= 128)
 constexpr int kChunks = kN / 2 / 32;  // 32-bin chunks of bins 0 .. N/2 - 1
 // fused front end: 2 x 16 FIR taps (16 zero crossings), resampled outputs per chunk (two per lane)
@@ -109,10 +111,15 @@ __device__ inline void split_f16_scaled(double v, uint16_t &hi, uint16_t &lo)
 //                          k2 = m (ort 0) or 15 - m (ort 1)
 //                          which 1: bins 16 q from (Y[.][0], Y[.][16]); row 16 ort + 4 gg + 2 qq + po is
 //                          part po of q = 4 gg + 2 ort + qq
-//   winP [n2][n1]          w[16 n1 + n2] (zero at and past W)
+//   winP [n2][n1]          w[16 n1 + n2] (zero at and past W); nfft 1024: [p 2][n2][n1] w[32 n1 + 2 n2 + p]
 //   tw   [k1 16][n2 16][2] 2^-12 (cos, sin)(2 pi n2 k1 / 512); k1 = 0: (2^-12, 0)
-__global__ void fft_tables_kernel(int W, const float *__restrict__ win, uint16_t *__restrict__ a1,
-                                  uint16_t *__restrict__ a2, float *__restrict__ winP, float *__restrict__ tw)
+//   tw2  nfft 1024 only, [g 4][k1 16][j 4][2]: for value j of the stage-2 slot k1 in lane group g (bin
+//        k = 64 g + (k1 ? {k1, k1 + 32, 32 - k1, 64 - k1}[j] : 16 j)) the parity-0 factor
+//        conj(W1024^k) = (cos, sin)(2 pi k / 1024), conjugated for the values the slot holds
+//        conjugated (k1 > 0, j >= 2)
+__global__ void fft_tables_kernel(int W, int nfft, const float *__restrict__ win, uint16_t *__restrict__ a1,
+                                  uint16_t *__restrict__ a2, float *__restrict__ winP, float *__restrict__ tw,
+                                  float *__restrict__ tw2)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     // a1: 2 rt x 512 elements per plane
@@ -158,7 +165,20 @@ __global__ void fft_tables_kernel(int W, const float *__restrict__ win, uint16_t
         a2[(((which * 2 + ort) * 2 + 0) * 64 + lane) * 8 + j] = hi;
         a2[(((which * 2 + ort) * 2 + 1) * 64 + lane) * 8 + j] = lo;
     }
-    if (t < kN2 * kN1) {
+    if (nfft == 2 * kN) {
+        if (t < 2 * kN2 * kN1) {
+            const int p = t / (kN2 * kN1), n2 = (t / kN1) % kN2, n1 = t % kN1, n = 32 * n1 + 2 * n2 + p;
+            winP[t] = n < W ? win[n] : 0.0f;
+        }
+        if (t < 4 * 16 * 4) {
+            const int g = t >> 6, k1 = (t >> 2) & 15, j = t & 3;
+            const int idx = k1 == 0 ? 16 * j : (j == 0 ? k1 : (j == 1 ? k1 + 32 : (j == 2 ? 32 - k1 : 64 - k1)));
+            const double th = 2.0 * kPi * static_cast<double>(64 * g + idx) / (2 * kN);
+            const double sgn = (k1 > 0 && j >= 2) ? -1.0 : 1.0;
+            tw2[2 * t] = static_cast<float>(cos(th));
+            tw2[2 * t + 1] = static_cast<float>(sgn * sin(th));
+        }
+    } else if (t < kN2 * kN1) {
         const int n2 = t / kN1, n1 = t % kN1, n = 16 * n1 + n2;
         winP[t] = n < W ? win[n] : 0.0f;
     }
@@ -189,65 +209,82 @@ __device__ inline uint16_t bf16_bits(float v)
 }
 
 // Mel weight fragments of the (16-filter tile, 32-bin chunk) pairs the filters touch, bf16 hi/lo in
-// A-operand lane order ([pair][hl][64][8]: lane l holds filter 16 mt + l % 16, bins
-// 32 c + 8 (l / 16) .. + 7).  Pairs are numbered chunk by chunk, tiles ascending within a chunk: the
-// tiles whose band [c0, c1) contains chunk c are a contiguous run [t0, t1) (the filters ascend in
-// frequency), so pairOff[c] = t0 | t1 << 8 | p0 << 16 with p0 the run's first pair (t0 = t1: no
-// pair), pairOff[256] = pair count, pairOff[255] = 1 if some chunk's tiles are not contiguous (the
+// A-operand lane order ([pair][hl][64][8]: lane l holds filter 16 mt + l % 16, positions
+// 32 c + 8 (l / 16) .. + 7 of the chunk).  Pairs are numbered chunk by chunk, tiles ascending within
+// a chunk: the tiles whose band [c0, c1) contains chunk c are a contiguous run [t0, t1) (the filters
+// ascend in frequency), so pairOff[c] = t0 | t1 << 8 | p0 << 16 with p0 the run's first pair (t0 = t1:
+// no pair), pairOff[256] = pair count, pairOff[255] = 1 if some chunk's tiles are not contiguous (the
 // factored kernel then cannot use the table).  One workgroup.
+// r2 = 0: position u is bin u (bins 0 .. N/2 - 1).  r2 = 1 (nfft 1024, radix-2 split): each chunk has
+// two runs, first the lower half's (position u is bin u < 256, word pairOff[c]) and then the upper
+// half's (position u is bin 512 - u, bins 257 .. 512, word pairOff[16 + c]), numbered in that order;
+// bin 256's weights go to w256[M].
 __global__ void __launch_bounds__(256) mel_frag_kernel(const float *__restrict__ dense, const int *__restrict__ filterBins,
-                                                       int M, int numBins, uint16_t *__restrict__ frag,
-                                                       int *__restrict__ pairOff)
+                                                       int M, int numBins, int r2, uint16_t *__restrict__ frag,
+                                                       int *__restrict__ pairOff, float *__restrict__ w256)
 {
-    __shared__ int band[16][2];
-    __shared__ int pairOf[16][16];   // [chunk][tile]
-    __shared__ int word[16];
+    __shared__ int band[2][16][2];
+    __shared__ int pairOf[2][16][16];   // [half][chunk][tile]
+    __shared__ int word[32];
     __shared__ int total, broken;
     const int nMT = (M + 15) / 16;
-    const int nCh = (numBins - 1) / 32;   // bin N/2 carries no weight
+    const int nCh = r2 ? kChunks : min((numBins - 1) / 32, kChunks);   // bin N/2 carries no weight
+    const int halves = r2 ? 2 : 1;
     const int tid = threadIdx.x;
-    if (tid < 16) {
+    if (tid < 32) {
+        const int half = tid >> 4, mt = tid & 15;
         int lo = 1 << 30, hi = 0;
-        for (int i = 0; i < 16 && tid < nMT; i++) {
-            const int m = tid * 16 + i;
+        for (int i = 0; i < 16 && mt < nMT; i++) {
+            const int m = mt * 16 + i;
             if (m < M && filterBins[2 * m] < filterBins[2 * m + 1]) {
                 lo = min(lo, filterBins[2 * m]);
                 hi = max(hi, filterBins[2 * m + 1]);
             }
         }
-        band[tid][0] = lo < hi ? lo / 32 : 0;
-        band[tid][1] = lo < hi ? min((hi + 31) / 32, nCh) : 0;
+        if (r2 && half == 0) hi = min(hi, 256);
+        if (r2 && half == 1) {
+            // bins [max(lo, 257), min(hi, 513)) are positions u = 512 - bin in [513 - hi', 513 - lo')
+            const int bl = max(lo, 257), bh = min(hi, 513);
+            lo = bl < bh ? 513 - bh : 0;
+            hi = bl < bh ? 513 - bl : 0;
+        }
+        band[half][mt][0] = lo < hi ? lo / 32 : 0;
+        band[half][mt][1] = lo < hi ? min((hi + 31) / 32, nCh) : 0;
     }
+    if (r2)
+        for (int m = tid; m < M; m += blockDim.x) w256[m] = dense[static_cast<size_t>(m) * numBins + 256];
     __syncthreads();
     if (tid == 0) {
         int n = 0, bad = 0;
-        for (int c = 0; c < 16; c++) {
-            int t0 = -1, t1 = -1;
-            for (int mt = 0; mt < 16; mt++) {
-                const bool in = mt < nMT && c >= band[mt][0] && c < band[mt][1];
-                pairOf[c][mt] = -1;
-                if (!in) continue;
-                if (t0 < 0) t0 = mt;
-                else if (t1 != mt) bad = 1;   // a gap in the run
-                t1 = mt + 1;
-                pairOf[c][mt] = n + (mt - t0);
+        for (int c = 0; c < 16; c++)
+            for (int half = 0; half < 2; half++) {
+                int t0 = -1, t1 = -1;
+                for (int mt = 0; mt < 16; mt++) {
+                    const bool in = half < halves && mt < nMT && c >= band[half][mt][0] && c < band[half][mt][1];
+                    pairOf[half][c][mt] = -1;
+                    if (!in) continue;
+                    if (t0 < 0) t0 = mt;
+                    else if (t1 != mt) bad = 1;   // a gap in the run
+                    t1 = mt + 1;
+                    pairOf[half][c][mt] = n + (mt - t0);
+                }
+                if (t0 < 0) t0 = t1 = 0;
+                word[16 * half + c] = t0 | (t1 << 8) | (n << 16);
+                n += t1 - t0;
             }
-            if (t0 < 0) t0 = t1 = 0;
-            word[c] = t0 | (t1 << 8) | (n << 16);
-            n += t1 - t0;
-        }
         total = n;
         broken = bad;
     }
     __syncthreads();
     for (int i = tid; i < 257; i += blockDim.x)
-        pairOff[i] = i < 16 ? word[i] : (i == 256 ? total : (i == 255 ? broken : 0));
-    for (int cm = 0; cm < 256; cm++) {
-        const int c = cm >> 4, mt = cm & 15, p = pairOf[c][mt];
+        pairOff[i] = i < 16 * halves ? word[i] : (i == 256 ? total : (i == 255 ? broken : 0));
+    for (int hcm = 0; hcm < 512; hcm++) {
+        const int half = hcm >> 8, c = (hcm >> 4) & 15, mt = hcm & 15, p = pairOf[half][c][mt];
         if (p < 0) continue;
         for (int e = tid; e < 512; e += blockDim.x) {
             const int lane = e >> 3, j = e & 7;
-            const int m = mt * 16 + (lane & 15), k = 32 * c + 8 * (lane >> 4) + j;
+            const int m = mt * 16 + (lane & 15), u = 32 * c + 8 * (lane >> 4) + j;
+            const int k = half ? 512 - u : u;
             const float v = (m < M && k < numBins) ? dense[static_cast<size_t>(m) * numBins + k] : 0.0f;
             const uint16_t h = bf16_bits(v);
             const uint16_t l = bf16_bits(v - __uint_as_float(static_cast<uint32_t>(h) << 16));
@@ -279,6 +316,8 @@ struct FftArgs {
     const uint16_t *melFrag;
     const int *pairOff;
     int nPairs;
+    // nfft 1024 (radix-2 launch): the radix-2 twiddles and the mel weights of bin 256
+    const float *tw2, *w256;
     int pitch;       // floats per phase row of a wave's segment
     // fused front end (FRONT kernels): interleaved multi-channel clips fsrc[b * fStride + t * C + c],
     // fDims[2 b] samples per channel and fDims[2 b + 1] channels, down-mixed and resampled by the
@@ -335,12 +374,19 @@ __device__ __forceinline__ Clip clip_of(const FftArgs &a, int b, const int *lenL
     return v;
 }
 
-__device__ __forceinline__ int items_of(const FftArgs &a, int L)
+__device__ __forceinline__ int items_of(const FftArgs &a, int L, int FI)
 {
     int F = max(num_windows(L, a.W, a.H, a.center != 0), 0);
     if (!a.dstOff) F = min(F, static_cast<int>(a.frameCap));
-    return (F + kFI - 1) / kFI;
+    return (F + FI - 1) / FI;
 }
+
+// The value of v in lane l ^ 8 (the other half of the lane's 16-lane row): DPP row rotate by 8.
+__device__ __forceinline__ uint32_t swap8u(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x128, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float swap8(float v) { return __uint_as_float(swap8u(__float_as_uint(v))); }
 
 // Pre-emphasised sample j of a clip of length L with the spectrogram's padding rule (the chain pads
 // the pre-emphasised signal): the reflect map for indices outside the clip, zero without reflect;
@@ -481,15 +527,37 @@ constexpr int pitch_for(int positions)
 // FRONT: 0, or the clips arrive interleaved at FRONT (2 or 3) times the rate and each item computes
 // the down-mixed, resampled samples it reads (bit for bit what the down-mix + resample launch
 // writes) instead of loading them.
-template <int NMT, int HOP, bool GENERAL, int FRONT>
-__global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
+//
+// NFFT 1024 (R2): radix-2 over the same 512-point machinery.  A 1024-point frame's even and odd
+// samples are two columns of the 512-point DFT (column l % 16 = frame l % 8, parity (l / 8) % 2;
+// 8 frames per item, the segment in 32 polyphase rows so column (f, p) reads rows 2 n2 + p); after
+// stage 2 the lanes of a frame's two parities swap their values (lanes l and l ^ 8) and each forms
+// X[k] = E[k] + W1024^k O[k] (parity 0, bin k) or X[512 - k] = conj(E[k] - W1024^k O[k]) (parity 1,
+// bin 512 - k) for the k < 256 it holds; bin 256 is E[256]^2 + O[256]^2 from the stage-1 sums.  The
+// mel runs over both halves with their own fragment tables (the lower half's columns valid in the
+// parity-0 accumulators, the upper half's in the parity-1 ones), which are then added across the
+// lane pair together with bin 256's weights.
+template <int NMT, int HOP, bool GENERAL, int FRONT, int NFFT = 512>
+__global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) logmel_fft_kernel(FftArgs a)
 {
-    constexpr int HQ = HOP / 16;                     // positions per hop
-    constexpr int SEGN = (kFI - 1) * HOP + kN;       // segment samples (zero past the frames' data)
-    constexpr int PITCH = pitch_for(SEGN / 16);
+    constexpr bool R2 = NFFT == 1024;
+    constexpr int FI = R2 ? 8 : kFI;                 // frames per item
+    constexpr int NW = R2 ? kWavesR2 : kWaves;       // waves per workgroup
+    constexpr int NT = 64 * NW;
+    constexpr int NPH = R2 ? 32 : 16;                // polyphase rows of a segment
+    constexpr int HQ = HOP / NPH;                    // positions per hop
+    constexpr int SEGN = (FI - 1) * HOP + NFFT;      // segment samples (zero past the frames' data)
+    // phase row r of a segment starts at row_at(r): nfft 512 a uniform pitch; nfft 1024 a pitch of
+    // 88 with every odd row 4 floats further on, so that a frame's two parities (rows 2 n2 and
+    // 2 n2 + 1) fall on different LDS banks
+    constexpr int PITCH = R2 ? kPitchR2 : pitch_for(SEGN / NPH);
+    constexpr int SEGF = R2 ? NPH * PITCH + 4 * (NPH / 2) : NPH * PITCH;   // floats per segment
+    auto row_at = [](int r) { return R2 ? r * PITCH + 4 * ((r + 1) >> 1) : r * PITCH; };
+    static_assert(!R2 || (FRONT == 0 && !GENERAL), "the radix-2 launch is the common one");
+    static_assert(!R2 || PITCH >= SEGN / NPH, "nfft 1024 segment rows");
     constexpr int ROUNDS = (SEGN + 1023) / 1024;
     static_assert(HOP % 32 == 0 && ROUNDS <= 3, "segment geometry");
-    static_assert(PITCH - SEGN / 16 >= 2, "two spare columns for the front end's predecessors");
+    static_assert(!FRONT || PITCH - SEGN / NPH >= 2, "two spare columns for the front end's predecessors");
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // LDS: stage fragments a1 (4 KB) and a2 (8 KB), window [16][32], twiddles [16][16] float2, mel
@@ -497,8 +565,10 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     u32x4 *A1 = reinterpret_cast<u32x4 *>(lds);
     u32x4 *A2 = A1 + 256;
     float *winP = reinterpret_cast<float *>(A2 + 512);
-    float *tw = winP + kN2 * kN1;
-    u32x4 *melF = reinterpret_cast<u32x4 *>(tw + 2 * 256);
+    float *tw = winP + (R2 ? 2 : 1) * kN2 * kN1;
+    const float4 *tw2 = reinterpret_cast<const float4 *>(tw + 2 * 256);   // R2: radix-2 twiddles [g][k1][4]
+    float *w256 = tw + 2 * 256 + (R2 ? 2 * 256 : 0);                      // R2: bin 256's mel weights
+    u32x4 *melF = reinterpret_cast<u32x4 *>(w256 + (R2 ? 16 * NMT : 0));
     // per clip (B rounded up to 4): the plan ts[0 .. B], the region length, the region start (with a
     // begin array) or the packed offsets; then the mel chunk words, then the waves' segments
     const int B4 = (a.B + 1 + 3) / 4 * 4;
@@ -507,23 +577,27 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     int *startL = lenL + B4;                                       // region start; front end: input samples
     int *chanL = startL + ((a.begin || FRONT) ? B4 : 0);           // front end: channels
     long long *offL = reinterpret_cast<long long *>(chanL + (FRONT ? B4 : 0));   // packed: [B][2]
-    int *chunkW = reinterpret_cast<int *>(offL + (a.srcOff ? 2 * B4 : 0));
-    float *firW = reinterpret_cast<float *>(chunkW + 16);          // front end: the 2 lobes FIR taps
+    int *chunkW = reinterpret_cast<int *>(offL + (a.srcOff ? 2 * B4 : 0));   // [16], R2: [32] (upper half at 16)
+    float *firW = reinterpret_cast<float *>(chunkW + (R2 ? 32 : 16));   // front end: the 2 lobes FIR taps
     float *dspan = firW + (FRONT ? kFeTaps : 0);                   // front end: [wave][kFeSpan] inputs
-    const int segOff = static_cast<int>(dspan + (FRONT ? kWaves * kFeSpan : 0) - lds);
+    const int segOff = static_cast<int>(dspan + (FRONT ? NW * kFeSpan : 0) - lds);
 
     {
         const u32x4 *s1 = reinterpret_cast<const u32x4 *>(a.a1), *s2 = reinterpret_cast<const u32x4 *>(a.a2);
-        for (int i = tid; i < 256; i += kThreads) A1[i] = s1[i];
-        for (int i = tid; i < 512; i += kThreads) A2[i] = s2[i];
-        for (int i = tid; i < kN2 * kN1; i += kThreads) winP[i] = a.winP[i];
-        for (int i = tid; i < 2 * 256; i += kThreads) tw[i] = a.tw[i];
+        for (int i = tid; i < 256; i += NT) A1[i] = s1[i];
+        for (int i = tid; i < 512; i += NT) A2[i] = s2[i];
+        for (int i = tid; i < (R2 ? 2 : 1) * kN2 * kN1; i += NT) winP[i] = a.winP[i];
+        for (int i = tid; i < 2 * 256; i += NT) tw[i] = a.tw[i];
         const u32x4 *src = reinterpret_cast<const u32x4 *>(a.melFrag);
-        for (int i = tid; i < a.nPairs * 2 * 64; i += kThreads) melF[i] = src[i];
+        for (int i = tid; i < a.nPairs * 2 * 64; i += NT) melF[i] = src[i];
+        if (R2) {
+            for (int i = tid; i < 2 * 256; i += NT) tw[2 * 256 + i] = a.tw2[i];
+            for (int i = tid; i < 16 * NMT; i += NT) w256[i] = i < a.M ? a.w256[i] : 0.0f;
+        }
     }
     // per-clip metadata into LDS (the item loop then reads no scalar memory: scalar loads and LDS
     // reads share one completion counter, so a scalar load's latency would stall every LDS wait)
-    for (int b = tid; b < a.B; b += kThreads) {
+    for (int b = tid; b < a.B; b += NT) {
         if (FRONT) {
             // the resample launch's output length: ceil(samples * outRate / inRate) in FP32
             const int C = max(a.fDims[2 * b + 1], 1);
@@ -542,7 +616,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
             if (a.begin) startL[b] = start;
         }
     }
-    if (tid < 16) chunkW[tid] = a.pairOff[tid];
+    if (tid < (R2 ? 32 : 16)) chunkW[tid] = a.pairOff[tid];
     if (FRONT && tid < kFeTaps) {
         // an unclipped output's window values at offsets -lobes .. lobes - 1 (all four-tap groups:
         // the truncating lookup), as the resample launch computes them
@@ -555,11 +629,11 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     __syncthreads();
     // the tile plan: items per clip and their exclusive prefix sum in ts[0 .. B]
     {
-        __shared__ int wsum[kWaves];
+        __shared__ int wsum[NW];
         int carry = 0;
-        for (int base = 0; base < a.B; base += kThreads) {
+        for (int base = 0; base < a.B; base += NT) {
             const int b = base + tid;
-            const int n = b < a.B ? items_of(a, lenL[b]) : 0;
+            const int n = b < a.B ? items_of(a, lenL[b], FI) : 0;
             int v = n;
             for (int o = 1; o < 64; o <<= 1) {
                 const int u = __shfl_up(v, o);
@@ -568,7 +642,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
             if (lane == 63) wsum[wave] = v;
             __syncthreads();
             int before = 0, total = 0;
-            for (int w = 0; w < kWaves; w++) {
+            for (int w = 0; w < NW; w++) {
                 if (w < wave) before += wsum[w];
                 total += wsum[w];
             }
@@ -588,17 +662,24 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     auto describe = [&](int itemIdx, int b) {
         Item t;
         t.v = clip_of(a, b, lenL, startL, offL);
-        t.f0 = (itemIdx - ts[b]) * kFI;
+        t.f0 = (itemIdx - ts[b]) * FI;
         t.s0 = t.f0 * HOP - off0;
-        const int nfr = min(kFI, t.v.F - t.f0);
+        const int nfr = min(FI, t.v.F - t.f0);
         t.valid = (nfr - 1) * HOP + a.W;
         t.interior = t.s0 >= 1 && t.s0 + t.valid <= t.v.L && (t.valid & 3) == 0;
         t.aligned = ((reinterpret_cast<uintptr_t>(t.v.x + t.s0)) & 15) == 0;
         return t;
     };
-    // an interior item's samples requested into px/pp: lane l, load m of round r covers segment
-    // position q = 64 r + 4 (l / 4) + m, phases 4 (l % 4) .. + 3 (samples 16 q + 4 (l % 4) + e), and
-    // pp the sample before each group; nothing past valid is read
+    // the first of the four consecutive samples lane l stages in load m of round r: segment position
+    // q = 64 r + 4 (l / 4) + m, phases 4 (l % 4) .. + 3 (nfft 512: samples 16 q + 4 (l % 4) + e); nfft
+    // 1024: q = 32 r + 4 (l % 8) + m, phases 4 (l / 8) .. + 3 (samples 32 q + 4 (l / 8) + e), so the
+    // eight lanes of one LDS store cycle write one phase row at eight positions
+    auto sample_of = [](int r, int m, int lane) {
+        return R2 ? 1024 * r + 128 * (lane & 7) + 32 * m + 4 * (lane >> 3)
+                  : 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
+    };
+    // an interior item's samples requested into px/pp (sample_of), and pp the sample before each
+    // group; nothing past valid is read
     // (interior items only: every group index is clamped to the last valid group, so each load is
     // in bounds and unconditional -- no per-lane branch -- and groups past valid are zeroed after)
     auto prefetch = [&](const Item &t) {
@@ -608,7 +689,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         for (int r = 0; r < ROUNDS; r++) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const int i = 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
+                const int i = sample_of(r, m, lane);
                 const bool in = i < t.valid;
                 const float *p = base + min(i, t.valid - 4);
                 float4 q;
@@ -718,7 +799,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
     // are in flight together).
     auto stage = [&](const Item &t) {
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
-        float *seg = lds + segOff + static_cast<size_t>(lane >= 0 ? wave : 0) * 16 * PITCH;
+        float *seg = lds + segOff + static_cast<size_t>(lane >= 0 ? wave : 0) * SEGF;
         if (FRONT) produce(t, seg, lane);
         float y[ROUNDS][4][4];
         float mx = 0.0f;
@@ -731,7 +812,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                 for (int r = 0; r < ROUNDS; r++)
 #pragma unroll
                     for (int m = 0; m < 4; m++) {
-                        const int i = 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
+                        const int i = sample_of(r, m, lane);
                         const bool in = i < t.valid;
                         const int ic = min(i, t.valid - 4);
                         const float4 q = make_float4(seg[slot(ic)], seg[slot(ic + 1)], seg[slot(ic + 2)], seg[slot(ic + 3)]);
@@ -780,7 +861,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                 for (int m = 0; m < 4; m++) {
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        const int i = 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3) + e;
+                        const int i = sample_of(r, m, lane) + e;
                         const int j0 = t.s0 + i;
                         const bool out = j0 < 0 || j0 >= L;
                         int j = j0 < 0 ? -j0 : (j0 >= L ? 2 * (L - 1) - j0 : j0);
@@ -811,24 +892,25 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
             // a clip no longer than half a window (or empty): sample by sample with RPP's reflect map
             // (any number of bounces), straight into LDS, rescaled below
             const float *xs = FRONT ? dspan + wave * kFeSpan : t.v.x;
-            for (int i = lane; i < 16 * PITCH; i += 64) {
+            for (int i = lane; i < NPH * PITCH; i += 64) {
                 const float yv = (i < t.valid && t.v.L > 0) ? pe_at(xs, t.v.L, t.s0 + i, c, a.border, a.reflect) : 0.0f;
-                seg[(i & 15) * PITCH + (i >> 4)] = yv;
+                seg[row_at(i % NPH) + i / NPH] = yv;
                 mx = fmaxf(mx, fabsf(yv));
             }
             mx = wave_max(mx);
             const float sc = mx > 0.0f ? ldexpf(1.0f, min(9 - ilogbf(mx), 114)) : 1.0f;
-            for (int i = lane; i < 16 * PITCH; i += 64) seg[(i & 15) * PITCH + (i >> 4)] *= sc;
+            for (int i = lane; i < NPH * PITCH; i += 64) seg[row_at(i % NPH) + i / NPH] *= sc;
             return sc;
         }
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
-            const int q0 = 64 * r + 4 * (lane >> 2);
+            const int q0 = R2 ? 32 * r + 4 * (lane & 7) : 64 * r + 4 * (lane >> 2);
             if (q0 + 3 < PITCH) {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const float4 o = make_float4(y[r][0][e], y[r][1][e], y[r][2][e], y[r][3][e]);
-                    *reinterpret_cast<float4 *>(&seg[(4 * (lane & 3) + e) * PITCH + q0]) = o;
+                    const int row = R2 ? 4 * (lane >> 3) + e : 4 * (lane & 3) + e;
+                    *reinterpret_cast<float4 *>(&seg[row_at(row) + q0]) = o;
                 }
             }
         }
@@ -881,18 +963,22 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                 a1l[rt] = A1[(rt * 2 + 1) * 64 + l1];
             }
         }
+        // the lane's column: frame l % 16 (nfft 512), or frame l % 8 and parity (l / 8) % 2 (nfft
+        // 1024: the parity's samples are phase rows 2 n2 + p, the window's values w[32 n1 + 2 n2 + p])
         const float *rowBase, *winBase;
+        constexpr int RS1 = R2 ? 2 * PITCH + 4 : PITCH;   // floats from row n2 to row n2 + 1 of the column
         {
             const int l1 = opaque(static_cast<int>(threadIdx.x));
-            const int w1 = l1 >> 6, f1 = l1 & 15, g1 = (l1 >> 4) & 3;
-            rowBase = lds + segOff + w1 * 16 * PITCH + f1 * HQ + 8 * g1;
-            winBase = winP + 8 * g1;
+            const int w1 = l1 >> 6, g1 = (l1 >> 4) & 3;
+            const int f1 = R2 ? l1 & 7 : l1 & 15, p1 = (l1 >> 3) & 1;
+            rowBase = lds + segOff + w1 * SEGF + (R2 ? row_at(p1) : 0) + f1 * HQ + 8 * g1;
+            winBase = winP + (R2 ? p1 * kN2 * kN1 : 0) + 8 * g1;
         }
         float2 dv[2][4];
         float4 wv[2][2];
         auto load1 = [&](int n2, int sl) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) dv[sl][k] = *reinterpret_cast<const float2 *>(rowBase + n2 * PITCH + 2 * k);
+            for (int k = 0; k < 4; k++) dv[sl][k] = *reinterpret_cast<const float2 *>(rowBase + n2 * RS1 + 2 * k);
             wv[sl][0] = *reinterpret_cast<const float4 *>(winBase + n2 * kN1);
             wv[sl][1] = *reinterpret_cast<const float4 *>(winBase + n2 * kN1 + 4);
         };
@@ -920,10 +1006,21 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                 acc1[n2][rt] = mfma3_f16(a1h[rt], a1l[rt], sh[n2 & 1], slo[n2 & 1], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
             __builtin_amdgcn_sched_barrier(0);
         }
+        // nfft 1024, bin 512 of a parity's 512-point DFT: sum_n2 (-1)^n2 Y[n2][0] (row 0 of row tile 0:
+        // lane group 0), real; bin 256 of the frame is E^2 + O^2 of the two parities' values, then
+        // broadcast to every lane of the frame's columns
+        float p256 = 0.0f;
+        if (R2) {
+            float s = 0.0f;
+#pragma unroll
+            for (int n2 = 0; n2 < kN2; n2++) s += (n2 & 1) ? -acc1[n2][0][0] : acc1[n2][0][0];
+            const float so = swap8(s);
+            p256 = __shfl(fmaf(s, s, so * so), static_cast<int>(threadIdx.x) & 15);
+        }
 
         // the next item (its request went out at the top of this one) and its clip, a scalar search
         // in the plan (requests are granted in increasing order, so it is not before this clip)
-        const int kNext = kWaves + __builtin_amdgcn_readfirstlane(fetched);
+        const int kNext = NW + __builtin_amdgcn_readfirstlane(fetched);
         const int nextItem = kNext < nMine ? static_cast<int>(blockIdx.x) + kNext * G : T;
         const int clipNext = nextItem < T ? owner(nextItem, v.b) : 0;
 
@@ -974,8 +1071,30 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                     const u32x4 bl = {pcL[p][0], pcL[p][1], pcL[p][2], pcL[p][3]};
                     const f32x4 c0 = mfma3_f16(a2h[which][0], a2l[which][0], bh, bl, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
                     const f32x4 c1 = mfma3_f16(a2h[which][1], a2l[which][1], bh, bl, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-                    const float q0 = fmaf(c0[1], c0[1], c0[0] * c0[0]), q1 = fmaf(c0[3], c0[3], c0[2] * c0[2]);
-                    const float q2 = fmaf(c1[1], c1[1], c1[0] * c1[0]), q3 = fmaf(c1[3], c1[3], c1[2] * c1[2]);
+                    float q0, q1, q2, q3;
+                    if (R2) {
+                        // the frame's other parity holds the same bins in lane l ^ 8: |V z + y|^2 with
+                        // z this lane's value, y the other's and V the twiddle of the bin (its real
+                        // part negated for parity 1)
+                        const float4 t01 = tw2[(g2 * 16 + kk) * 2], t23 = tw2[(g2 * 16 + kk) * 2 + 1];
+                        const float sg = (l2 & 8) ? -1.0f : 1.0f;
+                        auto rad2 = [&](float zr, float zi, float vr, float vi) {
+                            const float yr = swap8(zr), yi = swap8(zi);
+                            vr *= sg;
+                            const float xr = fmaf(vr, zr, fmaf(-vi, zi, yr));
+                            const float xi = fmaf(vr, zi, fmaf(vi, zr, yi));
+                            return fmaf(xi, xi, xr * xr);
+                        };
+                        q0 = rad2(c0[0], c0[1], t01.x, t01.y);
+                        q1 = rad2(c0[2], c0[3], t01.z, t01.w);
+                        q2 = rad2(c1[0], c1[1], t23.x, t23.y);
+                        q3 = rad2(c1[2], c1[3], t23.z, t23.w);
+                    } else {
+                        q0 = fmaf(c0[1], c0[1], c0[0] * c0[0]);
+                        q1 = fmaf(c0[3], c0[3], c0[2] * c0[2]);
+                        q2 = fmaf(c1[1], c1[1], c1[0] * c1[0]);
+                        q3 = fmaf(c1[3], c1[3], c1[2] * c1[2]);
+                    }
                     if (which) {
                         P[0] = q0; P[16] = q1; P[32] = q2; P[48] = q3;
                     } else {
@@ -1014,6 +1133,9 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         // ---- banded mel on MFMA, pairs in their table order (chunk by chunk, tiles ascending): chunk
         //      c's operand is PH/PL[16 (c % 2) + 4 (c / 2) + 0..3]; each pair's weight fragments were
         //      requested two pairs ahead
+        //      (nfft 1024: each chunk's second run is the upper half's, on the power of the lane's
+        //      partner l ^ 8 -- the frame's bins 512 - u -- so the frame's parity-0 column gathers the
+        //      whole filter bank; then bin 256's weights)
         f32x4 mel[NMT];
 #pragma unroll
         for (int mt = 0; mt < NMT; mt++) mel[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1021,13 +1143,7 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         u32x4 wch = melL[0], wcl = melL[64];
         u32x4 wnh = melL[(min(1, nP - 1) * 2) * 64], wnl = melL[(min(1, nP - 1) * 2 + 1) * 64];
         int pn = 2;
-#pragma unroll
-        for (int ch = 0; ch < kChunks; ch++) {
-            const int cw = __builtin_amdgcn_readfirstlane(chunkW[ch]);   // read per item from LDS
-            const int t0 = cw & 255, t1 = (cw >> 8) & 255;
-            const int base = 16 * (ch & 1) + 4 * (ch >> 1);
-            const u32x4 bh = {PH[base], PH[base + 1], PH[base + 2], PH[base + 3]};
-            const u32x4 bl = {PL[base], PL[base + 1], PL[base + 2], PL[base + 3]};
+        auto run = [&](int t0, int t1, const u32x4 &bh, const u32x4 &bl) {
 #pragma unroll
             for (int mt = 0; mt < NMT; mt++) {
                 if (mt >= t0 && mt < t1) {
@@ -1040,6 +1156,38 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
                     pn++;
                 }
             }
+        };
+#pragma unroll
+        for (int ch = 0; ch < kChunks; ch++) {
+            const int cw = __builtin_amdgcn_readfirstlane(chunkW[ch]);   // read per item from LDS
+            const int base = 16 * (ch & 1) + 4 * (ch >> 1);
+            const u32x4 bh = {PH[base], PH[base + 1], PH[base + 2], PH[base + 3]};
+            const u32x4 bl = {PL[base], PL[base + 1], PL[base + 2], PL[base + 3]};
+            run(cw & 255, (cw >> 8) & 255, bh, bl);
+            if (R2) {
+                const int cw2 = __builtin_amdgcn_readfirstlane(chunkW[16 + ch]);
+                const int t0 = cw2 & 255, t1 = (cw2 >> 8) & 255;
+                if (t0 < t1) {   // (no partner swap for a chunk without pairs)
+                    u32x4 sh, sl;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        sh[e] = swap8u(bh[e]);
+                        sl[e] = swap8u(bl[e]);
+                    }
+                    run(t0, t1, sh, sl);
+                }
+            }
+        }
+        if (R2) {
+            const int g = (opaque(static_cast<int>(threadIdx.x)) >> 4) & 3;
+#pragma unroll
+            for (int mt = 0; mt < NMT; mt++) {
+                const float4 w = *reinterpret_cast<const float4 *>(w256 + 16 * mt + 4 * g);
+                mel[mt][0] = fmaf(w.x, p256, mel[mt][0]);
+                mel[mt][1] = fmaf(w.y, p256, mel[mt][1]);
+                mel[mt][2] = fmaf(w.z, p256, mel[mt][2]);
+                mel[mt][3] = fmaf(w.w, p256, mel[mt][3]);
+            }
         }
 
 
@@ -1048,10 +1196,10 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
         const float unscale = 1.0f / (scale * 4096.0f);
         const float melScale = (GENERAL && a.power == 1) ? unscale : unscale * unscale;
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
-        const int fr = lane & 15, g = lane >> 4;
+        const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
         const int f = f0 + fr;
         if (!GENERAL) {
-            if (f < v.F) {
+            if (f < v.F && !(R2 && (lane & 8))) {
                 // row 16 mt + 4 g + i: a scalar base per (mt, i) plus the lane's offset 4 g hs + f
                 const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
                 const float k = melScale * a.invRef;
@@ -1111,25 +1259,31 @@ __global__ void __launch_bounds__(kThreads, 1) logmel_fft_kernel(FftArgs a)
 }  // namespace
 
 // ------------------------------------------------------------------------- host side
-// a1 4 KB, a2 8 KB, winP 2 KB, tw 2 KB, then W floats for the window itself
-size_t fft_table_bytes(int W) { return 4096 + 8192 + 2048 + 2048 + sizeof(float) * static_cast<size_t>(W); }
-
-void fft_tables_carve(char *mem, int W, FftTables &t)
+// a1 4 KB, a2 8 KB, winP 2 KB (nfft 1024: 4 KB), tw 2 KB, nfft 1024: tw2 2 KB, then W floats for the
+// window itself
+size_t fft_table_bytes(int W, int nfft)
 {
+    return 4096 + 8192 + (nfft == 2 * kN ? 4096 + 2048 : 2048) + 2048 + sizeof(float) * static_cast<size_t>(W);
+}
+
+void fft_tables_carve(char *mem, int W, FftTables &t, int nfft)
+{
+    const bool r2 = nfft == 2 * kN;
     t.a1 = reinterpret_cast<uint16_t *>(mem);
     t.a2 = reinterpret_cast<uint16_t *>(mem + 4096);
     t.winP = reinterpret_cast<float *>(mem + 4096 + 8192);
-    t.tw = t.winP + kN2 * kN1;
+    t.tw = t.winP + (r2 ? 2 : 1) * kN2 * kN1;
+    t.tw2 = r2 ? t.tw + 512 : nullptr;
     t.mem = mem;
-    t.nfft = kN;
+    t.nfft = r2 ? 2 * kN : kN;
     t.W = W;
 }
 
 hipError_t build_fft_tables_at(hipStream_t st, int W, FftTables &t)
 {
-    float *win = t.tw + 512;
+    float *win = (t.tw2 ? t.tw2 : t.tw) + 512;
     hipLaunchKernelGGL(fft_hann_kernel, dim3(ceil_div(W, 256)), dim3(256), 0, st, W, win);
-    hipLaunchKernelGGL(fft_tables_kernel, dim3(8), dim3(256), 0, st, W, win, t.a1, t.a2, t.winP, t.tw);
+    hipLaunchKernelGGL(fft_tables_kernel, dim3(8), dim3(256), 0, st, W, t.nfft, win, t.a1, t.a2, t.winP, t.tw, t.tw2);
     return hipGetLastError();
 }
 
@@ -1137,8 +1291,8 @@ size_t mel_frag_bytes(int M) { return static_cast<size_t>((M + 15) / 16) * kChun
 
 hipError_t build_mel_frag_at(hipStream_t st, const MelTables &mt, MelFrag &f)
 {
-    hipLaunchKernelGGL(mel_frag_kernel, dim3(1), dim3(256), 0, st, mt.dense, mt.filterBins, mt.M, mt.numBins, f.frag,
-                       f.pairOff);
+    hipLaunchKernelGGL(mel_frag_kernel, dim3(1), dim3(256), 0, st, mt.dense, mt.filterBins, mt.M, mt.numBins,
+                       f.w256 ? 1 : 0, f.frag, f.pairOff, f.w256);
     return hipGetLastError();
 }
 
@@ -1150,9 +1304,9 @@ const FftTables *get_fft_tables(rppHandle *h, int nfft, int W)
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
     char *mem = nullptr;
-    if (hipMalloc(&mem, fft_table_bytes(W)) != hipSuccess) return nullptr;
+    if (hipMalloc(&mem, fft_table_bytes(W, nfft)) != hipSuccess) return nullptr;
     FftTables t;
-    fft_tables_carve(mem, W, t);
+    fft_tables_carve(mem, W, t, nfft);
     if (build_fft_tables_at(h->stream, W, t) != hipSuccess) return nullptr;
     h->fft[key] = t;
     return &h->fft[key];
@@ -1165,8 +1319,11 @@ const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
     MelFrag f;
-    if (hipMalloc(&f.frag, mel_frag_bytes(mt->M)) != hipSuccess) return nullptr;
-    if (hipMalloc(&f.pairOff, sizeof(int) * 257) != hipSuccess) return nullptr;
+    const bool r2 = mt->numBins == kN + 1;   // nfft 1024 (513 bins): the two halves and bin 256
+    const size_t fb = mel_frag_bytes(mt->M);
+    if (hipMalloc(&f.frag, fb * (r2 ? 2 : 1)) != hipSuccess) return nullptr;
+    if (hipMalloc(&f.pairOff, sizeof(int) * (257 + (r2 ? 16 * kMaxMT : 0))) != hipSuccess) return nullptr;
+    if (r2) f.w256 = reinterpret_cast<float *>(f.pairOff + 257);
     if (build_mel_frag_at(h->stream, *mt, f) != hipSuccess) return nullptr;
     int n[2] = {0, 0};   // [255] tiles of a chunk not contiguous, [256] pair count
     if (hipMemcpyAsync(n, f.pairOff + 255, sizeof(n), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return nullptr;
@@ -1176,27 +1333,43 @@ const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string
     return &h->melFrag[key];
 }
 
-// LDS bytes of the factored kernel for hop H, B clips and nPairs mel pairs.
-size_t fft_lds_bytes(int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front)
+// LDS bytes of the factored kernel for nfft, hop H, B clips and nPairs mel pairs.
+size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front)
 {
-    const int pitch = pitch_for(((kFI - 1) * H + kN) / 16);
-    if (pitchOut) *pitchOut = pitch;
     const size_t B4 = static_cast<size_t>((B + 1 + 3) / 4 * 4);
     const size_t perClip = sizeof(int) * B4 * (2 + ((begin || front) ? 1 : 0) + (front ? 1 : 0)) +
                            (packed ? sizeof(long long) * 2 * B4 : 0);
+    if (nfft == 2 * kN) {
+        if (pitchOut) *pitchOut = kPitchR2;
+        // a1, a2, winP [2][16][32], tw, tw2, bin 256's weights, both halves' fragments, the plan,
+        // 32 chunk words, the segments
+        return 4096 + 8192 + sizeof(float) * (2 * kN2 * kN1 + 512 + 512 + 16 * kMaxMT) +
+               static_cast<size_t>(nPairs) * 2048 + perClip + sizeof(int) * 32 +
+               sizeof(float) * kWavesR2 * static_cast<size_t>(32 * kPitchR2 + 64);
+    }
+    const int pitch = pitch_for(((kFI - 1) * H + kN) / 16);
+    if (pitchOut) *pitchOut = pitch;
     const size_t frontBytes = front ? sizeof(float) * (kFeTaps + kWaves * kFeSpan) : 0;
     return 4096 + 8192 + sizeof(float) * (kN2 * kN1 + 512) + static_cast<size_t>(nPairs) * 2048 + perClip +
            sizeof(int) * 16 + frontBytes + sizeof(float) * kWaves * 16 * static_cast<size_t>(pitch);
 }
 
-// nfft 512 in fp16x3 with W <= 512, a hop that is a multiple of 32 with the 16-frame segment within
-// three staging rounds (H <= 160), at most 128 filters and 4096 clips (the tile plan lives in LDS)
+// fp16x3 with nfft 512, W <= 512 and hop 160, or nfft 1024, W <= 1024 and hop 256 (the hops the
+// kernel is instantiated for), at most 128 filters and 4096 clips (the tile plan lives in LDS)
 bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int B)
 {
     const int nfft = p->nfft ? p->nfft : p->windowLength;
-    if (precision != RPPT_LOGMEL_FP16X3 || nfft != kN || p->windowLength > kN || p->windowLength <= 0) return false;
-    if (p->windowStep != 160) return false;   // the hop the kernel is instantiated for
+    if (precision != RPPT_LOGMEL_FP16X3 || (nfft != kN && nfft != 2 * kN) || p->windowLength > nfft ||
+        p->windowLength <= 0)
+        return false;
+    if (p->windowStep != (nfft == kN ? 160 : 256)) return false;
     return p->numFilter <= 16 * kMaxMT && B <= 4096;
+}
+
+bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool callerMem, bool frontEnd)
+{
+    const float minRatio = static_cast<float>(std::pow(10.0, static_cast<double>(p->cutOffDB / p->multiplier)));
+    return p->power == 2 && !ntf && !clipRef && !callerMem && !frontEnd && minRatio >= 1.17549435e-38f;
 }
 
 bool fft_front_eligible(int k, int lobes) { return k >= 2 && k <= kFeMaxK && 2 * lobes == kFeTaps; }
@@ -1207,8 +1380,10 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     int pitch = 0;
     const bool front = L.fe != nullptr;
     if (front && (L.begin || L.srcOff)) return RPP_ERROR_NOT_IMPLEMENTED;
-    const size_t bytes = fft_lds_bytes(p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr, &pitch,
-                                       front);
+    const int nfft = L.ft->nfft;
+    const bool r2 = nfft == 2 * kN;
+    const size_t bytes = fft_lds_bytes(nfft, p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr,
+                                       &pitch, front);
     if (bytes > static_cast<size_t>(h->maxLdsOptin)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
     FftArgs a{};
     a.src = L.src;
@@ -1242,6 +1417,8 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     a.melFrag = L.mf->frag;
     a.pairOff = L.mf->pairOff;
     a.nPairs = L.mf->nPairs;
+    a.tw2 = L.ft->tw2;
+    a.w256 = L.mf->w256;
     a.pitch = pitch;
     if (front) {
         a.src = L.fe->src;   // only offset, never read: the kernel reads fsrc
@@ -1269,8 +1446,9 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
                                                                         p->windowLength, p->windowStep, p->centerWindows != 0),
                                                             0),
                                                    L.dstOff ? 0x7fffffffLL : static_cast<long long>(L.frameCap));
-        const long long items = static_cast<long long>(L.B) * ((maxF + kFI - 1) / kFI);
-        grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, (items + kWaves - 1) / kWaves)));
+        const int fi = r2 ? 8 : kFI, nw = r2 ? kWavesR2 : kWaves;
+        const long long items = static_cast<long long>(L.B) * ((maxF + fi - 1) / fi);
+        grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, (items + nw - 1) / nw)));
     }
     const bool general = a.power != 2 || a.ntf || a.clipMax || !(a.minRatio >= 1.17549435e-38f);
     // the front end runs with the common launch only (the caller falls back to two launches)
@@ -1283,6 +1461,22 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     } while (0)
     const int fk = front ? L.fe->k : 0;
     if (front && fk != 2 && fk != 3) return RPP_ERROR_NOT_IMPLEMENTED;
+    if (r2) {
+        if (general || front || p->windowStep != 256 || !a.tw2 || !a.w256) return RPP_ERROR_NOT_IMPLEMENTED;
+#define RPPA_FFT2_LAUNCH(NMT)                                                                                  \
+    do {                                                                                                       \
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_fft_kernel<NMT, 256, false, 0, 1024>), \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));        \
+        hipLaunchKernelGGL((logmel_fft_kernel<NMT, 256, false, 0, 1024>), dim3(grid), dim3(64 * kWavesR2), bytes, \
+                           h->stream, a);                                                                      \
+    } while (0)
+        if (nMT <= 5) RPPA_FFT2_LAUNCH(5);
+        else RPPA_FFT2_LAUNCH(8);
+#undef RPPA_FFT2_LAUNCH
+        RPPA_HIP(hipGetLastError());
+        return RPP_SUCCESS;
+    }
+    if (p->windowStep != 160) return RPP_ERROR_NOT_IMPLEMENTED;
     if (nMT <= 5) {
         if (fk == 3) RPPA_FFT_LAUNCH(5, false, 3);
         else if (fk == 2) RPPA_FFT_LAUNCH(5, false, 2);

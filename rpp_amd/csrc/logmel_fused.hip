@@ -1510,12 +1510,13 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const FftTables *ft = nullptr;
     const MelFrag *mf = nullptr;
     if (fft_eligible(p, precision, B) && !(h->tiles.blocksPerCU > 0 && h->tiles.framesPerTile > 0 &&
-                                            h->tiles.framesPerTile != 16)) {
+                                            h->tiles.framesPerTile != 16) &&
+        (nfft == 512 || fft_r2_launchable(p, ntf, clipRef, mem != nullptr, fe != nullptr))) {
         ft = mem ? mem->ft : get_fft_tables(h, nfft, p->windowLength);
         mf = mem ? mem->mf : get_mel_frag(h, mt, mel_key(p, K));
         if (!ft || !mf || mf->nPairs <= 0 ||
-            fft_lds_bytes(p->windowStep, B, mf->nPairs, begin != nullptr, srcOff != nullptr, nullptr, fe != nullptr) >
-                static_cast<size_t>(h->maxLdsOptin)) {
+            fft_lds_bytes(nfft, p->windowStep, B, mf->nPairs, begin != nullptr, srcOff != nullptr, nullptr,
+                          fe != nullptr) > static_cast<size_t>(h->maxLdsOptin)) {
             ft = nullptr;
             mf = nullptr;
         }
@@ -2046,7 +2047,7 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr
         MelFrag mf;
         bool hasFft = false;
         size_t end = used;
-        if (fft_eligible(p, precision, 1)) {
+        if (fft_eligible(p, precision, 1) && nfft == 512) {   // (nfft 1024's factored launch needs handle tables)
             const size_t e2 = layout_fft_tables(sd.ptr, used, p->windowLength, p->numFilter, ft, mf);
             if (e2) {
                 int n[2] = {0, 0};   // [255] tiles of a chunk not contiguous, [256] pair count
