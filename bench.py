@@ -30,10 +30,19 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# revisions of the fused kernels (the factored-DFT kernel runs at 16 frames per tile, the dense one at
-# 32/64); recorded traffic (profiles/*.md `traffic:` lines) applies only to the revision it was measured on
-KERNEL_REVISION = {16: "fft32x16-wave"}
+# revisions of the fused kernels (the factored-DFT kernel is selected by 16 frames per tile, the dense
+# one runs at 32/64); recorded traffic (profiles/*.md `traffic:` lines) applies only to the revision it
+# was measured on
+FACTORED_REVISION = {512: "fft32x16-wave", 1024: "fft2x32x16-radix2-wave"}
 DENSE_REVISION = "mirror-il2-selfplan"
+# (16-mel tile, 32-bin chunk) pairs the factored kernel's mel occupies per work item: the 80-mel
+# Slaney bank at nfft 512 / 16 kHz, and the 128-mel bank at nfft 1024 / 44.1 kHz (14 pairs over bins
+# 0-255 and 9 over bins 257-512)
+FACTORED_MEL_PAIRS = {512: 12, 1024: 23}
+
+
+def kernel_revision(frames_per_tile, nfft):
+    return FACTORED_REVISION.get(nfft, DENSE_REVISION) if frames_per_tile == 16 else DENSE_REVISION
 MI355X_PEAK_TFLOPS = {"fp16x3": 2500.0, "bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 MI355X_HBM_GBS = 8000.0
 
@@ -42,12 +51,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def recorded_traffic(workload, precision, tiles):
+def recorded_traffic(workload, precision, tiles, nfft):
     """HBM bytes per launch of the fused kernel for this configuration, from the rocprofv3
     FETCH_SIZE/WRITE_SIZE runs summarised under profiles/ (a `traffic:` line per configuration,
     counters already scaled by the copy-kernel calibration), or None when none matches."""
     import glob
-    key = {"kernel": KERNEL_REVISION.get(tiles[0], DENSE_REVISION), "workload": str(workload), "precision": precision,
+    key = {"kernel": kernel_revision(tiles[0], nfft), "workload": str(workload), "precision": precision,
            "frames_per_tile": str(tiles[0]), "blocks_per_cu": str(tiles[1])}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
@@ -418,7 +427,7 @@ def main() -> int:
     flop_per_frame = 2 * feat["W"] * 2 * K + 2 * K * feat["nmel"]
     flops = float(frames.sum()) * flop_per_frame
     achieved_tf = flops / kernel_s / 1e12
-    rec = recorded_traffic(a.workload, a.precision, tuned[a.precision][0])
+    rec = recorded_traffic(a.workload, a.precision, tuned[a.precision][0], feat["nfft"])
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MI355X_PEAK_TFLOPS[a.precision],
                 "unit": "TFLOP/s", "frac": round(achieved_tf / MI355X_PEAK_TFLOPS[a.precision], 4),
                 "traffic": rec[0] if rec else None, "flop_per_launch": flops, "kernel_ms": round(kernel_s * 1e3, 4),
@@ -428,14 +437,15 @@ def main() -> int:
         roofline["traffic_source"] = "profiles/" + rec[1]
         roofline["hbm_gbs_at_traffic"] = round(rec[0] / kernel_s / 1e9, 1)
     ft_chosen = tuned[a.precision][0][0]
-    roofline["kernel"] = KERNEL_REVISION.get(ft_chosen, DENSE_REVISION)
-    if ft_chosen == 16 and a.precision == "fp16x3":
-        # the factored DFT executes fewer matrix FLOPs than the dense count above: per 16-frame item
-        # 96 + 96 fp16 products (stage 1, stage 2) and one bf16 triple for each of the 12 (16-mel tile,
-        # 32-bin chunk) pairs the 80-mel Slaney bank at nfft 512 occupies, each v_mfma_*_16x16x32 =
+    roofline["kernel"] = kernel_revision(ft_chosen, feat["nfft"])
+    if ft_chosen == 16 and a.precision == "fp16x3" and feat["nfft"] in FACTORED_MEL_PAIRS:
+        # the factored DFT executes fewer matrix FLOPs than the dense count above: per work item
+        # (16 frames at nfft 512, 8 at nfft 1024) 96 + 96 fp16 products (stage 1, stage 2) and one
+        # bf16 triple per (16-mel tile, 32-bin chunk) pair of the bank, each v_mfma_*_16x16x32 =
         # 16,384 FLOP; split passes included
-        items = int(((frames + 15) // 16).sum())
-        mfma = items * (192 + 3 * 12) * 16384.0
+        fi = 16 if feat["nfft"] == 512 else 8
+        items = int(((frames + fi - 1) // fi).sum())
+        mfma = items * (192 + 3 * FACTORED_MEL_PAIRS[feat["nfft"]]) * 16384.0
         roofline["mfma_flop_executed_per_launch"] = mfma
         roofline["mfma_pipe_frac"] = round(mfma / kernel_s / 1e12 / MI355X_PEAK_TFLOPS[a.precision], 4)
 
@@ -549,7 +559,9 @@ def main() -> int:
                                 2: "batch 256 mono 44.1 kHz 2-10 s nfft 1024"}[a.workload],
                    "batch": B, "audio_seconds": round(audio_s, 2), "frames": int(frames.sum()),
                    "nfft": feat["nfft"], "window": feat["W"], "hop": H, "mels": feat["nmel"],
-                   "precision": a.precision, "launch": ("persistent fused kernel, factored DFT (32 x 16), hipGraph replay" if tuned[a.precision][0][0] == 16
+                   "precision": a.precision, "launch": (("persistent fused kernel, factored DFT (32 x 16), hipGraph replay" if feat["nfft"] == 512 else
+                               "persistent fused kernel, factored DFT (radix 2 over 32 x 16), hipGraph replay")
+                              if tuned[a.precision][0][0] == 16
                               else "persistent fused kernel, folded dense DFT, hipGraph replay"),
                    "tiles": {"frames_per_tile": tuned[a.precision][0][0], "blocks_per_cu": tuned[a.precision][0][1]}},
         "roofline": roofline,

@@ -340,10 +340,12 @@ RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
 
 /* Launch shape of the fused kernel; 0 fields mean "default". */
 typedef struct {
-    Rpp32s framesPerTile;   /* frames per work item: 16 selects the factored-DFT kernel (fp16x3, nfft 512,
-                               hop 160; one wave per item); the dense MFMA kernel runs 64 frames (nfft
-                               <= 512; at nfft 512, 32 on request) or 32 (above); the FP32 kernel 32;
-                               0 = default (factored where it applies) */
+    Rpp32s framesPerTile;   /* 16 selects the factored-DFT kernel where it applies (fp16x3; nfft 512
+                               with hop 160, 16 frames per wave item; nfft 1024 with hop 256, radix 2,
+                               8 frames per wave item, power 2, NFT, referenceMagnitude != 0); the
+                               dense MFMA kernel runs 64 frames (nfft <= 512; at nfft 512, 32 on
+                               request) or 32 (above); the FP32 kernel 32; 0 = default (factored
+                               where it applies) */
     Rpp32s blocksPerCU;     /* persistent workgroups per compute unit (dense kernels; default 2) */
 } RpptLogMelTiles;
 RppStatus rppt_logmel_set_tiles(rppHandle_t rppHandle, const RpptLogMelTiles *tiles);

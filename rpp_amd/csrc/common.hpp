@@ -206,16 +206,18 @@ bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int 
 bool fft_front_eligible(int k, int lobes);
 size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false);
 // nfft 1024 runs only the common launch (power 2, NFT output, a fixed dB reference, a normal cut-off
-// ratio) on tables the handle owns
-bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool callerMem, bool frontEnd);
+// ratio) on clips given at the output rate
+bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool frontEnd);
 const FftTables *get_fft_tables(rppHandle *h, int nfft, int W);
 const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string &key);
 // the same tables in caller-provided memory (no allocation): fft_table_bytes() bytes at `mem`
-// (256-byte aligned) for W samples of window; mel_frag_bytes(M) + 257 ints for the fragments
+// (256-byte aligned) for W samples of window; mel_frag_bytes(M, nfft) for the fragments and
+// mel_frag_word_bytes(nfft) for the chunk words (and nfft 1024's bin-256 weights after them)
 size_t fft_table_bytes(int W, int nfft = 512);
 void fft_tables_carve(char *mem, int W, FftTables &t, int nfft = 512);
 hipError_t build_fft_tables_at(hipStream_t st, int W, FftTables &t);
-size_t mel_frag_bytes(int M);
+size_t mel_frag_bytes(int M, int nfft = 512);
+size_t mel_frag_word_bytes(int nfft = 512);
 hipError_t build_mel_frag_at(hipStream_t st, const MelTables &mt, MelFrag &f);
 RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L);
 

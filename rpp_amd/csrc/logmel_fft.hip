@@ -556,6 +556,9 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
     static_assert(!R2 || (FRONT == 0 && !GENERAL), "the radix-2 launch is the common one");
     static_assert(!R2 || PITCH >= SEGN / NPH, "nfft 1024 segment rows");
     constexpr int ROUNDS = (SEGN + 1023) / 1024;
+    // staging rounds requested one item ahead (nfft 1024: the first only -- its stage-2 combine leaves
+    // no registers to hold the others across the mel phase; they are requested at staging)
+    constexpr int PFR = R2 ? 1 : ROUNDS;
     static_assert(HOP % 32 == 0 && ROUNDS <= 3, "segment geometry");
     static_assert(!FRONT || PITCH - SEGN / NPH >= 2, "two spare columns for the front end's predecessors");
     extern __shared__ __attribute__((aligned(16))) float lds[];
@@ -678,15 +681,16 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
         return R2 ? 1024 * r + 128 * (lane & 7) + 32 * m + 4 * (lane >> 3)
                   : 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
     };
-    // an interior item's samples requested into px/pp (sample_of), and pp the sample before each
-    // group; nothing past valid is read
+    // an interior item's samples of staging rounds [r0, r1) requested into px/pp (sample_of), and pp
+    // the sample before each group; nothing past valid is read
     // (interior items only: every group index is clamped to the last valid group, so each load is
     // in bounds and unconditional -- no per-lane branch -- and groups past valid are zeroed after)
-    auto prefetch = [&](const Item &t) {
+    auto prefetch = [&](const Item &t, int r0, int r1) {
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
         const float *base = t.v.x + t.s0;
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
+            if (r < r0 || r >= r1) continue;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const int i = sample_of(r, m, lane);
@@ -806,6 +810,7 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
         const float c = a.coeff;
         float scale;
         if (t.interior) {
+            if (!FRONT && PFR < ROUNDS) prefetch(t, PFR, ROUNDS);   // the rounds not requested ahead
             if (FRONT) {
                 // the values prefetch() would have loaded, from the slots
 #pragma unroll
@@ -929,18 +934,28 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
 
     // each item is described, and an interior item's samples requested, during its predecessor's mel
     // phase; it is staged at the top of its own iteration
-    // Work distribution: workgroup b owns items b, b + G, b + 2G, ... (G workgroups; neighbouring
-    // items run at the same time on different compute units).  Its waves take the first eight in wave
-    // order and every later one from a counter in LDS, requested one item ahead, so a wave that runs
-    // fast (an easier SIMD partner, fewer clip-edge items) takes more of them.
+    // Work distribution: workgroup b owns slots j = b, b + G, b + 2G, ... (G workgroups).  Its waves
+    // take the first NW in wave order and every later one from a counter in LDS, requested one item
+    // ahead, so a wave that runs fast (an easier SIMD partner, fewer clip-edge items) takes more of
+    // them.  Workgroups go to the eight XCDs in turn (b % 8), so with G a multiple of 8 slot j runs on
+    // XCD j % 8; the slots of one XCD take a contiguous run of items, j / 8 into it: the items running
+    // at once on an XCD are neighbours (a clip's consecutive frames), which share their window halo
+    // and the 128-byte lines of their output rows in that XCD's L2.  (A workgroup's items ascend
+    // either way, which the clip search relies on.)
     const int G = static_cast<int>(gridDim.x);
     const int nMine = T > static_cast<int>(blockIdx.x) ? (T - static_cast<int>(blockIdx.x) + G - 1) / G : 0;
+    const bool xcdRuns = (G & 7) == 0;
+    auto item_of = [&](int j) {
+        if (!xcdRuns) return j;
+        const int q = T >> 3, r = T & 7, x = j & 7;
+        return x * q + min(x, r) + (j >> 3);
+    };
     int k = wave;   // this wave's item, as an index into the workgroup's items
-    int item = static_cast<int>(blockIdx.x) + k * G;
+    int item = item_of(static_cast<int>(blockIdx.x) + k * G);
     Item cur{};
     if (k < nMine) {
         cur = describe(item, owner(item, 0));
-        if (!FRONT && cur.interior) prefetch(cur);
+        if (!FRONT && cur.interior) prefetch(cur, 0, PFR);
     }
 
     while (k < nMine) {
@@ -1021,7 +1036,7 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
         // the next item (its request went out at the top of this one) and its clip, a scalar search
         // in the plan (requests are granted in increasing order, so it is not before this clip)
         const int kNext = NW + __builtin_amdgcn_readfirstlane(fetched);
-        const int nextItem = kNext < nMine ? static_cast<int>(blockIdx.x) + kNext * G : T;
+        const int nextItem = kNext < nMine ? item_of(static_cast<int>(blockIdx.x) + kNext * G) : T;
         const int clipNext = nextItem < T ? owner(nextItem, v.b) : 0;
 
         // ---- twiddle, split, transpose and stage 2, one slot (rt, pair) at a time: the slot's
@@ -1127,7 +1142,7 @@ __global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) lo
         Item nxt{};
         if (nextItem < T) {
             nxt = describe(nextItem, clipNext);
-            if (!FRONT && nxt.interior) prefetch(nxt);
+            if (!FRONT && nxt.interior) prefetch(nxt, 0, PFR);
         }
 
         // ---- banded mel on MFMA, pairs in their table order (chunk by chunk, tiles ascending): chunk
@@ -1287,7 +1302,11 @@ hipError_t build_fft_tables_at(hipStream_t st, int W, FftTables &t)
     return hipGetLastError();
 }
 
-size_t mel_frag_bytes(int M) { return static_cast<size_t>((M + 15) / 16) * kChunks * 2048; }
+size_t mel_frag_bytes(int M, int nfft)
+{
+    return static_cast<size_t>((M + 15) / 16) * kChunks * 2048 * (nfft == 2 * kN ? 2 : 1);
+}
+size_t mel_frag_word_bytes(int nfft) { return sizeof(int) * 257 + (nfft == 2 * kN ? sizeof(float) * 16 * kMaxMT : 0); }
 
 hipError_t build_mel_frag_at(hipStream_t st, const MelTables &mt, MelFrag &f)
 {
@@ -1319,10 +1338,10 @@ const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
     MelFrag f;
-    const bool r2 = mt->numBins == kN + 1;   // nfft 1024 (513 bins): the two halves and bin 256
-    const size_t fb = mel_frag_bytes(mt->M);
-    if (hipMalloc(&f.frag, fb * (r2 ? 2 : 1)) != hipSuccess) return nullptr;
-    if (hipMalloc(&f.pairOff, sizeof(int) * (257 + (r2 ? 16 * kMaxMT : 0))) != hipSuccess) return nullptr;
+    const int nfft = mt->numBins == kN + 1 ? 2 * kN : kN;   // nfft 1024 (513 bins): the two halves and bin 256
+    const bool r2 = nfft == 2 * kN;
+    if (hipMalloc(&f.frag, mel_frag_bytes(mt->M, nfft)) != hipSuccess) return nullptr;
+    if (hipMalloc(&f.pairOff, mel_frag_word_bytes(nfft)) != hipSuccess) return nullptr;
     if (r2) f.w256 = reinterpret_cast<float *>(f.pairOff + 257);
     if (build_mel_frag_at(h->stream, *mt, f) != hipSuccess) return nullptr;
     int n[2] = {0, 0};   // [255] tiles of a chunk not contiguous, [256] pair count
@@ -1366,10 +1385,10 @@ bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int 
     return p->numFilter <= 16 * kMaxMT && B <= 4096;
 }
 
-bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool callerMem, bool frontEnd)
+bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool frontEnd)
 {
     const float minRatio = static_cast<float>(std::pow(10.0, static_cast<double>(p->cutOffDB / p->multiplier)));
-    return p->power == 2 && !ntf && !clipRef && !callerMem && !frontEnd && minRatio >= 1.17549435e-38f;
+    return p->power == 2 && !ntf && !clipRef && !frontEnd && minRatio >= 1.17549435e-38f;
 }
 
 bool fft_front_eligible(int k, int lobes) { return k >= 2 && k <= kFeMaxK && 2 * lobes == kFeTaps; }

@@ -1511,7 +1511,7 @@ RppStatus logmel_launch(rppHandle *h, const RpptLogMelParams *p, RpptLogMelPreci
     const MelFrag *mf = nullptr;
     if (fft_eligible(p, precision, B) && !(h->tiles.blocksPerCU > 0 && h->tiles.framesPerTile > 0 &&
                                             h->tiles.framesPerTile != 16) &&
-        (nfft == 512 || fft_r2_launchable(p, ntf, clipRef, mem != nullptr, fe != nullptr))) {
+        (nfft == 512 || fft_r2_launchable(p, ntf, clipRef, fe != nullptr))) {
         ft = mem ? mem->ft : get_fft_tables(h, nfft, p->windowLength);
         mf = mem ? mem->mf : get_mel_frag(h, mt, mel_key(p, K));
         if (!ft || !mf || mf->nPairs <= 0 ||
@@ -1960,14 +1960,15 @@ size_t layout_tables(char *base, int nfft, int W, int splitKind, bool wantBasis,
 }
 
 // The factored kernel's tables after the others (from byte `o`); returns the new end, 0 if past the array.
-size_t layout_fft_tables(char *base, size_t o, int W, int M, FftTables &ft, MelFrag &mf)
+size_t layout_fft_tables(char *base, size_t o, int nfft, int W, int M, FftTables &ft, MelFrag &mf)
 {
-    fft_tables_carve(base + o, W, ft);
-    o += align256(fft_table_bytes(W));
+    fft_tables_carve(base + o, W, ft, nfft);
+    o += align256(fft_table_bytes(W, nfft));
     mf.frag = reinterpret_cast<uint16_t *>(base + o);
-    o += align256(mel_frag_bytes(M));
+    o += align256(mel_frag_bytes(M, nfft));
     mf.pairOff = reinterpret_cast<int *>(base + o);
-    o += align256(sizeof(int) * 257);
+    mf.w256 = nfft == 1024 ? reinterpret_cast<float *>(mf.pairOff + 257) : nullptr;
+    o += align256(mel_frag_word_bytes(nfft));
     return o <= kScratchBytes ? o : 0;
 }
 
@@ -2047,8 +2048,8 @@ RPPA_EXPORT RppStatus rppt_logmel_fused_noalloc_gpu(RppPtr_t srcPtr, RpptDescPtr
         MelFrag mf;
         bool hasFft = false;
         size_t end = used;
-        if (fft_eligible(p, precision, 1) && nfft == 512) {   // (nfft 1024's factored launch needs handle tables)
-            const size_t e2 = layout_fft_tables(sd.ptr, used, p->windowLength, p->numFilter, ft, mf);
+        if (fft_eligible(p, precision, 1)) {
+            const size_t e2 = layout_fft_tables(sd.ptr, used, nfft, p->windowLength, p->numFilter, ft, mf);
             if (e2) {
                 int n[2] = {0, 0};   // [255] tiles of a chunk not contiguous, [256] pair count
                 if (build_fft_tables_at(h->stream, p->windowLength, ft) != hipSuccess ||
