@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 # revisions of the fused kernels (the factored-DFT kernel is selected by 16 frames per tile, the dense
 # one runs at 32/64); recorded traffic (profiles/*.md `traffic:` lines) applies only to the revision it
 # was measured on
-FACTORED_REVISION = {512: "fft32x16-wave", 1024: "fft2x32x16-radix2-wave"}
+FACTORED_REVISION = {512: "fft32x16-xcd", 1024: "fft2x32x16-radix2-xcd"}
 DENSE_REVISION = "mirror-il2-selfplan"
 # (16-mel tile, 32-bin chunk) pairs the factored kernel's mel occupies per work item: the 80-mel
 # Slaney bank at nfft 512 / 16 kHz, and the 128-mel bank at nfft 1024 / 44.1 kHz (14 pairs over bins
@@ -335,11 +335,12 @@ def main() -> int:
     if a.front_end_only:
         print(json.dumps({"front_end": front_end_measurement(api, synth, torch, a.precision, steps=a.steps)}))
         return 0
+    f1 = synth.make(1).params   # the trim and serving side lines run workload 1's 16 kHz features
+    params16k = api.logmel_params(nfft=f1["nfft"], window_length=f1["W"], window_step=f1["H"],
+                                  sample_rate=f1["sample_rate"], min_freq=f1["fmin"], max_freq=f1["fmax"],
+                                  num_filter=f1["nmel"])
     if a.trim_only:
-        f = synth.make(1).params
-        p = api.logmel_params(nfft=f["nfft"], window_length=f["W"], window_step=f["H"], sample_rate=f["sample_rate"],
-                              min_freq=f["fmin"], max_freq=f["fmax"], num_filter=f["nmel"])
-        print(json.dumps({"trim_workload1": trim_measurement(api, synth, torch, p, a.precision, steps=a.steps)}))
+        print(json.dumps({"trim_workload1": trim_measurement(api, synth, torch, params16k, a.precision, steps=a.steps)}))
         return 0
     bt = synth.make(a.workload)
     feat = bt.params
@@ -528,7 +529,7 @@ def main() -> int:
     if not a.no_side:
         side["front_end_workload4"] = front_end_measurement(api, synth, torch, a.precision, steps=20)
         log(f"front end: {side['front_end_workload4']}")
-        side["trim_workload1"] = trim_measurement(api, synth, torch, params, a.precision)
+        side["trim_workload1"] = trim_measurement(api, synth, torch, params16k, a.precision)
         log(f"trim: {side['trim_workload1']}")
         for wl in (0, 2):
             if wl != a.workload:
@@ -536,7 +537,7 @@ def main() -> int:
                 log(f"workload {wl}: {side[f'workload{wl}']}")
     serving = None
     if not a.no_side and a.serve_seconds > 0:
-        serving = serve_measurement(api, synth, params, a.precision, a.serve_seconds)
+        serving = serve_measurement(api, synth, params16k, a.precision, a.serve_seconds)
         log(f"serving: {serving}")
 
     line = {
