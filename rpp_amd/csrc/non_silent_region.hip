@@ -28,10 +28,16 @@ namespace {
 
 constexpr int kChunk = 32;          // samples per lane per prefetch step (three steps in flight)
 constexpr int kVecChunk = 16;       // the same with 16-byte loads
+constexpr int kVecSteps = 6;        // 16-byte-load chunks in flight per lane (3: 8% slower)
 constexpr int kSeg = 192;           // outputs between checkpoints of the running sum
 constexpr int kPass1Threads = 64;   // one wave per workgroup spreads the few lanes over CUs
+// rows (reset blocks) per pass-1 wave: 32 of the 64 lanes.  Measured on workload 1 (5,378 rows of up
+// to 10,240 samples over 512 x 39 lanes, block-major): 64 rows per wave 228.6 us, 32 219.4 us, 16 331 us, 8 515 us; each wave's
+// instruction stream is the same for any number of active lanes, and below 32 rows per wave the
+// waves outnumber what the compute units run at one wave per SIMD
+constexpr int kPass1Rows = 32;
 constexpr int kPass2Threads = 128;
-static_assert(kSeg % (3 * kChunk) == 0 && kSeg % (3 * kVecChunk) == 0, "checkpoints fall on loop steps");
+static_assert(kSeg % (3 * kChunk) == 0 && kSeg % (kVecSteps * kVecChunk) == 0, "checkpoints fall on loop steps");
 
 struct NsrArgs {
     const float *src;
@@ -172,52 +178,65 @@ __device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end
                                              Visit &visit, Mark &mark)
 {
     constexpr int G = kVecChunk / 4;                       // 16-byte groups per chunk
+    constexpr int NS = kVecSteps;                          // chunks in flight
     const int tb0 = start - W + 1 - M;                     // a multiple of 4
-    auto load = [&](float4 (&A)[G], float4 (&T)[G], int p0) {
+    float4 A[NS][G], T[NS][G];
+    auto load = [&](int c, int p0) {
         const int t0 = tb0 + (p0 - start);
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            A[g] = ld4(p0 + 4 * g);
-            T[g] = ld4(t0 + 4 * g);
+            A[c][g] = ld4(p0 + 4 * g);
+            T[c][g] = ld4(t0 + 4 * g);
         }
     };
-    auto run = [&](const float4 (&A)[G], const float4 (&T)[G], const float4 &Tn, int p0, auto masked) {
+    auto run = [&](int c, const float4 &Tn, int p0, auto masked) {
+        // the chunk's squares first (independent of the running sum), held behind an empty asm so
+        // they are not scheduled next to the additions: a lone wave would wait out each product's
+        // latency inside the dependent chain
+        float qv[kVecChunk], rv[kVecChunk];
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const float4 t1 = g + 1 < G ? T[g + 1] : Tn;
-            const float a[4] = {A[g].x, A[g].y, A[g].z, A[g].w};
-            const float t8[8] = {T[g].x, T[g].y, T[g].z, T[g].w, t1.x, t1.y, t1.z, t1.w};
+            const float4 t1 = g + 1 < G ? T[c][g + 1] : Tn;
+            const float a[4] = {A[c][g].x, A[c][g].y, A[c][g].z, A[c][g].w};
+            const float t8[8] = {T[c][g].x, T[c][g].y, T[c][g].z, T[c][g].w, t1.x, t1.y, t1.z, t1.w};
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int p = p0 + 4 * g + u;
-                float q = a[u] * a[u];
-                sum = sum + q;
-                visit(p, sum, decltype(masked)::value ? p < end : true);
-                float r = t8[M + u] * t8[M + u];
-                sum = sum - ((!Clamp || p - W + 1 >= 0) ? r : 0.0f);   // x - 0 is x
+                qv[4 * g + u] = a[u] * a[u];
+                rv[4 * g + u] = t8[M + u] * t8[M + u];
             }
         }
+#pragma unroll
+        for (int i = 0; i < kVecChunk; i += 8)
+            __asm__ volatile("" : "+v"(qv[i]), "+v"(qv[i + 1]), "+v"(qv[i + 2]), "+v"(qv[i + 3]), "+v"(qv[i + 4]),
+                                  "+v"(qv[i + 5]), "+v"(qv[i + 6]), "+v"(qv[i + 7]), "+v"(rv[i]), "+v"(rv[i + 1]),
+                                  "+v"(rv[i + 2]), "+v"(rv[i + 3]), "+v"(rv[i + 4]), "+v"(rv[i + 5]), "+v"(rv[i + 6]),
+                                  "+v"(rv[i + 7]));
+#pragma unroll
+        for (int i = 0; i < kVecChunk; i++) {
+            const int p = p0 + i;
+            sum = sum + qv[i];
+            visit(p, sum, decltype(masked)::value ? p < end : true);
+            sum = sum - ((!Clamp || p - W + 1 >= 0) ? rv[i] : 0.0f);   // x - 0 is x
+        }
     };
-    float4 A0[G], A1[G], A2[G], T0[G], T1[G], T2[G];
-    load(A0, T0, start);
-    load(A1, T1, start + kVecChunk);
-    load(A2, T2, start + 2 * kVecChunk);
+#pragma unroll
+    for (int c = 0; c < NS; c++) load(c, start + c * kVecChunk);
     int p0 = start;
-    for (; p0 + 3 * kVecChunk <= end; p0 += 3 * kVecChunk) {
+    for (; p0 + NS * kVecChunk <= end; p0 += NS * kVecChunk) {
         mark(p0 - start, sum);
-        run(A0, T0, T1[0], p0, std::false_type{});
-        load(A0, T0, p0 + 3 * kVecChunk);
-        run(A1, T1, T2[0], p0 + kVecChunk, std::false_type{});
-        load(A1, T1, p0 + 4 * kVecChunk);
-        run(A2, T2, T0[0], p0 + 2 * kVecChunk, std::false_type{});
-        load(A2, T2, p0 + 5 * kVecChunk);
+        // chunk c's last group reads the first trailing group of chunk c + 1; for the last chunk
+        // that is chunk 0's buffer, already reloaded with the next step's first chunk
+#pragma unroll
+        for (int c = 0; c < NS; c++) {
+            run(c, T[(c + 1) % NS][0], p0 + c * kVecChunk, std::false_type{});
+            load(c, p0 + (NS + c) * kVecChunk);
+        }
     }
     if (p0 < end) {
         mark(p0 - start, sum);
-        const float4 tn = ld4(tb0 + (p0 - start) + 3 * kVecChunk);   // the group after A2's chunk
-        run(A0, T0, T1[0], p0, std::true_type{});
-        run(A1, T1, T2[0], p0 + kVecChunk, std::true_type{});
-        run(A2, T2, tn, p0 + 2 * kVecChunk, std::true_type{});
+        const float4 tn = ld4(tb0 + (p0 - start) + NS * kVecChunk);   // the group after the last chunk
+#pragma unroll
+        for (int c = 0; c < NS; c++) run(c, c + 1 < NS ? T[c + 1][0] : tn, p0 + c * kVecChunk, std::true_type{});
     }
 }
 
@@ -226,24 +245,31 @@ __device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end
 __device__ __forceinline__ float fresh_sum_vec(const Row4 &ld4, int start, int W)
 {
     constexpr int G = kVecChunk / 4;
+    constexpr int NS = kVecSteps;
     float sum = 0.0f;
     const int f0 = max(start - W + 1, 0);
     if (f0 >= start) return sum;
     const int a0 = f0 & ~3;
-    auto load = [&](float4 (&v)[G], int i0) {
+    float4 bv[NS][G];
+    auto load = [&](int c, int i0) {
 #pragma unroll
-        for (int g = 0; g < G; g++) v[g] = ld4(i0 + 4 * g);
+        for (int g = 0; g < G; g++) bv[c][g] = ld4(i0 + 4 * g);
     };
-    auto add = [&](const float4 (&v)[G], int i0, auto masked) {
+    auto add = [&](int c, int i0, auto masked) {
+        float qv[kVecChunk];
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const float e[4] = {v[g].x, v[g].y, v[g].z, v[g].w};
+            const float e[4] = {bv[c][g].x, bv[c][g].y, bv[c][g].z, bv[c][g].w};
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                float q = e[u] * e[u];
-                sum = sum + ((!decltype(masked)::value || i0 + 4 * g + u < start) ? q : 0.0f);   // +0 keeps the sum
-            }
+            for (int u = 0; u < 4; u++) qv[4 * g + u] = e[u] * e[u];
         }
+#pragma unroll
+        for (int i = 0; i < kVecChunk; i += 8)
+            __asm__ volatile("" : "+v"(qv[i]), "+v"(qv[i + 1]), "+v"(qv[i + 2]), "+v"(qv[i + 3]), "+v"(qv[i + 4]),
+                                  "+v"(qv[i + 5]), "+v"(qv[i + 6]), "+v"(qv[i + 7]));
+#pragma unroll
+        for (int i = 0; i < kVecChunk; i++)
+            sum = sum + ((!decltype(masked)::value || i0 + i < start) ? qv[i] : 0.0f);   // +0 keeps the sum
     };
     {
         const float4 v = ld4(a0);
@@ -255,22 +281,18 @@ __device__ __forceinline__ float fresh_sum_vec(const Row4 &ld4, int start, int W
         }
     }
     int i0 = a0 + 4;
-    float4 b0[G], b1[G], b2[G];
-    load(b0, i0);
-    load(b1, i0 + kVecChunk);
-    load(b2, i0 + 2 * kVecChunk);
-    for (; i0 + 3 * kVecChunk <= start; i0 += 3 * kVecChunk) {
-        add(b0, i0, std::false_type{});
-        load(b0, i0 + 3 * kVecChunk);
-        add(b1, i0 + kVecChunk, std::false_type{});
-        load(b1, i0 + 4 * kVecChunk);
-        add(b2, i0 + 2 * kVecChunk, std::false_type{});
-        load(b2, i0 + 5 * kVecChunk);
+#pragma unroll
+    for (int c = 0; c < NS; c++) load(c, i0 + c * kVecChunk);
+    for (; i0 + NS * kVecChunk <= start; i0 += NS * kVecChunk) {
+#pragma unroll
+        for (int c = 0; c < NS; c++) {
+            add(c, i0 + c * kVecChunk, std::false_type{});
+            load(c, i0 + (NS + c) * kVecChunk);
+        }
     }
     if (i0 < start) {
-        add(b0, i0, std::true_type{});
-        add(b1, i0 + kVecChunk, std::true_type{});
-        add(b2, i0 + 2 * kVecChunk, std::true_type{});
+#pragma unroll
+        for (int c = 0; c < NS; c++) add(c, i0 + c * kVecChunk, std::true_type{});
     }
     return sum;
 }
@@ -316,7 +338,8 @@ __device__ __forceinline__ void mms_run(const NsrArgs &a, const float *x, int b,
 // rounded, and a block or span holds an mms >= cut exactly when its largest sum times 1/W does).
 __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, int B)
 {
-    const long long lane = static_cast<long long>(blockIdx.x) * kPass1Threads + threadIdx.x;
+    if (threadIdx.x >= kPass1Rows) return;
+    const long long lane = static_cast<long long>(blockIdx.x) * kPass1Rows + threadIdx.x;
     if (lane >= static_cast<long long>(B) * a.maxBlocks) return;
     // consecutive lanes take the same block of consecutive clips: the lanes of a wave then read
     // rows a clip stride apart rather than blocks a reset interval apart (a power-of-two stride
@@ -519,7 +542,7 @@ RPPA_EXPORT RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, Rppt
     a.segSum = reinterpret_cast<float *>(ws + segCells * 4);
     a.blockMax = reinterpret_cast<float *>(ws + segCells * 8);
 
-    const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Threads - 1) / rppa::kPass1Threads);
+    const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Rows - 1) / rppa::kPass1Rows);
     hipLaunchKernelGGL(rppa::nsr_blocks_kernel, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
     RPPA_HIP(hipGetLastError());
     hipLaunchKernelGGL(rppa::nsr_region_kernel, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
