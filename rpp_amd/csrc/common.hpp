@@ -204,7 +204,8 @@ struct FftLaunch {
 bool fft_eligible(const RpptLogMelParams *p, RpptLogMelPrecision precision, int B);
 // the fused front end's resampling shapes: integer ratio k and window lobes
 bool fft_front_eligible(int k, int lobes);
-size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false);
+size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front = false,
+                     int waves = 0);
 // nfft 1024 runs only the common launch (power 2, NFT output, a fixed dB reference, a normal cut-off
 // ratio) on clips given at the output rate
 bool fft_r2_launchable(const RpptLogMelParams *p, int ntf, bool clipRef, bool frontEnd);

@@ -68,7 +68,8 @@ constexpr int kN = kN1 * kN2;
 constexpr int kFI = 16;               // frames per work item
 constexpr int kWaves = 8;             // waves per workgroup (two per SIMD)
 constexpr int kThreads = 64 * kWaves;
-constexpr int kWavesR2 = 8;           // nfft 1024: waves per workgroup
+constexpr int kWavesR2 = 8;           // nfft 1024: waves per workgroup where the LDS holds eight segments,
+constexpr int kWavesR2Min = 6;        // else six
 constexpr int kPitchR2 = 88;          // nfft 1024: floats per phase row (odd rows skewed by 4 more)
 constexpr int kMaxMT = 8;             // filter tiles of 16 (numFilter <= 128)
 constexpr int kChunks = kN / 2 / 32;  // 32-bin chunks of bins 0 .. N/2 - 1
@@ -537,12 +538,12 @@ constexpr int pitch_for(int positions)
 // mel runs over both halves with their own fragment tables (the lower half's columns valid in the
 // parity-0 accumulators, the upper half's in the parity-1 ones), which are then added across the
 // lane pair together with bin 256's weights.
-template <int NMT, int HOP, bool GENERAL, int FRONT, int NFFT = 512>
-__global__ void __launch_bounds__(64 * (NFFT == 1024 ? kWavesR2 : kWaves), 1) logmel_fft_kernel(FftArgs a)
+template <int NMT, int HOP, bool GENERAL, int FRONT, int NFFT = 512, int WAVES = kWaves>
+__global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
 {
     constexpr bool R2 = NFFT == 1024;
     constexpr int FI = R2 ? 8 : kFI;                 // frames per item
-    constexpr int NW = R2 ? kWavesR2 : kWaves;       // waves per workgroup
+    constexpr int NW = WAVES;                        // waves per workgroup
     constexpr int NT = 64 * NW;
     constexpr int NPH = R2 ? 32 : 16;                // polyphase rows of a segment
     constexpr int HQ = HOP / NPH;                    // positions per hop
@@ -1352,8 +1353,9 @@ const MelFrag *get_mel_frag(rppHandle *h, const MelTables *mt, const std::string
     return &h->melFrag[key];
 }
 
-// LDS bytes of the factored kernel for nfft, hop H, B clips and nPairs mel pairs.
-size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front)
+// LDS bytes of the factored kernel for nfft, hop H, B clips and nPairs mel pairs; nfft 1024 with
+// `waves` waves per workgroup (0: the fewest it runs with).
+size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed, int *pitchOut, bool front, int waves)
 {
     const size_t B4 = static_cast<size_t>((B + 1 + 3) / 4 * 4);
     const size_t perClip = sizeof(int) * B4 * (2 + ((begin || front) ? 1 : 0) + (front ? 1 : 0)) +
@@ -1364,7 +1366,7 @@ size_t fft_lds_bytes(int nfft, int H, int B, int nPairs, bool begin, bool packed
         // 32 chunk words, the segments
         return 4096 + 8192 + sizeof(float) * (2 * kN2 * kN1 + 512 + 512 + 16 * kMaxMT) +
                static_cast<size_t>(nPairs) * 2048 + perClip + sizeof(int) * 32 +
-               sizeof(float) * kWavesR2 * static_cast<size_t>(32 * kPitchR2 + 64);
+               sizeof(float) * (waves > 0 ? waves : kWavesR2Min) * static_cast<size_t>(32 * kPitchR2 + 64);
     }
     const int pitch = pitch_for(((kFI - 1) * H + kN) / 16);
     if (pitchOut) *pitchOut = pitch;
@@ -1401,8 +1403,16 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     if (front && (L.begin || L.srcOff)) return RPP_ERROR_NOT_IMPLEMENTED;
     const int nfft = L.ft->nfft;
     const bool r2 = nfft == 2 * kN;
-    const size_t bytes = fft_lds_bytes(nfft, p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr,
-                                       &pitch, front);
+    // nfft 1024: eight waves per workgroup where their segments fit beside the mel table and the
+    // clip plan, else six
+    const auto r2Bytes = [&](int w) {
+        return fft_lds_bytes(nfft, p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr, L.srcOff != nullptr, &pitch,
+                             front, w);
+    };
+    const int r2Waves = r2 && r2Bytes(kWavesR2) <= static_cast<size_t>(h->maxLdsOptin) ? kWavesR2 : kWavesR2Min;
+    const size_t bytes = r2 ? r2Bytes(r2Waves)
+                            : fft_lds_bytes(nfft, p->windowStep, L.B, L.mf->nPairs, L.begin != nullptr,
+                                            L.srcOff != nullptr, &pitch, front);
     if (bytes > static_cast<size_t>(h->maxLdsOptin)) return RPP_ERROR_OUT_OF_BOUND_SHARED_MEMORY_SIZE;
     FftArgs a{};
     a.src = L.src;
@@ -1465,7 +1475,7 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
                                                                         p->windowLength, p->windowStep, p->centerWindows != 0),
                                                             0),
                                                    L.dstOff ? 0x7fffffffLL : static_cast<long long>(L.frameCap));
-        const int fi = r2 ? 8 : kFI, nw = r2 ? kWavesR2 : kWaves;
+        const int fi = r2 ? 8 : kFI, nw = r2 ? r2Waves : kWaves;
         const long long items = static_cast<long long>(L.B) * ((maxF + fi - 1) / fi);
         grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, (items + nw - 1) / nw)));
     }
@@ -1482,15 +1492,20 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     if (front && fk != 2 && fk != 3) return RPP_ERROR_NOT_IMPLEMENTED;
     if (r2) {
         if (general || front || p->windowStep != 256 || !a.tw2 || !a.w256) return RPP_ERROR_NOT_IMPLEMENTED;
-#define RPPA_FFT2_LAUNCH(NMT)                                                                                  \
-    do {                                                                                                       \
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_fft_kernel<NMT, 256, false, 0, 1024>), \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));        \
-        hipLaunchKernelGGL((logmel_fft_kernel<NMT, 256, false, 0, 1024>), dim3(grid), dim3(64 * kWavesR2), bytes, \
-                           h->stream, a);                                                                      \
+#define RPPA_FFT2_LAUNCH(NMT, NW)                                                                                  \
+    do {                                                                                                           \
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&logmel_fft_kernel<NMT, 256, false, 0, 1024, NW>), \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));            \
+        hipLaunchKernelGGL((logmel_fft_kernel<NMT, 256, false, 0, 1024, NW>), dim3(grid), dim3(64 * NW), bytes,     \
+                           h->stream, a);                                                                          \
     } while (0)
-        if (nMT <= 5) RPPA_FFT2_LAUNCH(5);
-        else RPPA_FFT2_LAUNCH(8);
+        if (r2Waves == kWavesR2) {
+            if (nMT <= 5) RPPA_FFT2_LAUNCH(5, kWavesR2);
+            else RPPA_FFT2_LAUNCH(8, kWavesR2);
+        } else {
+            if (nMT <= 5) RPPA_FFT2_LAUNCH(5, kWavesR2Min);
+            else RPPA_FFT2_LAUNCH(8, kWavesR2Min);
+        }
 #undef RPPA_FFT2_LAUNCH
         RPPA_HIP(hipGetLastError());
         return RPP_SUCCESS;

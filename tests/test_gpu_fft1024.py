@@ -14,6 +14,7 @@ for k < 256, X[256] = E[256] - i O[256].  The mel runs over both halves and bin 
     the same bits as aligned ones
   * the shapes the factored nfft-1024 launch does not take (NTF output, power 1, the clip-maximum
     reference) run the dense launch whatever the tiles ask: tiles 16 and 32 give the same bits
+  * a batch too large for eight waves' segments beside its clip plan runs with six waves
 """
 import numpy as np
 import pytest
@@ -131,3 +132,28 @@ def test_fft1024_general_shapes_run_dense(gpu, orc, shape):
             assert np.array_equal(a[b, :F, :], b_[b, :F, :]), b
         else:
             assert np.array_equal(a[b, :, :F], b_[b, :, :F]), b
+
+
+def test_fft1024_six_waves_for_large_batches(gpu, orc):
+    """A batch whose clip plan leaves no room for eight segments beside the 128-mel table (600
+    clips: 165 KB at eight waves) runs the factored launch with six waves per workgroup: within
+    0.05 dB of FP64 and of the dense launch, and not the dense launch's bits."""
+    c = CASES[0]
+    rng = np.random.default_rng(41)
+    L = rng.integers(1, 12000, 600).astype(np.int32)
+    L[:4] = [1, 512, 1024, 11999]
+    x = np.zeros((len(L), int(L.max()) + 8), np.float32)
+    for b, n in enumerate(L):
+        x[b, :n] = (0.3 * rng.normal(size=n)).astype(np.float32)
+    p = params(gpu, c)
+    fac, frames = run(gpu, x, L, p, c["nmel"], (16, 1))
+    den, _ = run(gpu, x, L, p, c["nmel"], (32, 2))
+    d = max(float(np.abs(fac[b, :, :F] - den[b, :, :F]).max()) for b, F in enumerate(frames))
+    same = all(np.array_equal(fac[b, :, :F], den[b, :, :F]) for b, F in enumerate(frames))
+    idx = np.arange(0, len(L), 10)
+    ref, _, fr = orc.logmel_f64(x[idx], L[idx], nfft=1024, W=c["W"], H=256, nmel=c["nmel"], sample_rate=c["sr"],
+                                fmax=c["sr"] / 2, nthreads=8)
+    err = max_db_err(fac[idx], ref, fr)
+    print(f"600 clips: factored vs dense {d:.3e} dB, vs FP64 (60 clips) {err:.3e} dB", flush=True)
+    assert err <= DB_TOL and d <= DB_TOL
+    assert not same, "the factored launch did not run"
