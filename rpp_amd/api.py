@@ -112,6 +112,7 @@ def lib():
         l.rppt_logmel_fused_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_noalloc_gpu.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_fused_region_gpu.argtypes = [P, DP, P, DP, P, P, C.POINTER(RpptLogMelParams), C.c_int, P]
+        l.rppt_logmel_fused_packed_gpu.argtypes = [P, P, P, C.c_int32, P, P, C.POINTER(RpptLogMelParams), C.c_int, P]
         l.rppt_logmel_set_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
         l.rppt_logmel_get_tiles.argtypes = [P, C.POINTER(RpptLogMelTiles)]
         l.rppt_logmel_autotune.argtypes = [P, DP, P, DP, P, C.POINTER(RpptLogMelParams), C.c_int, C.c_int32, P,
@@ -355,6 +356,17 @@ def logmel_fused_noalloc(h: Handle, src, dst, lengths, params: RpptLogMelParams,
                                              C.byref(params), PRECISION[precision], h.h)
     if check_status:
         check("rppt_logmel_fused_noalloc_gpu", st)
+    return st
+
+
+def logmel_fused_packed(h: Handle, src, src_offsets, lengths, dst, dst_offsets, params: RpptLogMelParams,
+                        precision="fp16x3", check_status=True) -> int:
+    """rppt_logmel_fused_packed_gpu: clip b is src[src_offsets[b] : + lengths[b]] (1-D f32 cuda), its
+    dB a dense [numFilter][frames_b] block at dst[dst_offsets[b]:]; offsets int64, lengths int32 (cuda)."""
+    st = lib().rppt_logmel_fused_packed_gpu(ptr(src), ptr(src_offsets), ptr(lengths), int(lengths.shape[0]), ptr(dst),
+                                            ptr(dst_offsets), C.byref(params), PRECISION[precision], h.h)
+    if check_status:
+        check("rppt_logmel_fused_packed_gpu", st)
     return st
 
 

@@ -15,6 +15,7 @@ for k < 256, X[256] = E[256] - i O[256].  The mel runs over both halves and bin 
   * the shapes the factored nfft-1024 launch does not take (NTF output, power 1, the clip-maximum
     reference) run the dense launch whatever the tiles ask: tiles 16 and 32 give the same bits
   * a batch too large for eight waves' segments beside its clip plan runs with six waves
+  * the region and packed entry points give the strided launch's bits
 """
 import numpy as np
 import pytest
@@ -157,3 +158,37 @@ def test_fft1024_six_waves_for_large_batches(gpu, orc):
     print(f"600 clips: factored vs dense {d:.3e} dB, vs FP64 (60 clips) {err:.3e} dB", flush=True)
     assert err <= DB_TOL and d <= DB_TOL
     assert not same, "the factored launch did not run"
+
+
+def test_fft1024_region_and_packed_entries(gpu):
+    """The region entry (per-clip begin/length into each row) and the packed entry (clips back to
+    back, outputs as dense per-clip blocks) take the same launch at nfft 1024: bit for bit the
+    strided launch on copies of the clips."""
+    c = CASES[0]
+    x, L = clips(c["W"], 256, seed=51)
+    rng = np.random.default_rng(52)
+    S = x.shape[1]
+    begin = rng.integers(0, 700, len(L)).astype(np.int32)
+    wide = np.zeros((len(L), S + 700), np.float32)
+    for b, n in enumerate(L):
+        wide[b, begin[b]:begin[b] + n] = x[b, :n]
+    p = params(gpu, c)
+    ref, frames = run(gpu, x, L, p, c["nmel"], (16, 1))
+    h = gpu.Handle(len(L))
+    gpu.set_tiles(h, 16, 1)
+    out = torch.full((len(L), c["nmel"], int(frames.max())), float(FILL), device="cuda")
+    gpu.logmel_fused_region(h, dev(wide), out, dev(begin), dev(L), p, "fp16x3")
+    src_off = np.concatenate([[0], np.cumsum(L[:-1].astype(np.int64) + 3)]).astype(np.int64)
+    flat = np.zeros(int(src_off[-1] + L[-1] + 8), np.float32)
+    for b, n in enumerate(L):
+        flat[src_off[b]:src_off[b] + n] = x[b, :n]
+    dst_off = np.concatenate([[0], np.cumsum(c["nmel"] * frames[:-1].astype(np.int64))]).astype(np.int64)
+    dst = torch.full((int(dst_off[-1] + c["nmel"] * frames[-1]),), float(FILL), device="cuda")
+    gpu.logmel_fused_packed(h, dev(flat), dev(src_off), dev(L), dst, dev(dst_off), p, "fp16x3")
+    torch.cuda.synchronize()
+    h.close()
+    o, d = out.cpu().numpy(), dst.cpu().numpy()
+    for b, F in enumerate(frames):
+        assert np.array_equal(o[b, :, :F], ref[b, :, :F]), ("region", b)
+        blk = d[dst_off[b]:dst_off[b] + c["nmel"] * F].reshape(c["nmel"], F)
+        assert np.array_equal(blk, ref[b, :, :F]), ("packed", b)
