@@ -1465,8 +1465,10 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
     }
     if (L.mf->nPairs <= 0) return RPP_ERROR_NOT_IMPLEMENTED;
     const int nMT = (p->numFilter + 15) / 16;
-    // one workgroup per CU, or fewer when the batch cannot give every wave an item: the host bounds
-    // the items by the row stride (strided layout; the packed layout's lengths are on the device)
+    // one workgroup per CU, or one per item when there are fewer items than CUs (each item's wave
+    // then has a compute unit to itself: workload 0's 105 items in 16.1 us instead of 25.0 us as 14
+    // full workgroups); the host bounds the items by the row stride (strided layout; the packed
+    // layout's lengths are on the device)
     unsigned grid = static_cast<unsigned>(h->numCU);
     if (!L.srcOff) {
         long long maxLen = L.srcStride;
@@ -1475,9 +1477,9 @@ RppStatus logmel_fft_launch(rppHandle *h, const FftLaunch &L)
                                                                         p->windowLength, p->windowStep, p->centerWindows != 0),
                                                             0),
                                                    L.dstOff ? 0x7fffffffLL : static_cast<long long>(L.frameCap));
-        const int fi = r2 ? 8 : kFI, nw = r2 ? r2Waves : kWaves;
+        const int fi = r2 ? 8 : kFI;
         const long long items = static_cast<long long>(L.B) * ((maxF + fi - 1) / fi);
-        grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, (items + nw - 1) / nw)));
+        grid = static_cast<unsigned>(std::max<long long>(1, std::min<long long>(grid, items)));
     }
     const bool general = a.power != 2 || a.ntf || a.clipMax || !(a.minRatio >= 1.17549435e-38f);
     // the front end runs with the common launch only (the caller falls back to two launches)
