@@ -168,14 +168,58 @@ struct Row4 {
     }
 };
 
+// Two samples of the recurrence, s += q0 (visited), s -= r0, s += q1 (visited), s -= r1, m = max of m
+// and the two visited sums, with the squares of two later samples (x, y -> nq, nr) formed between
+// the dependent operations: a lone wave issues in order, and each dependent FP32 operation waits
+// about 8 cycles for its operand, so the products fill those slots (the compiler schedules each
+// product next to the addition that uses it).  The operations and their rounding are the C++ ones.
+__device__ __forceinline__ void mms_pair(float &s, float &m, float q0, float q1, float r0, float r1, float x0, float x1,
+                                         float y0, float y1, float &nq0, float &nq1, float &nr0, float &nr1)
+{
+    float a0, a1;
+    __asm__("v_add_f32 %[a0], %[s], %[q0]\n\t"
+            "v_mul_f32 %[nq0], %[x0], %[x0]\n\t"
+            "v_sub_f32 %[s], %[a0], %[r0]\n\t"
+            "v_mul_f32 %[nr0], %[y0], %[y0]\n\t"
+            "v_add_f32 %[a1], %[s], %[q1]\n\t"
+            "v_mul_f32 %[nq1], %[x1], %[x1]\n\t"
+            "v_sub_f32 %[s], %[a1], %[r1]\n\t"
+            "v_mul_f32 %[nr1], %[y1], %[y1]\n\t"
+            "v_max3_f32 %[m], %[m], %[a0], %[a1]"
+            : [s] "+v"(s), [m] "+v"(m), [a0] "=&v"(a0), [a1] "=&v"(a1), [nq0] "=&v"(nq0), [nq1] "=&v"(nq1),
+              [nr0] "=&v"(nr0), [nr1] "=&v"(nr1)
+            : [q0] "v"(q0), [q1] "v"(q1), [r0] "v"(r0), [r1] "v"(r1), [x0] "v"(x0), [x1] "v"(x1), [y0] "v"(y0),
+              [y1] "v"(y1));
+}
+
+// Two terms of a running sum, s += q0, s += q1, with the squares of two later samples formed between.
+__device__ __forceinline__ void add_pair(float &s, float q0, float q1, float x0, float x1, float &nq0, float &nq1)
+{
+    __asm__("v_add_f32 %[s], %[s], %[q0]\n\t"
+            "v_mul_f32 %[nq0], %[x0], %[x0]\n\t"
+            "v_add_f32 %[s], %[s], %[q1]\n\t"
+            "v_mul_f32 %[nq1], %[x1], %[x1]"
+            : [s] "+v"(s), [nq0] "=&v"(nq0), [nq1] "=&v"(nq1)
+            : [q0] "v"(q0), [q1] "v"(q1), [x0] "v"(x0), [x1] "v"(x1));
+}
+
+__device__ __forceinline__ float comp(const float4 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// The running maximum of the visited sums (the pass-1 visitor; the unmasked loop steps update m
+// directly).
+struct RunMax {
+    float m = -INFINITY;
+    __device__ __forceinline__ void operator()(int, float s, bool valid) { m = fmaxf(m, valid ? s : -INFINITY); }
+};
+
 // The same recurrence on 16-byte loads, for rows that are 16-byte aligned with a start that is a
 // multiple of 4: the operations and their order are mms_main's, only the loads differ (the
 // trailing samples x[p - W + 1] are elements M .. M+3 of two aligned groups, M = (start - W + 1) & 3).
 // Full loop steps run unmasked; the last, partial step flags the outputs past `end`.  Clamp
 // (start < W - 1) keeps the HOST's skipped subtraction for outputs p < W - 1.
 template <int M, bool Clamp, class Visit, class Mark>
-__device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end, int W, float sum,
-                                             Visit &visit, Mark &mark)
+__device__ __forceinline__ float mms_main_vec(const Row4 &ld4, int start, int end, int W, float sum,
+                                              Visit &visit, Mark &mark)
 {
     constexpr int G = kVecChunk / 4;                       // 16-byte groups per chunk
     constexpr int NS = kVecSteps;                          // chunks in flight
@@ -219,16 +263,47 @@ __device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end
             sum = sum - ((!Clamp || p - W + 1 >= 0) ? rv[i] : 0.0f);   // x - 0 is x
         }
     };
+    // unclamped full steps: sample i's value and trailing value (chunk c's last group reads the first
+    // trailing group of chunk c + 1; for the last chunk that is chunk 0's buffer, already reloaded
+    // with the next step's first chunk); the squares of the next two samples are carried in q, r
+    auto xval = [&](int c, int i) { return comp(A[c][i >> 2], i & 3); };
+    auto yval = [&](int c, int i) {
+        const int g = i >> 2, k = M + (i & 3);
+        return comp(k < 4 ? T[c][g] : (g + 1 < G ? T[c][g + 1] : T[(c + 1) % NS][0]), k & 3);
+    };
+    float q[2], r[2];
+    auto run_fast = [&](int c) {
+        const int cn = (c + 1) % NS;
+#pragma unroll
+        for (int i = 0; i < kVecChunk; i += 2) {
+            const float x0 = i + 2 < kVecChunk ? xval(c, i + 2) : xval(cn, i + 2 - kVecChunk);
+            const float x1 = i + 3 < kVecChunk ? xval(c, i + 3) : xval(cn, i + 3 - kVecChunk);
+            const float y0 = i + 2 < kVecChunk ? yval(c, i + 2) : yval(cn, i + 2 - kVecChunk);
+            const float y1 = i + 3 < kVecChunk ? yval(c, i + 3) : yval(cn, i + 3 - kVecChunk);
+            float nq0, nq1, nr0, nr1;
+            mms_pair(sum, visit.m, q[0], q[1], r[0], r[1], x0, x1, y0, y1, nq0, nq1, nr0, nr1);
+            q[0] = nq0;
+            q[1] = nq1;
+            r[0] = nr0;
+            r[1] = nr1;
+        }
+    };
 #pragma unroll
     for (int c = 0; c < NS; c++) load(c, start + c * kVecChunk);
     int p0 = start;
+    if (!Clamp && p0 + NS * kVecChunk <= end) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            q[u] = xval(0, u) * xval(0, u);
+            r[u] = yval(0, u) * yval(0, u);
+        }
+    }
     for (; p0 + NS * kVecChunk <= end; p0 += NS * kVecChunk) {
         mark(p0 - start, sum);
-        // chunk c's last group reads the first trailing group of chunk c + 1; for the last chunk
-        // that is chunk 0's buffer, already reloaded with the next step's first chunk
 #pragma unroll
         for (int c = 0; c < NS; c++) {
-            run(c, T[(c + 1) % NS][0], p0 + c * kVecChunk, std::false_type{});
+            if (Clamp) run(c, T[(c + 1) % NS][0], p0 + c * kVecChunk, std::false_type{});
+            else run_fast(c);
             load(c, p0 + (NS + c) * kVecChunk);
         }
     }
@@ -238,6 +313,7 @@ __device__ __forceinline__ void mms_main_vec(const Row4 &ld4, int start, int end
 #pragma unroll
         for (int c = 0; c < NS; c++) run(c, c + 1 < NS ? T[c + 1][0] : tn, p0 + c * kVecChunk, std::true_type{});
     }
+    return sum;
 }
 
 // fresh_sum on aligned 16-byte groups: the first group is masked below f0, the full loop steps
@@ -283,10 +359,26 @@ __device__ __forceinline__ float fresh_sum_vec(const Row4 &ld4, int start, int W
     int i0 = a0 + 4;
 #pragma unroll
     for (int c = 0; c < NS; c++) load(c, i0 + c * kVecChunk);
+    // full steps: the squares of the next two samples carried in q (see add_pair)
+    float q[2];
+    auto xval = [&](int c, int i) { return comp(bv[c][i >> 2], i & 3); };
+    if (i0 + NS * kVecChunk <= start) {
+        q[0] = xval(0, 0) * xval(0, 0);
+        q[1] = xval(0, 1) * xval(0, 1);
+    }
     for (; i0 + NS * kVecChunk <= start; i0 += NS * kVecChunk) {
 #pragma unroll
         for (int c = 0; c < NS; c++) {
-            add(c, i0 + c * kVecChunk, std::false_type{});
+            const int cn = (c + 1) % NS;
+#pragma unroll
+            for (int i = 0; i < kVecChunk; i += 2) {
+                const float x0 = i + 2 < kVecChunk ? xval(c, i + 2) : xval(cn, i + 2 - kVecChunk);
+                const float x1 = i + 3 < kVecChunk ? xval(c, i + 3) : xval(cn, i + 3 - kVecChunk);
+                float nq0, nq1;
+                add_pair(sum, q[0], q[1], x0, x1, nq0, nq1);
+                q[0] = nq0;
+                q[1] = nq1;
+            }
             load(c, i0 + (NS + c) * kVecChunk);
         }
     }
@@ -311,8 +403,15 @@ __device__ __forceinline__ float fresh(const NsrArgs &a, const float *x, int b, 
 template <int M, class Visit, class Mark>
 __device__ __forceinline__ void mms_vec(const Row4 &ld4, int start, int end, int W, float sum, Visit &visit, Mark &mark)
 {
-    if (start - W + 1 < 0) mms_main_vec<M, true>(ld4, start, end, W, sum, visit, mark);
-    else mms_main_vec<M, false>(ld4, start, end, W, sum, visit, mark);
+    if (start - W + 1 < 0) {
+        // the clamped form only up to the first checkpoint at or past output W - 1 (a multiple of
+        // kSeg from start, so the checkpoints and the trailing alignment M carry on unchanged)
+        const int split = start + (W - 1 - start + kSeg - 1) / kSeg * kSeg;
+        sum = mms_main_vec<M, true>(ld4, start, min(split, end), W, sum, visit, mark);
+        if (split < end) mms_main_vec<M, false>(ld4, split, end, W, sum, visit, mark);
+    } else {
+        mms_main_vec<M, false>(ld4, start, end, W, sum, visit, mark);
+    }
 }
 
 template <class Visit, class Mark>
@@ -355,23 +454,23 @@ __global__ void __launch_bounds__(kPass1Threads) nsr_blocks_kernel(NsrArgs a, in
     const float sum0 = fresh(a, x, b, L, start);
     float *segMax = a.segMax + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
     float *segSum = a.segSum + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
-    float m = -INFINITY, mb = -INFINITY;
+    float mb = -INFINITY;
     int k = -1;
-    auto visit = [&](int, float s, bool valid) { m = fmaxf(m, valid ? s : -INFINITY); };
+    RunMax visit;
     auto mark = [&](int off, float s) {
         if (off % kSeg == 0) {
             if (k >= 0) {
-                segMax[k] = m;
-                mb = fmaxf(mb, m);
+                segMax[k] = visit.m;
+                mb = fmaxf(mb, visit.m);
             }
             ++k;
-            m = -INFINITY;
+            visit.m = -INFINITY;
             segSum[k] = s;
         }
     };
     mms_run(a, x, b, L, start, end, sum0, visit, mark);
-    segMax[k] = m;   // start < end: the first loop step marked checkpoint 0
-    a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, m);
+    segMax[k] = visit.m;   // start < end: the first loop step marked checkpoint 0
+    a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, visit.m);
 }
 
 // Pass 2: one workgroup per clip reduces pass 1's records to the cut-off (unless the reference
