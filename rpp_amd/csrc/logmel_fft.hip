@@ -428,7 +428,6 @@ struct Item {
     Clip v;
     int f0, s0, valid;
     bool interior;   // every staged sample and its predecessor inside the clip, valid a multiple of 4
-    bool aligned;    // the clip samples at s0 are 16-byte aligned
 };
 
 // fp16 hi/lo of the exact products d0*w0 and d1*w1 as two packed pairs: each half is one
@@ -492,6 +491,13 @@ __device__ __forceinline__ void transpose4(uint32_t &r0, uint32_t &r1, uint32_t 
     r1 = t01[1];
     r2 = t23[0];
     r3 = t23[1];
+}
+
+// m = max(m, |a|, |b|) in one instruction (the library max would first canonicalise each loaded
+// operand; the magnitudes here only set a power-of-two scale)
+__device__ __forceinline__ void max3_abs(float &m, float a, float b)
+{
+    __asm__("v_max3_f32 %0, |%1|, |%2|, %0" : "+v"(m) : "v"(a), "v"(b));
 }
 
 __device__ __forceinline__ float wave_max(float v)
@@ -671,7 +677,6 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
         const int nfr = min(FI, t.v.F - t.f0);
         t.valid = (nfr - 1) * HOP + a.W;
         t.interior = t.s0 >= 1 && t.s0 + t.valid <= t.v.L && (t.valid & 3) == 0;
-        t.aligned = ((reinterpret_cast<uintptr_t>(t.v.x + t.s0)) & 15) == 0;
         return t;
     };
     // the first of the four consecutive samples lane l stages in load m of round r: segment position
@@ -683,26 +688,29 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                   : 1024 * r + 64 * (lane >> 2) + 16 * m + 4 * (lane & 3);
     };
     // an interior item's samples of staging rounds [r0, r1) requested into px/pp (sample_of), and pp
-    // the sample before each group; nothing past valid is read
-    // (interior items only: every group index is clamped to the last valid group, so each load is
-    // in bounds and unconditional -- no per-lane branch -- and groups past valid are zeroed after)
+    // the sample before each group; nothing past valid is read.  Interior items only: every group
+    // index is clamped to the last valid group, so each load is in bounds and unconditional (no
+    // per-lane branch).  A group past valid repeats the last valid group, which is staged too: the
+    // item's maximum, and so its scale, is unchanged, and every position such a group fills is one
+    // the window zeroes for each frame the item stores.  Buffer loads from the item's sample s0 - 1
+    // on: a 32-bit byte offset per lane, and 16-byte loads need only 4-byte alignment.
     auto prefetch = [&](const Item &t, int r0, int r1) {
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
-        const float *base = t.v.x + t.s0;
+        // (the item's values are wave-uniform; read from lane 0 so the descriptor sits in scalar registers)
+        const uintptr_t xa = reinterpret_cast<uintptr_t>(t.v.x + t.s0 - 1);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(xa));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(xa >> 32));
+        const int valid = __builtin_amdgcn_readfirstlane(t.valid);
+        float *xs = reinterpret_cast<float *>(static_cast<uintptr_t>(lo) | (static_cast<uintptr_t>(hi) << 32));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(xs, 0, 4 * (valid + 1), 0x00020000);
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
             if (r < r0 || r >= r1) continue;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const int i = sample_of(r, m, lane);
-                const bool in = i < t.valid;
-                const float *p = base + min(i, t.valid - 4);
-                float4 q;
-                if (t.aligned) q = *reinterpret_cast<const float4 *>(p);
-                else q = make_float4(p[0], p[1], p[2], p[3]);
-                const float pv = p[-1];
-                px[r][m] = in ? q : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                pp[r][m] = in ? pv : 0.0f;
+                const unsigned o = 4u * static_cast<unsigned>(min(sample_of(r, m, lane), valid - 4));
+                px[r][m] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, o + 4u, 0, 0));
+                pp[r][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0));
             }
         }
     };
@@ -829,14 +837,20 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             }
             // the scale from the raw samples: |x[n] - c x[n-1]| <= (1 + |c|) max |x|, so the
             // pre-emphasised samples are formed in the scaled domain at once, one FMA each
+            {
+                // three running maxima, two new magnitudes per v_max3 each
+                float m1 = 0.0f, m2 = 0.0f;
 #pragma unroll
-            for (int r = 0; r < ROUNDS; r++)
+                for (int r = 0; r < ROUNDS; r++)
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    const float4 q = px[r][m];
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fmaxf(fabsf(q.z), fabsf(q.w))));
-                    mx = fmaxf(mx, fabsf(pp[r][m]));
-                }
+                    for (int m = 0; m < 4; m++) {
+                        const float4 q = px[r][m];
+                        max3_abs(mx, q.x, q.y);
+                        max3_abs(m1, q.z, q.w);
+                        if (m & 1) max3_abs(m2, pp[r][m - 1], pp[r][m]);
+                    }
+                mx = fmaxf(mx, fmaxf(m1, m2));
+            }
             mx = wave_max(mx) * (1.0f + fabsf(c));
             scale = mx > 0.0f ? ldexpf(1.0f, min(9 - ilogbf(mx), 114)) : 1.0f;
 #pragma unroll
@@ -1209,7 +1223,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
 
         // ---- dB (or the power, for the clip-maximum reference) and the stores
         // the planes hold power of operands scaled by scale * 2^12: undo once per output
-        const float unscale = 1.0f / (scale * 4096.0f);
+        const float unscale = ldexpf(1.0f, -12 - ilogbf(scale));   // scale is a power of two
         const float melScale = (GENERAL && a.power == 1) ? unscale : unscale * unscale;
         const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
         const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
@@ -1219,13 +1233,27 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                 // row 16 mt + 4 g + i: a scalar base per (mt, i) plus the lane's offset 4 g hs + f
                 const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
                 const float k = melScale * a.invRef;
+                if (a.M == 16 * NMT && v.hs < (1u << 26)) {
+                    // every row exists and the lane's byte offset fits 32 bits: one scalar base per row
+                    const uint32_t laneB = 4u * laneOff;
 #pragma unroll
-                for (int mt = 0; mt < NMT; mt++) {
+                    for (int mt = 0; mt < NMT; mt++) {
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
-                        float *rowp = v.out + static_cast<size_t>(16 * mt + i) * v.hs;
-                        if (16 * mt + 4 * g + i < a.M) rowp[laneOff] = val;
+                        for (int i = 0; i < 4; i++) {
+                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
+                            char *rowp = reinterpret_cast<char *>(v.out + static_cast<size_t>(16 * mt + i) * v.hs);
+                            *reinterpret_cast<float *>(rowp + laneB) = val;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int mt = 0; mt < NMT; mt++) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
+                            float *rowp = v.out + static_cast<size_t>(16 * mt + i) * v.hs;
+                            if (16 * mt + 4 * g + i < a.M) rowp[laneOff] = val;
+                        }
                     }
                 }
             }

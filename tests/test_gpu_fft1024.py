@@ -10,8 +10,8 @@ for k < 256, X[256] = E[256] - i O[256].  The mel runs over both halves and bin 
     cells past each clip's frames stay untouched
   * the factored launch (framesPerTile 16) and the dense launch (framesPerTile 32) agree within
     0.05 dB and are not bit-identical, so both kernels really ran
-  * clips whose samples are not 16-byte aligned (row stride S + 1: the scalar staging loads) give
-    the same bits as aligned ones
+  * clips whose samples are not 16-byte aligned (row stride S + 1; the staging's 16-byte buffer
+    loads take any 4-byte alignment) give the same bits as aligned ones
   * the shapes the factored nfft-1024 launch does not take (NTF output, power 1, the clip-maximum
     reference) run the dense launch whatever the tiles ask: tiles 16 and 32 give the same bits
   * a batch too large for eight waves' segments beside its clip plan runs with six waves
