@@ -73,6 +73,9 @@ constexpr int kWavesR2Min = 6;        // else six
 constexpr int kPitchR2 = 88;          // nfft 1024: floats per phase row (odd rows skewed by 4 more)
 constexpr int kMaxMT = 8;             // filter tiles of 16 (numFilter <= 128)
 constexpr int kChunks = kN / 2 / 32;  // 32-bin chunks of bins 0 .. N/2 - 1
+// the mel window of a launch with NMT filter tiles: the filter tiles one 32-bin chunk may touch (all
+// of them when NMT <= 5, else at most six)
+constexpr int mel_window(int nmt) { return nmt < 6 ? nmt : 6; }
 // fused front end: 2 x 16 FIR taps (16 zero crossings), resampled outputs per chunk (two per lane)
 // and the down-mixed inputs a chunk reads at up to 3 inputs per output
 constexpr int kFeTaps = 32;
@@ -214,8 +217,9 @@ __device__ inline uint16_t bf16_bits(float v)
 // 32 c + 8 (l / 16) .. + 7 of the chunk).  Pairs are numbered chunk by chunk, tiles ascending within
 // a chunk: the tiles whose band [c0, c1) contains chunk c are a contiguous run [t0, t1) (the filters
 // ascend in frequency), so pairOff[c] = t0 | t1 << 8 | p0 << 16 with p0 the run's first pair (t0 = t1:
-// no pair), pairOff[256] = pair count, pairOff[255] = 1 if some chunk's tiles are not contiguous (the
-// factored kernel then cannot use the table).  One workgroup.
+// no pair), pairOff[256] = pair count, pairOff[255] = 1 if some chunk's tiles are not contiguous, a
+// run touches more tiles than the mel window holds or the runs' first tiles descend (the factored
+// kernel then cannot use the table).  One workgroup.
 // r2 = 0: position u is bin u (bins 0 .. N/2 - 1).  r2 = 1 (nfft 1024, radix-2 split): each chunk has
 // two runs, first the lower half's (position u is bin u < 256, word pairOff[c]) and then the upper
 // half's (position u is bin 512 - u, bins 257 .. 512, word pairOff[16 + c]), numbered in that order;
@@ -273,6 +277,18 @@ __global__ void __launch_bounds__(256) mel_frag_kernel(const float *__restrict__
                 word[16 * half + c] = t0 | (t1 << 8) | (n << 16);
                 n += t1 - t0;
             }
+        // the kernel streams the runs in ascending frequency (the lower half's by ascending chunk,
+        // then the upper half's by descending chunk) with the mel window's tiles in flight (the
+        // launch's tile count: 5 up to 80 filters, else 8): each run's first tile must not decrease,
+        // and its tiles must fit the window
+        const int win = mel_window(nMT <= 5 ? 5 : 8);
+        for (int r = 0, prev = 0; r < halves * kChunks; r++) {
+            const int w = word[r < kChunks ? r : 16 + (2 * kChunks - 1 - r)];
+            const int t0 = w & 255, t1 = (w >> 8) & 255;
+            if (t0 >= t1) continue;
+            if (t0 < prev || t1 - t0 > win) bad = 1;
+            prev = t0;
+        }
         total = n;
         broken = bad;
     }
@@ -1160,139 +1176,246 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             if (!FRONT && nxt.interior) prefetch(nxt, 0, PFR);
         }
 
-        // ---- banded mel on MFMA, pairs in their table order (chunk by chunk, tiles ascending): chunk
-        //      c's operand is PH/PL[16 (c % 2) + 4 (c / 2) + 0..3]; each pair's weight fragments were
-        //      requested two pairs ahead
-        //      (nfft 1024: each chunk's second run is the upper half's, on the power of the lane's
-        //      partner l ^ 8 -- the frame's bins 512 - u -- so the frame's parity-0 column gathers the
-        //      whole filter bank; then bin 256's weights)
-        f32x4 mel[NMT];
+        if constexpr (!GENERAL) {
+            // ---- banded mel on MFMA, streamed over the filter tiles, then dB and the stores.  The runs
+            //      (a chunk's tiles; nfft 1024: the lower half's runs by ascending chunk, then the upper
+            //      half's by descending chunk -- bins 512 - u, on the power of the lane's partner l ^ 8,
+            //      so the frame's parity-0 column gathers the whole filter bank) ascend in frequency, and
+            //      so does each run's first tile (the table builder checks it): a window of MW
+            //      accumulators holds tiles lo .. lo + MW - 1, and a tile below the current run's
+            //      first tile is complete, so it is converted to dB (nfft 1024: bin 256's weights added
+            //      first) and stored, and the window moves up.  Chunk c's operand is
+            //      PH/PL[16 (c % 2) + 4 (c / 2) + 0..3]; a run's first two pairs' weight fragments are
+            //      requested one run ahead, into the register set of the run's parity, the others when
+            //      used.
+            constexpr int NR = R2 ? 2 * kChunks : kChunks;
+            constexpr int MW = mel_window(NMT);
+            int runW[NR];   // all read at once: one LDS wait, not one per run
 #pragma unroll
-        for (int mt = 0; mt < NMT; mt++) mel[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const u32x4 *melL = melF + (opaque(static_cast<int>(threadIdx.x)) & 63);
-        u32x4 wch = melL[0], wcl = melL[64];
-        u32x4 wnh = melL[(min(1, nP - 1) * 2) * 64], wnl = melL[(min(1, nP - 1) * 2 + 1) * 64];
-        int pn = 2;
-        auto run = [&](int t0, int t1, const u32x4 &bh, const u32x4 &bl) {
+            for (int r = 0; r < NR; r++) runW[r] = chunkW[r < kChunks ? r : 16 + (NR - 1 - r)];
+            const u32x4 *melL = melF + (opaque(static_cast<int>(threadIdx.x)) & 63);
+            u32x4 wf[2][2][2];   // [run parity][first, second pair][hi, lo]
+            auto request = [&](int w, int par) {
+                const int p0 = w >> 16;
+                const int q0 = max(min(p0, nP - 1), 0), q1 = max(min(p0 + 1, nP - 1), 0);
+                wf[par][0][0] = melL[(q0 * 2) * 64];
+                wf[par][0][1] = melL[(q0 * 2 + 1) * 64];
+                wf[par][1][0] = melL[(q1 * 2) * 64];
+                wf[par][1][1] = melL[(q1 * 2 + 1) * 64];
+            };
+            request(__builtin_amdgcn_readfirstlane(runW[0]), 0);
+            // lane l stores frame f0 + l % 16 (nfft 1024: f0 + l % 8, from the parity-0 lanes), rows
+            // 16 t + 4 g + i of tile t: a scalar row base plus the lane's offset 4 g hs + f
+            const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
+            const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
+            const int f = f0 + fr;
+            const bool storeLane = f < v.F && !(R2 && (lane & 8));
+            const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
+            const bool off32 = v.hs < (1u << 26);   // the lane's byte offset fits 32 bits
+            // the planes hold power of operands scaled by scale * 2^12 (scale a power of two): undo once
+            // per output
+            const float unscale = ldexpf(1.0f, -12 - ilogbf(scale));
+            const float kdb = unscale * unscale * a.invRef;
+            const int nMT = (a.M + 15) >> 4;
+            f32x4 acc[MW];
 #pragma unroll
-            for (int mt = 0; mt < NMT; mt++) {
-                if (mt >= t0 && mt < t1) {
-                    mel[mt] = mfma3_bf16(wch, wcl, bh, bl, mel[mt]);
-                    wch = wnh;
-                    wcl = wnl;
-                    const int q = min(pn, nP - 1);
-                    wnh = melL[(q * 2) * 64];
-                    wnl = melL[(q * 2 + 1) * 64];
-                    pn++;
+            for (int j = 0; j < MW; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            int lo = 0;
+            auto retire = [&]() {
+                f32x4 t = acc[0];
+                if (R2) {
+                    const float4 w = *reinterpret_cast<const float4 *>(w256 + 16 * lo + 4 * g);
+                    t[0] = fmaf(w.x, p256, t[0]);
+                    t[1] = fmaf(w.y, p256, t[1]);
+                    t[2] = fmaf(w.z, p256, t[2]);
+                    t[3] = fmaf(w.w, p256, t[3]);
                 }
-            }
-        };
-#pragma unroll
-        for (int ch = 0; ch < kChunks; ch++) {
-            const int cw = __builtin_amdgcn_readfirstlane(chunkW[ch]);   // read per item from LDS
-            const int base = 16 * (ch & 1) + 4 * (ch >> 1);
-            const u32x4 bh = {PH[base], PH[base + 1], PH[base + 2], PH[base + 3]};
-            const u32x4 bl = {PL[base], PL[base + 1], PL[base + 2], PL[base + 3]};
-            run(cw & 255, (cw >> 8) & 255, bh, bl);
-            if (R2) {
-                const int cw2 = __builtin_amdgcn_readfirstlane(chunkW[16 + ch]);
-                const int t0 = cw2 & 255, t1 = (cw2 >> 8) & 255;
-                if (t0 < t1) {   // (no partner swap for a chunk without pairs)
-                    u32x4 sh, sl;
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        sh[e] = swap8u(bh[e]);
-                        sl[e] = swap8u(bl[e]);
-                    }
-                    run(t0, t1, sh, sl);
-                }
-            }
-        }
-        if (R2) {
-            const int g = (opaque(static_cast<int>(threadIdx.x)) >> 4) & 3;
-#pragma unroll
-            for (int mt = 0; mt < NMT; mt++) {
-                const float4 w = *reinterpret_cast<const float4 *>(w256 + 16 * mt + 4 * g);
-                mel[mt][0] = fmaf(w.x, p256, mel[mt][0]);
-                mel[mt][1] = fmaf(w.y, p256, mel[mt][1]);
-                mel[mt][2] = fmaf(w.z, p256, mel[mt][2]);
-                mel[mt][3] = fmaf(w.w, p256, mel[mt][3]);
-            }
-        }
-
-
-        // ---- dB (or the power, for the clip-maximum reference) and the stores
-        // the planes hold power of operands scaled by scale * 2^12: undo once per output
-        const float unscale = ldexpf(1.0f, -12 - ilogbf(scale));   // scale is a power of two
-        const float melScale = (GENERAL && a.power == 1) ? unscale : unscale * unscale;
-        const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
-        const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
-        const int f = f0 + fr;
-        if (!GENERAL) {
-            if (f < v.F && !(R2 && (lane & 8))) {
-                // row 16 mt + 4 g + i: a scalar base per (mt, i) plus the lane's offset 4 g hs + f
-                const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
-                const float k = melScale * a.invRef;
-                if (a.M == 16 * NMT && v.hs < (1u << 26)) {
-                    // every row exists and the lane's byte offset fits 32 bits: one scalar base per row
-                    const uint32_t laneB = 4u * laneOff;
-#pragma unroll
-                    for (int mt = 0; mt < NMT; mt++) {
+                if (storeLane) {
+                    if (off32 && 16 * lo + 16 <= a.M) {
+                        const uint32_t laneB = 4u * laneOff;
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
-                            char *rowp = reinterpret_cast<char *>(v.out + static_cast<size_t>(16 * mt + i) * v.hs);
+                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, t[i] * kdb));
+                            char *rowp = reinterpret_cast<char *>(v.out + static_cast<size_t>(16 * lo + i) * v.hs);
                             *reinterpret_cast<float *>(rowp + laneB) = val;
                         }
-                    }
-                } else {
-#pragma unroll
-                    for (int mt = 0; mt < NMT; mt++) {
+                    } else {
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
-                            float *rowp = v.out + static_cast<size_t>(16 * mt + i) * v.hs;
-                            if (16 * mt + 4 * g + i < a.M) rowp[laneOff] = val;
+                            const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, t[i] * kdb));
+                            float *rowp = v.out + static_cast<size_t>(16 * lo + i) * v.hs;
+                            if (16 * lo + 4 * g + i < a.M) rowp[laneOff] = val;
                         }
                     }
                 }
+#pragma unroll
+                for (int j = 0; j + 1 < MW; j++) acc[j] = acc[j + 1];
+                acc[MW - 1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                lo++;
+            };
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int w = __builtin_amdgcn_readfirstlane(runW[r]);
+                if (r + 1 < NR) request(__builtin_amdgcn_readfirstlane(runW[r + 1]), (r + 1) & 1);
+                const int t0 = w & 255, t1 = (w >> 8) & 255, p0 = w >> 16;
+                if (t0 >= t1) continue;
+                while (lo < t0) retire();
+                const int c = r < kChunks ? r : NR - 1 - r;
+                const int base = 16 * (c & 1) + 4 * (c >> 1);
+                u32x4 bh = {PH[base], PH[base + 1], PH[base + 2], PH[base + 3]};
+                u32x4 bl = {PL[base], PL[base + 1], PL[base + 2], PL[base + 3]};
+                if (R2 && r >= kChunks) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        bh[e] = swap8u(bh[e]);
+                        bl[e] = swap8u(bl[e]);
+                    }
+                }
+                acc[0] = mfma3_bf16(wf[r & 1][0][0], wf[r & 1][0][1], bh, bl, acc[0]);
+                if (t1 - t0 > 1) acc[1] = mfma3_bf16(wf[r & 1][1][0], wf[r & 1][1][1], bh, bl, acc[1]);
+#pragma unroll
+                for (int j = 2; j < MW; j++)
+                    if (t1 - t0 > j) {
+                        const int q = p0 + j;
+                        acc[j] = mfma3_bf16(melL[(q * 2) * 64], melL[(q * 2 + 1) * 64], bh, bl, acc[j]);
+                    }
             }
+            while (lo < nMT) retire();
         } else {
-        float cmax = -3.40282347e38f;
-#pragma unroll
-        for (int mt = 0; mt < NMT; mt++) {
-            const int m0 = 16 * mt + 4 * g;
-            float val[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float pw = mel[mt][i] * melScale;
-                if (a.clipMax) {
-                    val[i] = pw;
-                    if (f < v.F && m0 + i < a.M) cmax = fmaxf(cmax, pw);
-                } else if (a.minRatio >= 1.17549435e-38f) {
-                    val[i] = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, pw * a.invRef));
-                } else {
-                    const float r = pw * a.invRef;
-                    val[i] = a.mult * log2f(a.minRatio < r ? r : a.minRatio);
+            // ---- banded mel on MFMA, pairs in their table order (chunk by chunk, tiles ascending): chunk
+            //      c's operand is PH/PL[16 (c % 2) + 4 (c / 2) + 0..3]; each pair's weight fragments were
+            //      requested two pairs ahead
+            //      (nfft 1024: each chunk's second run is the upper half's, on the power of the lane's
+            //      partner l ^ 8 -- the frame's bins 512 - u -- so the frame's parity-0 column gathers the
+            //      whole filter bank; then bin 256's weights)
+            f32x4 mel[NMT];
+    #pragma unroll
+            for (int mt = 0; mt < NMT; mt++) mel[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const u32x4 *melL = melF + (opaque(static_cast<int>(threadIdx.x)) & 63);
+            u32x4 wch = melL[0], wcl = melL[64];
+            u32x4 wnh = melL[(min(1, nP - 1) * 2) * 64], wnl = melL[(min(1, nP - 1) * 2 + 1) * 64];
+            int pn = 2;
+            auto run = [&](int t0, int t1, const u32x4 &bh, const u32x4 &bl) {
+    #pragma unroll
+                for (int mt = 0; mt < NMT; mt++) {
+                    if (mt >= t0 && mt < t1) {
+                        mel[mt] = mfma3_bf16(wch, wcl, bh, bl, mel[mt]);
+                        wch = wnh;
+                        wcl = wnl;
+                        const int q = min(pn, nP - 1);
+                        wnh = melL[(q * 2) * 64];
+                        wnl = melL[(q * 2 + 1) * 64];
+                        pn++;
+                    }
+                }
+            };
+    #pragma unroll
+            for (int ch = 0; ch < kChunks; ch++) {
+                const int cw = __builtin_amdgcn_readfirstlane(chunkW[ch]);   // read per item from LDS
+                const int base = 16 * (ch & 1) + 4 * (ch >> 1);
+                const u32x4 bh = {PH[base], PH[base + 1], PH[base + 2], PH[base + 3]};
+                const u32x4 bl = {PL[base], PL[base + 1], PL[base + 2], PL[base + 3]};
+                run(cw & 255, (cw >> 8) & 255, bh, bl);
+                if (R2) {
+                    const int cw2 = __builtin_amdgcn_readfirstlane(chunkW[16 + ch]);
+                    const int t0 = cw2 & 255, t1 = (cw2 >> 8) & 255;
+                    if (t0 < t1) {   // (no partner swap for a chunk without pairs)
+                        u32x4 sh, sl;
+    #pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            sh[e] = swap8u(bh[e]);
+                            sl[e] = swap8u(bl[e]);
+                        }
+                        run(t0, t1, sh, sl);
+                    }
                 }
             }
-            if (f < v.F) {
-                if (a.ntf) {
-                    float *o = v.out + static_cast<size_t>(f) * v.hs + m0;
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if (m0 + i < a.M) o[i] = val[i];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if (m0 + i < a.M) v.out[static_cast<size_t>(m0 + i) * v.hs + f] = val[i];
+            if (R2) {
+                const int g = (opaque(static_cast<int>(threadIdx.x)) >> 4) & 3;
+    #pragma unroll
+                for (int mt = 0; mt < NMT; mt++) {
+                    const float4 w = *reinterpret_cast<const float4 *>(w256 + 16 * mt + 4 * g);
+                    mel[mt][0] = fmaf(w.x, p256, mel[mt][0]);
+                    mel[mt][1] = fmaf(w.y, p256, mel[mt][1]);
+                    mel[mt][2] = fmaf(w.z, p256, mel[mt][2]);
+                    mel[mt][3] = fmaf(w.w, p256, mel[mt][3]);
                 }
             }
-        }
-        if (a.clipMax) {
-            cmax = wave_max(cmax);
-            if (lane == 0) atomicMax(&a.clipMax[v.b], __float_as_int(cmax));
-        }
+
+
+            // ---- dB (or the power, for the clip-maximum reference) and the stores
+            // the planes hold power of operands scaled by scale * 2^12: undo once per output
+            const float unscale = ldexpf(1.0f, -12 - ilogbf(scale));   // scale is a power of two
+            const float melScale = (GENERAL && a.power == 1) ? unscale : unscale * unscale;
+            const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
+            const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
+            const int f = f0 + fr;
+            if (!GENERAL) {
+                if (f < v.F && !(R2 && (lane & 8))) {
+                    // row 16 mt + 4 g + i: a scalar base per (mt, i) plus the lane's offset 4 g hs + f
+                    const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
+                    const float k = melScale * a.invRef;
+                    if (a.M == 16 * NMT && v.hs < (1u << 26)) {
+                        // every row exists and the lane's byte offset fits 32 bits: one scalar base per row
+                        const uint32_t laneB = 4u * laneOff;
+    #pragma unroll
+                        for (int mt = 0; mt < NMT; mt++) {
+    #pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
+                                char *rowp = reinterpret_cast<char *>(v.out + static_cast<size_t>(16 * mt + i) * v.hs);
+                                *reinterpret_cast<float *>(rowp + laneB) = val;
+                            }
+                        }
+                    } else {
+    #pragma unroll
+                        for (int mt = 0; mt < NMT; mt++) {
+    #pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
+                                float *rowp = v.out + static_cast<size_t>(16 * mt + i) * v.hs;
+                                if (16 * mt + 4 * g + i < a.M) rowp[laneOff] = val;
+                            }
+                        }
+                    }
+                }
+            } else {
+            float cmax = -3.40282347e38f;
+    #pragma unroll
+            for (int mt = 0; mt < NMT; mt++) {
+                const int m0 = 16 * mt + 4 * g;
+                float val[4];
+    #pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float pw = mel[mt][i] * melScale;
+                    if (a.clipMax) {
+                        val[i] = pw;
+                        if (f < v.F && m0 + i < a.M) cmax = fmaxf(cmax, pw);
+                    } else if (a.minRatio >= 1.17549435e-38f) {
+                        val[i] = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, pw * a.invRef));
+                    } else {
+                        const float r = pw * a.invRef;
+                        val[i] = a.mult * log2f(a.minRatio < r ? r : a.minRatio);
+                    }
+                }
+                if (f < v.F) {
+                    if (a.ntf) {
+                        float *o = v.out + static_cast<size_t>(f) * v.hs + m0;
+    #pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            if (m0 + i < a.M) o[i] = val[i];
+                    } else {
+    #pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            if (m0 + i < a.M) v.out[static_cast<size_t>(m0 + i) * v.hs + f] = val[i];
+                    }
+                }
+            }
+            if (a.clipMax) {
+                cmax = wave_max(cmax);
+                if (lane == 0) atomicMax(&a.clipMax[v.b], __float_as_int(cmax));
+            }
+            }
         }
 
         cur = nxt;
