@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 # revisions of the fused kernels (the factored-DFT kernel is selected by 16 frames per tile, the dense
 # one runs at 32/64); recorded traffic (profiles/*.md `traffic:` lines) applies only to the revision it
 # was measured on
-FACTORED_REVISION = {512: "fft32x16-xcd", 1024: "fft2x32x16-radix2-xcd"}
+FACTORED_REVISION = {512: "fft32x16-stream", 1024: "fft2x32x16-radix2-stream"}
 DENSE_REVISION = "mirror-il2-selfplan"
 # (16-mel tile, 32-bin chunk) pairs the factored kernel's mel occupies per work item: the 80-mel
 # Slaney bank at nfft 512 / 16 kHz, and the 128-mel bank at nfft 1024 / 44.1 kHz (14 pairs over bins
