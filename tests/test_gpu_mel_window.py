@@ -4,8 +4,9 @@ The factored kernel streams the filter tiles (16 filters each) in ascending freq
 window of accumulators: all tiles when the bank has at most five, else six.  A 32-bin chunk whose
 filters span more tiles than the window cannot be streamed, and the table builder then sends the bank
 to the dense kernel.  128 Slaney filters over nfft 512 put five (48 kHz) and six (192 kHz) tiles on
-the first chunk, the window's largest runs; 80 filters at 48 kHz put three of five.  Checks, on ragged
-clips with edge lengths:
+the first chunk, the window's largest runs; 80 filters at 48 kHz put three of five; 72 and 100 filters
+end in a partial tile, whose missing rows are never stored.  Checks, on ragged clips with edge
+lengths:
 
   * the factored launch (framesPerTile 16) is within 0.05 dB of the FP64 oracle, and cells past each
     clip's frames stay untouched
@@ -25,6 +26,8 @@ CASES = [
     dict(sr=48000.0, nmel=128),
     dict(sr=192000.0, nmel=128),
     dict(sr=48000.0, nmel=80),
+    dict(sr=16000.0, nmel=72),    # the last tile's rows 8-15 do not exist
+    dict(sr=48000.0, nmel=100),   # seven tiles, the last one partial
 ]
 
 
