@@ -516,6 +516,24 @@ __device__ __forceinline__ void max3_abs(float &m, float a, float b)
     __asm__("v_max3_f32 %0, |%1|, |%2|, %0" : "+v"(m) : "v"(a), "v"(b));
 }
 
+// The wave's maximum of non-negative values, uniform: DPP steps within each 16-lane row (lane swaps
+// in quads, half-row and row mirrors), then row broadcasts carry the rows' maxima up to lane 63.  Lanes
+// a step has no source for take 0, which a non-negative maximum ignores.  Vector ALU only: no LDS
+// permute, whose latency each step would wait out.
+__device__ __forceinline__ float wave_max_nonneg(float v)
+{
+#define RPPA_DPP_MAX(CTRL, ROWS) \
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false)))
+    RPPA_DPP_MAX(0xb1, 0xf);    // quad_perm [1, 0, 3, 2]
+    RPPA_DPP_MAX(0x4e, 0xf);    // quad_perm [2, 3, 0, 1]
+    RPPA_DPP_MAX(0x141, 0xf);   // row_half_mirror
+    RPPA_DPP_MAX(0x140, 0xf);   // row_mirror: every lane holds its row's maximum
+    RPPA_DPP_MAX(0x142, 0xa);   // row_bcast:15 into rows 1 and 3
+    RPPA_DPP_MAX(0x143, 0xc);   // row_bcast:31 into rows 2 and 3
+#undef RPPA_DPP_MAX
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ float wave_max(float v)
 {
 #pragma unroll
@@ -867,7 +885,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                     }
                 mx = fmaxf(mx, fmaxf(m1, m2));
             }
-            mx = wave_max(mx) * (1.0f + fabsf(c));
+            mx = wave_max_nonneg(mx) * (1.0f + fabsf(c));
             scale = mx > 0.0f ? ldexpf(1.0f, min(9 - ilogbf(mx), 114)) : 1.0f;
 #pragma unroll
             for (int r = 0; r < ROUNDS; r++) {
@@ -915,7 +933,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                     }
                 }
             }
-            mx = wave_max(mx);
+            mx = wave_max_nonneg(mx);
             // |y * scale| < 2^10 (exponent capped so the scaled domain stays finite for near-silent items)
             scale = mx > 0.0f ? ldexpf(1.0f, min(9 - ilogbf(mx), 114)) : 1.0f;
 #pragma unroll
@@ -933,7 +951,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                 seg[row_at(i % NPH) + i / NPH] = yv;
                 mx = fmaxf(mx, fabsf(yv));
             }
-            mx = wave_max(mx);
+            mx = wave_max_nonneg(mx);
             const float sc = mx > 0.0f ? ldexpf(1.0f, min(9 - ilogbf(mx), 114)) : 1.0f;
             for (int i = lane; i < NPH * PITCH; i += 64) seg[row_at(i % NPH) + i / NPH] *= sc;
             return sc;
