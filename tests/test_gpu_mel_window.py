@@ -76,3 +76,33 @@ def test_mel_window_factored_vs_fp64_and_dense(gpu, orc, ci):
     print(f"  factored vs FP64 {err:.3e} dB, vs dense {d:.3e} dB")
     assert err <= DB_TOL and d <= DB_TOL
     assert not same, "the factored launch did not run (its output equals the dense launch's bit for bit)"
+
+
+def test_mel_window_overflow_runs_dense(gpu, orc):
+    """128 filters between 0 and 1 kHz at 16 kHz put all eight tiles on the first 32-bin chunk, more than
+    the window holds: the table builder refuses the bank, so framesPerTile 16 runs the dense kernel too
+    (bit for bit the framesPerTile 32 launch), within 0.05 dB of FP64."""
+    c = dict(sr=16000.0, nmel=128)
+    x, L = clips(7)
+    frames = L // 160 + 1
+    hs = int(frames.max()) + 2
+
+    def launch(tiles):
+        p = gpu.logmel_params(nfft=512, window_length=400, window_step=160, sample_rate=c["sr"], min_freq=0.0,
+                              max_freq=1000.0, num_filter=c["nmel"], formula="slaney", normalize=True)
+        h = gpu.Handle(len(L))
+        gpu.set_tiles(h, *tiles)
+        out = torch.full((len(L), c["nmel"], hs), FILL, device="cuda")
+        gpu.logmel_fused(h, dev(x), out, dev(L), p, "fp16x3")
+        torch.cuda.synchronize()
+        h.close()
+        return out.cpu().numpy()
+
+    a, b_ = launch((16, 1)), launch((32, 1))
+    for b, F in enumerate(frames):
+        assert np.array_equal(a[b, :, :F], b_[b, :, :F]), b
+    ref, _, fr = orc.logmel_f64(x, L, nfft=512, W=400, H=160, nmel=c["nmel"], sample_rate=c["sr"], fmax=1000.0,
+                                nthreads=8)
+    err = max_db_err(a, ref, fr)
+    print(f"128 filters to 1 kHz: dense fallback, {err:.3e} dB vs FP64")
+    assert err <= DB_TOL
