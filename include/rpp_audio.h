@@ -342,7 +342,9 @@ RppStatus rppt_logmel_chain_gpu(RppPtr_t srcPtr, RpptDescPtr srcDescPtr, RppPtr_
 typedef struct {
     Rpp32s framesPerTile;   /* 16 selects the factored-DFT kernel where it applies (fp16x3; nfft 512
                                with hop 160, 16 frames per wave item; nfft 1024 with hop 256, radix 2,
-                               8 frames per wave item, power 2, NFT, referenceMagnitude != 0); the
+                               8 frames per wave item, power 2, NFT, referenceMagnitude != 0; a filter
+                               bank whose 32-bin chunks each meet at most six 16-filter tiles, or any
+                               bank of at most 80 filters); the
                                dense MFMA kernel runs 64 frames (nfft <= 512; at nfft 512, 32 on
                                request) or 32 (above); the FP32 kernel 32; 0 = default (factored
                                where it applies) */
