@@ -1307,14 +1307,14 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             //      partner l ^ 8 -- the frame's bins 512 - u -- so the frame's parity-0 column gathers the
             //      whole filter bank; then bin 256's weights)
             f32x4 mel[NMT];
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < NMT; mt++) mel[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             const u32x4 *melL = melF + (opaque(static_cast<int>(threadIdx.x)) & 63);
             u32x4 wch = melL[0], wcl = melL[64];
             u32x4 wnh = melL[(min(1, nP - 1) * 2) * 64], wnl = melL[(min(1, nP - 1) * 2 + 1) * 64];
             int pn = 2;
             auto run = [&](int t0, int t1, const u32x4 &bh, const u32x4 &bl) {
-    #pragma unroll
+#pragma unroll
                 for (int mt = 0; mt < NMT; mt++) {
                     if (mt >= t0 && mt < t1) {
                         mel[mt] = mfma3_bf16(wch, wcl, bh, bl, mel[mt]);
@@ -1327,7 +1327,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                     }
                 }
             };
-    #pragma unroll
+#pragma unroll
             for (int ch = 0; ch < kChunks; ch++) {
                 const int cw = __builtin_amdgcn_readfirstlane(chunkW[ch]);   // read per item from LDS
                 const int base = 16 * (ch & 1) + 4 * (ch >> 1);
@@ -1339,7 +1339,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                     const int t0 = cw2 & 255, t1 = (cw2 >> 8) & 255;
                     if (t0 < t1) {   // (no partner swap for a chunk without pairs)
                         u32x4 sh, sl;
-    #pragma unroll
+#pragma unroll
                         for (int e = 0; e < 4; e++) {
                             sh[e] = swap8u(bh[e]);
                             sl[e] = swap8u(bl[e]);
@@ -1350,7 +1350,7 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             }
             if (R2) {
                 const int g = (opaque(static_cast<int>(threadIdx.x)) >> 4) & 3;
-    #pragma unroll
+#pragma unroll
                 for (int mt = 0; mt < NMT; mt++) {
                     const float4 w = *reinterpret_cast<const float4 *>(w256 + 16 * mt + 4 * g);
                     mel[mt][0] = fmaf(w.x, p256, mel[mt][0]);
@@ -1368,42 +1368,12 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             const int lane = opaque(static_cast<int>(threadIdx.x)) & 63;
             const int fr = R2 ? lane & 7 : lane & 15, g = lane >> 4;
             const int f = f0 + fr;
-            if (!GENERAL) {
-                if (f < v.F && !(R2 && (lane & 8))) {
-                    // row 16 mt + 4 g + i: a scalar base per (mt, i) plus the lane's offset 4 g hs + f
-                    const uint32_t laneOff = 4u * static_cast<uint32_t>(g) * v.hs + static_cast<uint32_t>(f);
-                    const float k = melScale * a.invRef;
-                    if (a.M == 16 * NMT && v.hs < (1u << 26)) {
-                        // every row exists and the lane's byte offset fits 32 bits: one scalar base per row
-                        const uint32_t laneB = 4u * laneOff;
-    #pragma unroll
-                        for (int mt = 0; mt < NMT; mt++) {
-    #pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
-                                char *rowp = reinterpret_cast<char *>(v.out + static_cast<size_t>(16 * mt + i) * v.hs);
-                                *reinterpret_cast<float *>(rowp + laneB) = val;
-                            }
-                        }
-                    } else {
-    #pragma unroll
-                        for (int mt = 0; mt < NMT; mt++) {
-    #pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                const float val = a.mult * __builtin_amdgcn_logf(fmaxf(a.minRatio, mel[mt][i] * k));
-                                float *rowp = v.out + static_cast<size_t>(16 * mt + i) * v.hs;
-                                if (16 * mt + 4 * g + i < a.M) rowp[laneOff] = val;
-                            }
-                        }
-                    }
-                }
-            } else {
             float cmax = -3.40282347e38f;
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < NMT; mt++) {
                 const int m0 = 16 * mt + 4 * g;
                 float val[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const float pw = mel[mt][i] * melScale;
                     if (a.clipMax) {
@@ -1419,11 +1389,11 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
                 if (f < v.F) {
                     if (a.ntf) {
                         float *o = v.out + static_cast<size_t>(f) * v.hs + m0;
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 4; i++)
                             if (m0 + i < a.M) o[i] = val[i];
                     } else {
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 4; i++)
                             if (m0 + i < a.M) v.out[static_cast<size_t>(m0 + i) * v.hs + f] = val[i];
                     }
@@ -1432,7 +1402,6 @@ __global__ void __launch_bounds__(64 * WAVES, 1) logmel_fft_kernel(FftArgs a)
             if (a.clipMax) {
                 cmax = wave_max(cmax);
                 if (lane == 0) atomicMax(&a.clipMax[v.b], __float_as_int(cmax));
-            }
             }
         }
 
