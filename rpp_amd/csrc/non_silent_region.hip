@@ -13,7 +13,8 @@
 // so mms is bit-identical to that sequence and the detected indices match it exactly.  The
 // reference's HIP kernel uses per-tile prefix sums instead, whose rounding differs from HOST.
 //
-// Parallelism is the reset blocks: one lane runs one block's recurrence.  The recurrence's whole
+// Parallelism is the reset blocks: one lane runs one block's recurrence, or on 16-byte aligned rows
+// four lanes share one block's loads and squares and run its recurrence together.  The recurrence's whole
 // state is the running sum, so pass 1 records it every kSeg outputs together with the largest sum
 // of those outputs; pass 2, one workgroup per clip, reduces the records to the cut-off (unless the
 // reference power fixes it) and resumes the recurrence from the recorded sums over only the two
@@ -431,6 +432,229 @@ __device__ __forceinline__ void mms_run(const NsrArgs &a, const float *x, int b,
     }
 }
 
+// ---- Pass 1 on aligned rows, four lanes per reset block ----------------------------------------
+// Lane q of a quad loads samples 4q .. 4q+3 of each 16-sample chunk of its block, and of the trailing
+// samples W - 1 behind, and squares them; every lane of the quad runs the block's recurrence on the
+// squares taken from the lanes that hold them: each 16-sample chunk's 32 squares are first broadcast
+// across the quad (DPP quad permutes, independent of one another), then the chain runs in plain
+// additions and subtractions, so no permute waits on the chain or the chain on a permute (31 cycles
+// per sample with the permutes folded into the chain's operations, 166 us per detection on workload 1;
+// 151 us this way).  A block's loads run four times as far ahead for the same registers as with one
+// lane per block, and each square is formed once.  The operations and their order are mms_main's.
+constexpr int kQuadRows = 16;                  // reset blocks per wave
+constexpr int kQuadChunks = 24;                // 16-sample chunks of each stream in flight per block
+constexpr int kQuadWaves = 2;                  // waves per workgroup (1: 152.6 us per detection, 2: 151.3 us)
+static_assert((kQuadChunks * kVecChunk) % kSeg == 0 && kSeg % kVecChunk == 0, "checkpoints fall on chunks");
+
+__device__ __forceinline__ float4 sq4(const float4 &v) { return make_float4(v.x * v.x, v.y * v.y, v.z * v.z, v.w * v.w); }
+__device__ __forceinline__ float el(const float4 &v, int e) { return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w)); }
+
+// Broadcast of quad lane K's value to the quad with bound_ctrl set (no read of the old destination).
+template <int K>
+__device__ __forceinline__ float qbb(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), K * 0x55, 0xf, 0xf, true));
+}
+// Two steps of the recurrence with plain operations: s += q0 (a0), s -= r0, s += q1 (a1), s -= r1,
+// m = max(m, a0, a1) -- mms_pair without the products.
+__device__ __forceinline__ void qpair(float &s, float &m, float q0, float q1, float r0, float r1)
+{
+    float a0, a1;
+    __asm__ volatile("v_add_f32 %[a0], %[s], %[q0]\n\t"
+                     "v_sub_f32 %[s], %[a0], %[r0]\n\t"
+                     "v_add_f32 %[a1], %[s], %[q1]\n\t"
+                     "v_sub_f32 %[s], %[a1], %[r1]\n\t"
+                     "v_max3_f32 %[m], %[m], %[a0], %[a1]"
+                     : [s] "+v"(s), [m] "+v"(m), [a0] "=&v"(a0), [a1] "=&v"(a1)
+                     : [q0] "v"(q0), [q1] "v"(q1), [r0] "v"(r0), [r1] "v"(r1));
+}
+
+// One 16-sample chunk of the recurrence: each sample's squares are first broadcast from the lanes that
+// hold them (DPP, off the dependent chain), then the chain runs in plain operations.  q4/rv are this
+// chunk's squares (this lane's group), rn the next trailing chunk's (trailing values past this chunk's
+// end).  MASK: the tail chunk, whose samples from `lim` on are not visited.
+template <int M, bool MASK>
+__device__ __forceinline__ void qchunk(float &s, float &m, const float4 &q4, const float4 &rv, const float4 &rn, int lim)
+{
+    float bq[16], br[16];
+#define RPPA_QBR(U)                                                                                                 \
+    {                                                                                                               \
+        constexpr int t_ = (U) + M;                                                                                 \
+        bq[U] = qbb<((U) >> 2)>(el(q4, (U) & 3));                                                                   \
+        br[U] = qbb<((t_ < 16) ? (t_ >> 2) : 0)>((t_ < 16) ? el(rv, t_ & 3) : el(rn, t_ & 3));                      \
+    }
+    RPPA_QBR(0) RPPA_QBR(1) RPPA_QBR(2) RPPA_QBR(3) RPPA_QBR(4) RPPA_QBR(5) RPPA_QBR(6) RPPA_QBR(7)
+    RPPA_QBR(8) RPPA_QBR(9) RPPA_QBR(10) RPPA_QBR(11) RPPA_QBR(12) RPPA_QBR(13) RPPA_QBR(14) RPPA_QBR(15)
+#undef RPPA_QBR
+    if constexpr (!MASK) {
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) qpair(s, m, bq[i], bq[i + 1], br[i], br[i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            s = s + bq[i];
+            m = (i < lim) ? fmaxf(m, s) : m;
+            s = s - br[i];
+        }
+    }
+}
+
+// The recurrence over outputs [start, end) of one block from `sum` (quad-cooperative; see above):
+// calls mark(off, sum, m) at every checkpoint and keeps the running maximum in m.
+template <int M, class Mark>
+__device__ __forceinline__ void mms_quad(const Row4 &ld4, int q4i, int start, int end, int W, float &sum, float &m,
+                                         Mark &mark)
+{
+    constexpr int NC = kQuadChunks;
+    const int tb0 = start - W + 1 - M;                     // a multiple of 4
+    float4 X[NC], T[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        X[c] = ld4(start + 16 * c + 4 * q4i);
+        T[c] = ld4(tb0 + 16 * c + 4 * q4i);
+    }
+    // trailing squares: positions before the clip's start subtract 0 (the HOST's skipped subtraction)
+    auto rsq = [&](const float4 &v, int pos) {
+        float4 r = sq4(v);
+        if (pos < 0) {
+            r.x = pos + 0 >= 0 ? r.x : 0.0f;
+            r.y = pos + 1 >= 0 ? r.y : 0.0f;
+            r.z = pos + 2 >= 0 ? r.z : 0.0f;
+            r.w = pos + 3 >= 0 ? r.w : 0.0f;
+        }
+        return r;
+    };
+    float4 rv = rsq(T[0], tb0 + 4 * q4i);
+    int p0 = start;
+    for (; p0 + NC * 16 <= end; p0 += NC * 16) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if ((c * 16) % kSeg == 0) mark(p0 + 16 * c - start, sum, m);
+            const float4 q4 = sq4(X[c]);
+            const float4 rn = rsq(T[(c + 1) % NC], tb0 + (p0 - start) + 16 * (c + 1) + 4 * q4i);
+            qchunk<M, false>(sum, m, q4, rv, rn, 16);
+            rv = rn;
+            X[c] = ld4(p0 + 16 * (NC + c) + 4 * q4i);
+            T[c] = ld4(tb0 + (p0 - start) + 16 * (NC + c) + 4 * q4i);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (p0 + 16 * c < end) {
+            if ((c * 16) % kSeg == 0) mark(p0 + 16 * c - start, sum, m);
+            const float4 q4 = sq4(X[c]);
+            const float4 rn = rsq(T[(c + 1) % NC], tb0 + (p0 - start) + 16 * (c + 1) + 4 * q4i);
+            qchunk<M, true>(sum, m, q4, rv, rn, end - (p0 + 16 * c));
+            rv = rn;
+            T[c] = ld4(tb0 + (p0 - start) + 16 * (NC + c) + 4 * q4i);   // the last chunk reads chunk NC's first group
+        }
+    }
+}
+
+// The fresh sum of the W - 1 squares before `start`, in order, quad-cooperative.
+__device__ __forceinline__ float fresh_quad(const Row4 &ld4, int q4i, int start, int W)
+{
+    constexpr int NC = kQuadChunks;
+    float sum = 0.0f;
+    const int f0 = max(start - W + 1, 0);
+    if (f0 >= start) return sum;
+    const int a0 = f0 & ~3;
+    float4 X[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) X[c] = ld4(a0 + 16 * c + 4 * q4i);
+    auto msq = [&](const float4 &v, int pos) {
+        float4 r = sq4(v);
+        r.x = (pos + 0 >= f0 && pos + 0 < start) ? r.x : 0.0f;   // adding +0 keeps the sum
+        r.y = (pos + 1 >= f0 && pos + 1 < start) ? r.y : 0.0f;
+        r.z = (pos + 2 >= f0 && pos + 2 < start) ? r.z : 0.0f;
+        r.w = (pos + 3 >= f0 && pos + 3 < start) ? r.w : 0.0f;
+        return r;
+    };
+    for (int i0 = a0; i0 < start; i0 += NC * 16) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if (i0 + 16 * c < start) {
+                const float4 q4 = msq(X[c], i0 + 16 * c + 4 * q4i);
+                float bq[16];
+#define RPPA_QF(U) bq[U] = qbb<((U) >> 2)>(el(q4, (U) & 3));
+                RPPA_QF(0) RPPA_QF(1) RPPA_QF(2) RPPA_QF(3) RPPA_QF(4) RPPA_QF(5) RPPA_QF(6) RPPA_QF(7)
+                RPPA_QF(8) RPPA_QF(9) RPPA_QF(10) RPPA_QF(11) RPPA_QF(12) RPPA_QF(13) RPPA_QF(14) RPPA_QF(15)
+#pragma unroll
+                for (int u = 0; u < 16; u++) sum = sum + bq[u];
+                X[c] = ld4(i0 + 16 * (NC + c) + 4 * q4i);
+            }
+        }
+    }
+#undef RPPA_QF
+    return sum;
+}
+
+// The blocks are numbered clip by clip (the prefix of each clip's block count, in LDS) and dealt to
+// the quads in that order, so every wave but the last holds 16 blocks: the waves are as few as the
+// blocks allow, one per SIMD.
+__global__ void __launch_bounds__(64 * kQuadWaves) nsr_blocks_quad_kernel(NsrArgs a, int B)
+{
+    extern __shared__ int pre[];   // [B + 1]
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int wv = static_cast<int>(threadIdx.x) >> 6;
+    int carry = 0;
+    if (wv == 0)
+    for (int base = 0; base < B; base += 64) {
+        const int bb = base + lane;
+        const int n = bb < B ? blocks_of(clip_len(a, bb), a.R) : 0;
+        int v = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o);
+            if (lane >= o) v += u;
+        }
+        if (bb < B) pre[bb] = carry + v - n;
+        carry += __shfl(v, 63);
+    }
+    if (threadIdx.x == 0) pre[B] = carry;
+    __syncthreads();
+    const int q4 = lane & 3;
+    const int row = (static_cast<int>(blockIdx.x) * kQuadWaves + wv) * kQuadRows + (lane >> 2);
+    if (row >= pre[B]) return;   // a whole quad returns together
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= row) lo = mid;
+        else hi = mid;
+    }
+    const int b = lo;
+    const int j = row - pre[b];
+    const int L = clip_len(a, b);
+    const int R = a.R <= 0 ? L : a.R;
+    const int start = j * R;
+    const int end = static_cast<int>(min(static_cast<long long>(L), static_cast<long long>(start) + R));
+    const Row4 ld4 = row4(a, b);
+    float sum = fresh_quad(ld4, q4, start, a.W);
+    float *segMax = a.segMax + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+    float *segSum = a.segSum + static_cast<long long>(b) * a.maxSegs + static_cast<long long>(j) * a.segsPerBlock;
+    float mb = -INFINITY, m = -INFINITY;
+    int k = -1;
+    auto mark = [&](int off, float s, float &mm) {
+        if (k >= 0) {
+            if (q4 == 0) segMax[k] = mm;
+            mb = fmaxf(mb, mm);
+        }
+        ++k;
+        mm = -INFINITY;
+        if (q4 == 0) segSum[k] = s;
+    };
+    switch ((start - a.W + 1) & 3) {
+    case 0: mms_quad<0>(ld4, q4, start, end, a.W, sum, m, mark); break;
+    case 1: mms_quad<1>(ld4, q4, start, end, a.W, sum, m, mark); break;
+    case 2: mms_quad<2>(ld4, q4, start, end, a.W, sum, m, mark); break;
+    default: mms_quad<3>(ld4, q4, start, end, a.W, sum, m, mark); break;
+    }
+    if (q4 == 0) {
+        segMax[k] = m;   // start < end: the first chunk marked checkpoint 0
+        a.blockMax[static_cast<long long>(b) * a.maxBlocks + j] = fmaxf(mb, m);
+    }
+}
+
 // Pass 1: one lane per reset block records, every kSeg outputs, the running sum the recurrence
 // resumes from and the largest running sum of those outputs, and the block's largest running sum
 // (mms = sum * (1/W) rounded is monotone in sum, so the largest mms is the largest sum times 1/W,
@@ -641,8 +865,13 @@ RPPA_EXPORT RppStatus rppt_non_silent_region_detection_gpu(RppPtr_t srcPtr, Rppt
     a.segSum = reinterpret_cast<float *>(ws + segCells * 4);
     a.blockMax = reinterpret_cast<float *>(ws + segCells * 8);
 
-    const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Rows - 1) / rppa::kPass1Rows);
-    hipLaunchKernelGGL(rppa::nsr_blocks_kernel, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
+    if (a.vec && B <= 16383) {
+        const unsigned g1 = static_cast<unsigned>((cells + rppa::kQuadRows * rppa::kQuadWaves - 1) / (rppa::kQuadRows * rppa::kQuadWaves));
+        hipLaunchKernelGGL(rppa::nsr_blocks_quad_kernel, dim3(g1), dim3(64 * rppa::kQuadWaves), sizeof(int) * (B + 1), rppHandle->stream, a, B);
+    } else {
+        const unsigned g1 = static_cast<unsigned>((cells + rppa::kPass1Rows - 1) / rppa::kPass1Rows);
+        hipLaunchKernelGGL(rppa::nsr_blocks_kernel, dim3(g1), dim3(rppa::kPass1Threads), 0, rppHandle->stream, a, B);
+    }
     RPPA_HIP(hipGetLastError());
     hipLaunchKernelGGL(rppa::nsr_region_kernel, dim3(B), dim3(rppa::kPass2Threads), 0, rppHandle->stream, a);
     RPPA_HIP(hipGetLastError());
