@@ -570,19 +570,22 @@ __device__ __forceinline__ float fresh_quad(const Row4 &ld4, int q4i, int start,
         r.w = (pos + 3 >= f0 && pos + 3 < start) ? r.w : 0.0f;
         return r;
     };
+    // Whole loop steps, branch-free: the chunks of the last step past `start` add masked zeros, which
+    // leave the sum unchanged (it is never negative), and their loads read ahead harmlessly.  A branch
+    // per chunk, or the scheduler gathering every chunk's squares ahead of the adds, made the loop
+    // wait for all of its loads once a step.
     for (int i0 = a0; i0 < start; i0 += NC * 16) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            if (i0 + 16 * c < start) {
-                const float4 q4 = msq(X[c], i0 + 16 * c + 4 * q4i);
-                float bq[16];
+            const float4 q4 = msq(X[c], i0 + 16 * c + 4 * q4i);
+            float bq[16];
 #define RPPA_QF(U) bq[U] = qbb<((U) >> 2)>(el(q4, (U) & 3));
-                RPPA_QF(0) RPPA_QF(1) RPPA_QF(2) RPPA_QF(3) RPPA_QF(4) RPPA_QF(5) RPPA_QF(6) RPPA_QF(7)
-                RPPA_QF(8) RPPA_QF(9) RPPA_QF(10) RPPA_QF(11) RPPA_QF(12) RPPA_QF(13) RPPA_QF(14) RPPA_QF(15)
+            RPPA_QF(0) RPPA_QF(1) RPPA_QF(2) RPPA_QF(3) RPPA_QF(4) RPPA_QF(5) RPPA_QF(6) RPPA_QF(7)
+            RPPA_QF(8) RPPA_QF(9) RPPA_QF(10) RPPA_QF(11) RPPA_QF(12) RPPA_QF(13) RPPA_QF(14) RPPA_QF(15)
 #pragma unroll
-                for (int u = 0; u < 16; u++) sum = sum + bq[u];
-                X[c] = ld4(i0 + 16 * (NC + c) + 4 * q4i);
-            }
+            for (int u = 0; u < 16; u++) sum = sum + bq[u];
+            X[c] = ld4(i0 + 16 * (NC + c) + 4 * q4i);
+            __builtin_amdgcn_sched_barrier(0);   // chunk by chunk: each waits only for its own load
         }
     }
 #undef RPPA_QF
